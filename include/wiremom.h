@@ -1,0 +1,233 @@
+/*
+ * wiremom.h -- C ABI of libwiremom.so, the MI355X (gfx950) wire-MoM hot path:
+ * impedance-matrix fill, dense complex LU factor/solve, current and Zin.
+ *
+ * Two layers share one implementation:
+ *
+ *  1. Drop-in entry points with the reference program's names and parameter
+ *     lists (zmatrix, zmatrix_gpu, CalculateByFreq, LU_cuBLAS, geoinp,
+ *     config, configwire).  They take and return host arrays exactly as the
+ *     reference's do; complex values are 16-byte {double re, im} pairs
+ *     (wm_c128, layout-identical to the reference's clDoubleComplex).
+ *
+ *  2. The native wm_* API: a device context, a geometry object resident in
+ *     HBM, batched fill / getrf / getrs on device buffers, whole frequency
+ *     sweeps in FP64 or in mixed precision with FP64 iterative refinement,
+ *     and a continuous-batching request server.
+ *
+ * Status codes: every wm_* call returns int.  0 = ok; >0 = LAPACK-style info
+ * (1-based column of the first exactly-zero pivot); <0 = error, see WM_ERR_*
+ * and wm_last_error() for a message (thread-local).
+ *
+ * Matrices are column-major complex (interleaved re, im), element (i, j) of
+ * a batch member b at  A + b*strideA + (i + j*lda)  in complex units.
+ */
+#ifndef WIREMOM_H
+#define WIREMOM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct { double re, im; } wm_c128;
+typedef struct { float re, im; } wm_c64;
+
+/* ---- status ------------------------------------------------------------ */
+#define WM_OK 0
+#define WM_ERR_ARG (-1)       /* bad argument */
+#define WM_ERR_HIP (-2)       /* HIP runtime error */
+#define WM_ERR_NOMEM (-3)     /* device or host allocation failed / over budget */
+#define WM_ERR_SPIN (-4)      /* a bounded device-side wait expired */
+#define WM_ERR_NOCONV (-5)    /* iterative refinement did not converge */
+#define WM_ERR_IO (-6)        /* file could not be read or written */
+#define WM_ERR_GEOMETRY (-7)  /* geometry capacity or topology error */
+#define WM_ERR_STATE (-8)     /* call not valid in the object's state */
+
+/* Precision of the factorization. */
+#define WM_PREC_F64 0         /* complex128 LU, FP64 MFMA trailing update */
+#define WM_PREC_F32 1         /* complex64 LU, FP32 MFMA update, FP64 refinement */
+#define WM_PREC_BF16X3 2      /* complex64 LU, split-BF16x3 MFMA update, FP64 refinement */
+
+const char *wm_last_error(void);
+const char *wm_version(void);
+
+/* ---- context ------------------------------------------------------------ */
+typedef struct wm_ctx wm_ctx;
+
+/* Opens device `device` (HIP ordinal).  One context per device and host
+ * thread group; calls on one context are serialised internally. */
+int wm_init(int device, wm_ctx **out);
+int wm_shutdown(wm_ctx *ctx);
+/* Synchronises the context's stream. */
+int wm_sync(wm_ctx *ctx);
+/* Device name, CU count and HBM bytes. */
+int wm_device_info(wm_ctx *ctx, char *name, int name_len, int *num_cu, size_t *hbm_bytes);
+
+/* Device memory helpers so that callers need no HIP headers. */
+int wm_dev_alloc(wm_ctx *ctx, size_t bytes, void **ptr);
+int wm_dev_free(wm_ctx *ctx, void *ptr);
+int wm_memcpy_h2d(wm_ctx *ctx, void *dst, const void *src, size_t bytes);
+int wm_memcpy_d2h(wm_ctx *ctx, void *dst, const void *src, size_t bytes);
+
+/* ---- geometry ------------------------------------------------------------ */
+typedef struct wm_geom wm_geom;
+
+/* Builds the device geometry from the arrays configwire produces (0-based
+ * positions, 1-based stored ids, two-column arrays with stride nsmax), for
+ * basis functions 0..npls-1.  Free space (ienv = 1). */
+int wm_geom_create(wm_ctx *ctx, const float *x, const float *y, const float *z, const float *rad,
+                   const int *ipc, const int *ips, const int *ise, int nsmax, int npls, wm_geom **out);
+/* Reads a .geo file and runs the reference segmentation (configwire) on it;
+ * writes configwire.txt to `wirefile` unless it is NULL or empty. */
+int wm_geom_load(wm_ctx *ctx, const char *geo_path, const char *wirefile, wm_geom **out);
+int wm_geom_destroy(wm_geom *g);
+/* Number of unknowns N, and the padded leading dimension used on device. */
+int wm_geom_npls(const wm_geom *g);
+int wm_geom_ld(const wm_geom *g);
+
+/* ---- fill ----------------------------------------------------------------- */
+/* Fills nfreq complex128 matrices Z_f (ld = wm_geom_ld, padded rows/columns
+ * set to the identity) on device, f = 0..nfreq-1, Z_f at Z + f*strideZ.
+ * ka[f] is the wavenumber (host array).  If Z32 is not NULL, a complex64
+ * copy is written there as well (same ld, stride strideZ32). */
+int wm_zfill(wm_ctx *ctx, const wm_geom *g, int nfreq, const double *ka,
+             wm_c128 *Z, int64_t strideZ, wm_c64 *Z32, int64_t strideZ32);
+
+/* ---- dense LU (device buffers) ---------------------------------------------- */
+/* Batched LU with partial pivoting (pivot = first max of |re|+|im|).
+ * A: n x n per member (n a multiple of 64), complex128 for WM_PREC_F64 and
+ * complex64 otherwise.  ipiv: device int32, n per member, 1-based.
+ * info (host, batch entries, may be NULL). */
+int wm_getrf_batched(wm_ctx *ctx, int prec, int n, void *A, int lda, int64_t strideA,
+                     int *ipiv, int64_t strideP, int batch, int *info);
+/* Solves A X = B with the factors of wm_getrf_batched; B complex128
+ * (n x nrhs, ldb) per member, overwritten by X.  The factors may be complex64
+ * (prec != WM_PREC_F64); the solve then runs in FP64 on widened values. */
+int wm_getrs_batched(wm_ctx *ctx, int prec, int n, int nrhs, const void *A, int lda, int64_t strideA,
+                     const int *ipiv, int64_t strideP, wm_c128 *B, int ldb, int64_t strideB, int batch);
+
+/* ---- whole solves ------------------------------------------------------------ */
+typedef struct {
+    int prec;              /* WM_PREC_* */
+    int max_iter;          /* refinement cap (mixed precision), default 30 */
+    double tol;            /* stop when ||dx||/||x|| <= tol, default 1e-14 */
+    int batch;             /* frequencies in flight, 0 = auto by HBM budget */
+    double hbm_fraction;   /* share of free HBM the sweep may use, default 0.8 */
+    int fallback_f64;      /* refactor in FP64 if refinement fails, default 1 */
+} wm_solve_opts;
+
+void wm_solve_opts_default(wm_solve_opts *o);
+
+/* Per-frequency statistics of a sweep. */
+typedef struct {
+    int iters;             /* refinement iterations (0 for FP64) */
+    int info;              /* LU info */
+    int fell_back;         /* 1 if the FP64 fallback produced the result */
+    double backward_err;   /* max_r ||b - Z x||_inf / (||Z||_inf ||x||_inf + ||b||_inf) */
+    double dx_rel;         /* last ||dx||_inf / ||x||_inf */
+} wm_freq_stats;
+
+/* Sweep over nfreq frequencies (MHz, float32 like the reference's freq.txt)
+ * with nrhs delta-gap feeds (1-based basis indices gaps[r], source value
+ * ampl*exp(j*pi*phas/180)).  Outputs (host, may be NULL):
+ *   I   : complex128 [nfreq][nrhs][npls]   current coefficients
+ *   zin : complex128 [nfreq][nrhs]          V/I at each feed
+ *   st  : wm_freq_stats [nfreq]
+ * Returns the first nonzero status. */
+int wm_solve_sweep(wm_ctx *ctx, const wm_geom *g, const wm_solve_opts *opts, int nfreq,
+                   const float *freq_mhz, int nrhs, const int *gaps, float ampl, float phas,
+                   wm_c128 *I, wm_c128 *zin, wm_freq_stats *st);
+
+/* ---- serving: continuous batching ------------------------------------------------ */
+typedef struct wm_server wm_server;
+
+typedef struct {
+    size_t hbm_budget;     /* bytes the server may hold for requests, 0 = 80% of free HBM */
+    int max_inflight;      /* frequencies in flight at once, 0 = auto */
+    int prec;              /* WM_PREC_* for every request */
+    int streams;           /* device streams used for overlap, default 2 */
+} wm_server_opts;
+
+void wm_server_opts_default(wm_server_opts *o);
+int wm_server_create(wm_ctx *ctx, const wm_server_opts *o, wm_server **out);
+/* Queues a request: a geometry (the server keeps a reference; do not destroy
+ * it before the request completes), its frequencies and feeds.  Returns a
+ * request id (>= 0) in *id.  The request is admitted once its HBM need fits
+ * the budget; admission never blocks the caller. */
+int wm_server_submit(wm_server *s, const wm_geom *g, int nfreq, const float *freq_mhz, int nrhs,
+                     const int *gaps, float ampl, float phas, int64_t *id);
+/* Drives the device: admits queued requests, launches work, retires
+ * finished frequencies.  Returns the number of requests completed by this
+ * call in *done.  Non-blocking unless `wait` is nonzero (then it returns
+ * after at least one request completed or nothing is pending). */
+int wm_server_step(wm_server *s, int wait, int *done);
+/* Results of a completed request (host arrays sized as in wm_solve_sweep);
+ * latency = completion - submission in seconds.  Releases the request. */
+int wm_server_result(wm_server *s, int64_t id, wm_c128 *I, wm_c128 *zin, wm_freq_stats *st,
+                     double *latency_s);
+/* Queued, admitted and completed-but-unread request counts, HBM in use. */
+int wm_server_status(wm_server *s, int *queued, int *running, int *completed, size_t *hbm_used);
+int wm_server_destroy(wm_server *s);
+
+/* ---- drop-in entry points (reference names and parameter lists) --------------------- */
+
+/* Replaces geoinp, "wersja referencyjna/Configuration.cpp":5-75 and
+ * Configuration.h:10-11.  The reference reads the fixed file name dipol.geo;
+ * this reads WM_GEO_PATH if set, else dipol.geo.  Arrays are 1-based. */
+void geoinp(int nnmax, int nwmax, int ntmax, int *nnod, float *xnod, float *ynod, float *znod,
+            int *nw, int *iwn, float *wrad, int *nsw, int *nt, int *iwt);
+
+/* Replaces config, "wersja referencyjna/Configuration.cpp":78-108 and
+ * Configuration.h:13-15 (npls = nplsw).  Writes configwire.txt. */
+void config(int *npnt, int nnod, float *xnod, float *ynod, float *znod, float *x, float *y, float *z,
+            int nw, int *iwn, float *wrad, int *nsw, int *ise, int *isp, int *ips, int *ipc, int *iwe,
+            float *rad, int *npls, int *nplsw, int *nseg, int nsmax, int nwmax, int ienv);
+
+/* Replaces configwire, "wersja referencyjna/Configuration.cpp":112-520 and
+ * Configuration.h:17-19.  Writes configwire.txt in the working directory. */
+void configwire(int *npnt, int nnod, float *xnod, float *ynod, float *znod, float *x, float *y, float *z,
+                int nw, int *iwn, float *wrad, int *nsw, int *ise, int *isp, int *ips, int *ipc, int *iwe,
+                float *rad, int *nplsw, int *nseg, int nsmax, int nwmax, int ienv);
+
+/* Replaces zmatrix, "wersja referencyjna/Zmatrix.cpp":27-52 and Zmatrix.h:7-9:
+ * fills cz (host, npls x npls, column-major) for wavenumber ka.  The fill
+ * runs on the GPU.  ienv must be 1 (free space); sgn* are accepted and
+ * ignored as they are in free space. */
+void zmatrix(wm_c128 *cz, float *x, float *y, float *z, int *ipc, int *ips, int *ise, float *rad,
+             int npls, double ka, int nsmax, int ienv, float sgnx, float sgny, float sgnz, float sgnenv);
+
+/* Replaces zmatrix_gpu, "wersja referencyjna/Wire_MoM.cu":4052-4197 and
+ * Wire_MoM.h:49-51 (the "wersja gpu/Wire_MoM.h":45-47 form with `rank`
+ * is zmatrix_gpu_rank).  Same contract as zmatrix. */
+void zmatrix_gpu(wm_c128 *cz_h, float *x_h, float *y_h, float *z_h, float *rad_h, int *ipc_h,
+                 int *ips_h, int *ise_h, int nsmax, double ka, int npls, int ienv, float sgnx,
+                 float sgny, float sgnz, float sgnenv);
+void zmatrix_gpu_rank(wm_c128 *cz_h, float *x_h, float *y_h, float *z_h, float *rad_h, int *ipc_h,
+                      int *ips_h, int *ise_h, int nsmax, double ka, int npls, int ienv, float sgnx,
+                      float sgny, float sgnz, float sgnenv, int rank);
+
+/* Replaces LU_cuBLAS, "wersja gpu/Wire_MoM.cu":417-504 and
+ * "wersja gpu/Wire_MoM.h":43: solves cz * X = ci_cu in FP64 on the GPU,
+ * ci_cu (host, n x nrhs) in, solution out; cz (host) is not modified.
+ * cu_cu is unused, as in the reference. batchSize must be 1. */
+void LU_cuBLAS(int n, int nrhs, wm_c128 *ci_cu, wm_c128 *cu_cu, wm_c128 *cz, int batchSize, int rank);
+
+/* Replaces CalculateByFreq, "wersja gpu/Wire_MoM.cu":355-413 and
+ * "wersja gpu/Wire_MoM.h":49-52: one frequency (MHz) end to end -- fill,
+ * optional hard-wired loads (iload == 1, reference behaviour), delta-gap
+ * excitation at basis ngap, FP64 LU solve on the GPU -- and the reference's
+ * "Process - r Freq = f Re Im" line on stdout.  cz receives Z (host),
+ * cu_cu the excitation, ci_cu the currents. */
+void CalculateByFreq(int i, int npls, int nfreq, float freq, float *x, float *y, float *z, float *rad,
+                     int *ipc, int *ips, int *ise, const int nsmax, int ienv, float sgnx, float sgny,
+                     float sgnz, float sgnenv, int iload, float phas, int ngap, float ampl, wm_c128 *cz,
+                     wm_c128 *cu_cu, wm_c128 *ci_cu, int rank);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WIREMOM_H */

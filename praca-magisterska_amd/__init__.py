@@ -1,0 +1,463 @@
+"""Python mirror of libwiremom's C ABI (include/wiremom.h).
+
+The compute path is the HIP library ``libwiremom.so`` built in this directory
+for gfx950; this module only marshals arguments.  There is no CPU fallback:
+if the library is missing, every entry point raises.
+
+Reference interfaces mirrored here (same names, argument meaning and
+units): ``geoinp``/``configwire`` ("wersja referencyjna/Configuration.cpp":5-75,
+112-520), ``zmatrix`` ("wersja referencyjna/Zmatrix.cpp":27-52),
+``zmatrix_gpu`` ("wersja referencyjna/Wire_MoM.cu":4052-4197), ``LU_cuBLAS``
+and ``CalculateByFreq`` ("wersja gpu/Wire_MoM.cu":355-504).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libwiremom.so")
+
+PREC_F64, PREC_F32, PREC_BF16X3 = 0, 1, 2
+PREC_NAMES = {"f64": PREC_F64, "f32": PREC_F32, "bf16x3": PREC_BF16X3}
+# capacities of the reference driver ("wersja referencyjna/Wire_MoM.cu":43-44)
+NNMAX, NTMAX, NWMAX, NPWMAX = 3000, 5000, 10000, 20480
+NSMAX = NWMAX + NPWMAX
+
+EXPORTS = [
+    "wm_last_error", "wm_version", "wm_init", "wm_shutdown", "wm_sync", "wm_device_info", "wm_dev_alloc",
+    "wm_dev_free", "wm_memcpy_h2d", "wm_memcpy_d2h", "wm_geom_create", "wm_geom_load", "wm_geom_destroy",
+    "wm_geom_npls", "wm_geom_ld", "wm_zfill", "wm_getrf_batched", "wm_getrs_batched", "wm_solve_opts_default",
+    "wm_solve_sweep", "wm_server_opts_default", "wm_server_create", "wm_server_submit", "wm_server_step",
+    "wm_server_result", "wm_server_status", "wm_server_destroy", "geoinp", "config", "configwire", "zmatrix",
+    "zmatrix_gpu", "zmatrix_gpu_rank", "LU_cuBLAS", "CalculateByFreq",
+]
+
+
+class WireMoMError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"libwiremom error {code}: {msg}")
+        self.code = code
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [("prec", C.c_int), ("max_iter", C.c_int), ("tol", C.c_double), ("batch", C.c_int),
+                ("hbm_fraction", C.c_double), ("fallback_f64", C.c_int)]
+
+
+class FreqStats(C.Structure):
+    _fields_ = [("iters", C.c_int), ("info", C.c_int), ("fell_back", C.c_int), ("backward_err", C.c_double),
+                ("dx_rel", C.c_double)]
+
+
+class ServerOpts(C.Structure):
+    _fields_ = [("hbm_budget", C.c_size_t), ("max_inflight", C.c_int), ("prec", C.c_int), ("streams", C.c_int)]
+
+
+_lib = None
+
+
+def _p(t):
+    return C.POINTER(t)
+
+
+def lib() -> C.CDLL:
+    """Loads libwiremom.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with python praca-magisterska_amd/build.py")
+        L = C.CDLL(LIB_PATH)
+        vp, ip, fp, dp = C.c_void_p, _p(C.c_int), _p(C.c_float), _p(C.c_double)
+        L.wm_last_error.restype = C.c_char_p
+        L.wm_version.restype = C.c_char_p
+        L.wm_init.argtypes = [C.c_int, _p(vp)]
+        L.wm_shutdown.argtypes = [vp]
+        L.wm_sync.argtypes = [vp]
+        L.wm_device_info.argtypes = [vp, C.c_char_p, C.c_int, ip, _p(C.c_size_t)]
+        L.wm_dev_alloc.argtypes = [vp, C.c_size_t, _p(vp)]
+        L.wm_dev_free.argtypes = [vp, vp]
+        L.wm_memcpy_h2d.argtypes = [vp, vp, vp, C.c_size_t]
+        L.wm_memcpy_d2h.argtypes = [vp, vp, vp, C.c_size_t]
+        L.wm_geom_create.argtypes = [vp, fp, fp, fp, fp, ip, ip, ip, C.c_int, C.c_int, _p(vp)]
+        L.wm_geom_load.argtypes = [vp, C.c_char_p, C.c_char_p, _p(vp)]
+        L.wm_geom_destroy.argtypes = [vp]
+        L.wm_geom_npls.argtypes = [vp]
+        L.wm_geom_ld.argtypes = [vp]
+        L.wm_zfill.argtypes = [vp, vp, C.c_int, dp, vp, C.c_int64, vp, C.c_int64]
+        L.wm_getrf_batched.argtypes = [vp, C.c_int, C.c_int, vp, C.c_int, C.c_int64, vp, C.c_int64, C.c_int, ip]
+        L.wm_getrs_batched.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, C.c_int, C.c_int64, vp, C.c_int64, vp,
+                                       C.c_int, C.c_int64, C.c_int]
+        L.wm_solve_opts_default.argtypes = [_p(SolveOpts)]
+        L.wm_solve_opts_default.restype = None
+        L.wm_solve_sweep.argtypes = [vp, vp, _p(SolveOpts), C.c_int, fp, C.c_int, ip, C.c_float, C.c_float, vp, vp,
+                                     _p(FreqStats)]
+        L.wm_server_opts_default.argtypes = [_p(ServerOpts)]
+        L.wm_server_opts_default.restype = None
+        L.wm_server_create.argtypes = [vp, _p(ServerOpts), _p(vp)]
+        L.wm_server_submit.argtypes = [vp, vp, C.c_int, fp, C.c_int, ip, C.c_float, C.c_float, _p(C.c_int64)]
+        L.wm_server_step.argtypes = [vp, C.c_int, ip]
+        L.wm_server_result.argtypes = [vp, C.c_int64, vp, vp, _p(FreqStats), dp]
+        L.wm_server_status.argtypes = [vp, ip, ip, ip, _p(C.c_size_t)]
+        L.wm_server_destroy.argtypes = [vp]
+        L.geoinp.argtypes = [C.c_int, C.c_int, C.c_int, ip, fp, fp, fp, ip, ip, fp, ip, ip, ip]
+        L.geoinp.restype = None
+        L.configwire.argtypes = [ip, C.c_int, fp, fp, fp, fp, fp, fp, C.c_int, ip, fp, ip, ip, ip, ip, ip, ip, fp,
+                                 ip, ip, C.c_int, C.c_int, C.c_int]
+        L.configwire.restype = None
+        L.zmatrix.argtypes = [vp, fp, fp, fp, ip, ip, ip, fp, C.c_int, C.c_double, C.c_int, C.c_int, C.c_float,
+                              C.c_float, C.c_float, C.c_float]
+        L.zmatrix.restype = None
+        L.zmatrix_gpu.argtypes = [vp, fp, fp, fp, fp, ip, ip, ip, C.c_int, C.c_double, C.c_int, C.c_int, C.c_float,
+                                  C.c_float, C.c_float, C.c_float]
+        L.zmatrix_gpu.restype = None
+        L.LU_cuBLAS.argtypes = [C.c_int, C.c_int, vp, vp, vp, C.c_int, C.c_int]
+        L.LU_cuBLAS.restype = None
+        L.CalculateByFreq.argtypes = [C.c_int, C.c_int, C.c_int, C.c_float, fp, fp, fp, fp, ip, ip, ip, C.c_int,
+                                      C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int, C.c_float,
+                                      C.c_int, C.c_float, vp, vp, vp, C.c_int]
+        L.CalculateByFreq.restype = None
+        _lib = L
+    return _lib
+
+
+def _check(rc: int) -> int:
+    if rc < 0:
+        raise WireMoMError(rc, lib().wm_last_error().decode(errors="replace"))
+    return rc
+
+
+def _ptr(a: np.ndarray, t):
+    return a.ctypes.data_as(_p(t))
+
+
+def wavenumber(freq_mhz: float) -> float:
+    """k for a frequency in MHz stored as float32 ("wersja referencyjna/Wire_MoM.cu":5-6,338-340)."""
+    f = float(np.float32(freq_mhz))
+    ambda = 2.99792448E+8 / f * 1E-6
+    return (np.pi + np.pi) / ambda
+
+
+# ---------------------------------------------------------------------------- front end
+@dataclass
+class Topology:
+    """Arrays after configwire (0-based positions, 1-based ids, stride nsmax)."""
+    nsmax: int
+    npnt: int
+    nseg: int
+    npls: int
+    x: np.ndarray
+    y: np.ndarray
+    z: np.ndarray
+    rad: np.ndarray
+    ise: np.ndarray
+    isp: np.ndarray
+    ips: np.ndarray
+    ipc: np.ndarray
+    iwe: np.ndarray
+
+
+def read_geo(path: str):
+    """Parses a .geo file into 1-based tables (same reading rules as geoinp)."""
+    with open(path, "rb") as f:
+        f.readline()
+        f.readline()
+        tok = f.read().split()
+    it = iter(tok)
+    nnod = int(next(it))
+    xn = np.zeros(nnod + 1, np.float32)
+    yn = np.zeros_like(xn)
+    zn = np.zeros_like(xn)
+    for i in range(1, nnod + 1):
+        xn[i], yn[i], zn[i] = (np.float32(float(next(it))) for _ in range(3))
+    nw = int(next(it))
+    iwn = np.zeros(2 * nw + 1, np.int32)
+    wrad = np.zeros(nw + 1, np.float32)
+    nsw = np.zeros(nw + 1, np.int32)
+    for i in range(1, nw + 1):
+        iwn[i] = int(next(it))
+        iwn[i + nw] = int(next(it))
+        wrad[i] = np.float32(float(next(it)))
+        nsw[i] = int(next(it))
+    return nnod, xn, yn, zn, nw, iwn, wrad, nsw
+
+
+def configwire(geo_path: str, ienv: int = 1, nsmax: int = NSMAX, nwmax: int = NWMAX, cwd: str | None = None) -> Topology:
+    """Runs the library's configwire (reference parameter list) on a .geo file.
+
+    configwire.txt is written to ``cwd`` (default: the current directory)."""
+    nnod, xn, yn, zn, nw, iwn, wrad, nsw = read_geo(geo_path)
+    x = np.zeros(nsmax, np.float32)
+    y = np.zeros_like(x)
+    z = np.zeros_like(x)
+    rad = np.zeros_like(x)
+    ise = np.zeros(2 * nsmax, np.int32)
+    isp = np.zeros_like(ise)
+    ips = np.zeros_like(ise)
+    iwe = np.zeros_like(ise)
+    ipc = np.zeros(nsmax, np.int32)
+    npnt, npls, nseg = C.c_int(), C.c_int(), C.c_int()
+    old = os.getcwd()
+    if cwd:
+        os.chdir(cwd)
+    try:
+        lib().configwire(C.byref(npnt), nnod, _ptr(xn, C.c_float), _ptr(yn, C.c_float), _ptr(zn, C.c_float),
+                         _ptr(x, C.c_float), _ptr(y, C.c_float), _ptr(z, C.c_float), nw, _ptr(iwn, C.c_int),
+                         _ptr(wrad, C.c_float), _ptr(nsw, C.c_int), _ptr(ise, C.c_int), _ptr(isp, C.c_int),
+                         _ptr(ips, C.c_int), _ptr(ipc, C.c_int), _ptr(iwe, C.c_int), _ptr(rad, C.c_float),
+                         C.byref(npls), C.byref(nseg), nsmax, nwmax, ienv)
+    finally:
+        os.chdir(old)
+    return Topology(nsmax, npnt.value, nseg.value, npls.value, x, y, z, rad, ise, isp, ips, ipc, iwe)
+
+
+# ---------------------------------------------------------------------------- device objects
+class Context:
+    """A device context (wm_init)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        _check(lib().wm_init(device, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().wm_shutdown(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def sync(self):
+        _check(lib().wm_sync(self.h))
+
+    def info(self):
+        name = C.create_string_buffer(256)
+        cu = C.c_int()
+        hbm = C.c_size_t()
+        _check(lib().wm_device_info(self.h, name, 256, C.byref(cu), C.byref(hbm)))
+        return name.value.decode(), cu.value, hbm.value
+
+
+class DeviceArray:
+    """Device buffer holding a copy of a numpy array (for the batched LU entry points)."""
+
+    def __init__(self, ctx: Context, arr: np.ndarray):
+        self.ctx = ctx
+        self.shape, self.dtype = arr.shape, arr.dtype
+        self.nbytes = arr.nbytes
+        p = C.c_void_p()
+        _check(lib().wm_dev_alloc(ctx.h, self.nbytes, C.byref(p)))
+        self.ptr = p
+        self.upload(arr)
+
+    def upload(self, arr: np.ndarray):
+        a = np.ascontiguousarray(arr)
+        _check(lib().wm_memcpy_h2d(self.ctx.h, self.ptr, a.ctypes.data, a.nbytes))
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        _check(lib().wm_memcpy_d2h(self.ctx.h, out.ctypes.data, self.ptr, self.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().wm_dev_free(self.ctx.h, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Geometry:
+    """Device-resident geometry (wm_geom_create / wm_geom_load)."""
+
+    def __init__(self, ctx: Context, handle):
+        self.ctx, self.h = ctx, handle
+        self.npls = lib().wm_geom_npls(handle)
+        self.ld = lib().wm_geom_ld(handle)
+
+    @classmethod
+    def from_topology(cls, ctx: Context, t: Topology) -> "Geometry":
+        h = C.c_void_p()
+        _check(lib().wm_geom_create(ctx.h, _ptr(t.x, C.c_float), _ptr(t.y, C.c_float), _ptr(t.z, C.c_float),
+                                    _ptr(t.rad, C.c_float), _ptr(t.ipc, C.c_int), _ptr(t.ips, C.c_int),
+                                    _ptr(t.ise, C.c_int), t.nsmax, t.npls, C.byref(h)))
+        return cls(ctx, h)
+
+    @classmethod
+    def load(cls, ctx: Context, geo_path: str, wirefile: str | None = None) -> "Geometry":
+        h = C.c_void_p()
+        _check(lib().wm_geom_load(ctx.h, geo_path.encode(), (wirefile or "").encode(), C.byref(h)))
+        return cls(ctx, h)
+
+    def close(self):
+        if self.h:
+            lib().wm_geom_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def fill(self, freqs_mhz, f32_copy: bool = False):
+        """Z matrices (complex128, N x N each) for the given frequencies, filled on the GPU."""
+        ka = np.array([wavenumber(f) for f in np.atleast_1d(freqs_mhz)], np.float64)
+        nf, ld, n = len(ka), self.ld, self.npls
+        Zd = DeviceArray(self.ctx, np.zeros((nf, ld, ld), np.complex128))
+        Z32 = DeviceArray(self.ctx, np.zeros((nf, ld, ld), np.complex64)) if f32_copy else None
+        _check(lib().wm_zfill(self.ctx.h, self.h, nf, _ptr(ka, C.c_double), Zd.ptr, ld * ld,
+                              Z32.ptr if Z32 else None, ld * ld))
+        # device layout is column-major: element (i, j) at j*ld + i
+        Z = Zd.download().transpose(0, 2, 1)
+        out = [Z[:, :n, :n]]
+        if Z32:
+            out.append(Z32.download().transpose(0, 2, 1)[:, :n, :n])
+        return out if f32_copy else out[0]
+
+
+def solve_sweep(geom: Geometry, freqs_mhz, gaps, prec="f64", ampl: float = 1.0, phas: float = 0.0,
+                batch: int = 0, max_iter: int = 30, tol: float = 1e-14, fallback: bool = True):
+    """Whole solves (wm_solve_sweep).  Returns (zin[nf, nrhs], I[nf, nrhs, N], stats list)."""
+    f = np.ascontiguousarray(np.atleast_1d(freqs_mhz), np.float32)
+    g = np.ascontiguousarray(np.atleast_1d(gaps), np.int32)
+    nf, nr, n = len(f), len(g), geom.npls
+    o = SolveOpts()
+    lib().wm_solve_opts_default(C.byref(o))
+    o.prec = PREC_NAMES[prec] if isinstance(prec, str) else int(prec)
+    o.batch, o.max_iter, o.tol, o.fallback_f64 = batch, max_iter, tol, int(fallback)
+    I = np.zeros((nf, nr, n), np.complex128)
+    zin = np.zeros((nf, nr), np.complex128)
+    st = (FreqStats * nf)()
+    rc = lib().wm_solve_sweep(geom.ctx.h, geom.h, C.byref(o), nf, _ptr(f, C.c_float), nr, _ptr(g, C.c_int),
+                              float(ampl), float(phas), I.ctypes.data, zin.ctypes.data, st)
+    _check(rc)
+    stats = [dict(iters=s.iters, info=s.info, fell_back=s.fell_back, dx_rel=s.dx_rel) for s in st]
+    return zin, I, stats
+
+
+def getrf_batched(ctx: Context, A: np.ndarray, prec="f64"):
+    """LU of a stack of n x n matrices (n a multiple of 64) on the GPU.
+
+    Returns (LU stack as numpy, ipiv 1-based, info list); complex64 for mixed precisions."""
+    p = PREC_NAMES[prec] if isinstance(prec, str) else int(prec)
+    dt = np.complex128 if p == PREC_F64 else np.complex64
+    A = np.asarray(A)
+    if A.ndim == 2:
+        A = A[None]
+    b, n, _ = A.shape
+    dev = DeviceArray(ctx, np.ascontiguousarray(A.transpose(0, 2, 1).astype(dt)))  # column-major
+    piv = DeviceArray(ctx, np.zeros((b, n), np.int32))
+    info = np.zeros(b, np.int32)
+    rc = lib().wm_getrf_batched(ctx.h, p, n, dev.ptr, n, n * n, piv.ptr, n, b, _ptr(info, C.c_int))
+    _check(rc)
+    return dev, piv, info
+
+
+def getrs_batched(ctx: Context, dev: DeviceArray, piv: DeviceArray, B: np.ndarray, prec="f64") -> np.ndarray:
+    p = PREC_NAMES[prec] if isinstance(prec, str) else int(prec)
+    b, n = piv.shape
+    B = np.asarray(B, np.complex128)
+    if B.ndim == 2:
+        B = B[None]
+    nr = B.shape[2]
+    Bd = DeviceArray(ctx, np.ascontiguousarray(B.transpose(0, 2, 1)))
+    _check(lib().wm_getrs_batched(ctx.h, p, n, nr, dev.ptr, n, n * n, piv.ptr, n, Bd.ptr, n, n * nr, b))
+    return Bd.download().transpose(0, 2, 1)
+
+
+# ---------------------------------------------------------------------------- drop-in wrappers
+def zmatrix(t: Topology, ka: float) -> np.ndarray:
+    """zmatrix with the reference's arguments; returns Z (npls x npls, numpy row/col indexing)."""
+    n = t.npls
+    cz = np.zeros((n, n), np.complex128)  # filled column-major: cz.T is Z
+    lib().zmatrix(cz.ctypes.data, _ptr(t.x, C.c_float), _ptr(t.y, C.c_float), _ptr(t.z, C.c_float),
+                  _ptr(t.ipc, C.c_int), _ptr(t.ips, C.c_int), _ptr(t.ise, C.c_int), _ptr(t.rad, C.c_float), n,
+                  float(ka), t.nsmax, 1, 1.0, 1.0, 1.0, 1.0)
+    return cz.T.copy()
+
+
+def LU_cuBLAS(Z: np.ndarray, b: np.ndarray) -> np.ndarray:
+    """Solves Z x = b with the drop-in LU_cuBLAS (FP64 on the GPU)."""
+    n = Z.shape[0]
+    cz = np.ascontiguousarray(np.asarray(Z, np.complex128).T)  # column-major
+    B = np.asarray(b, np.complex128)
+    nr = 1 if B.ndim == 1 else B.shape[1]
+    ci = np.ascontiguousarray(B.reshape(n, nr).T)
+    cu = np.zeros_like(ci)
+    lib().LU_cuBLAS(n, nr, ci.ctypes.data, cu.ctypes.data, cz.ctypes.data, 1, 0)
+    x = ci.T
+    return x[:, 0] if B.ndim == 1 else x
+
+
+def CalculateByFreq(t: Topology, freq_mhz: float, ngap: int, ampl: float = 1.0, phas: float = 0.0,
+                    iload: int = 2, rank: int = 0):
+    """One frequency through the drop-in CalculateByFreq; returns (Zin, I, Z)."""
+    n = t.npls
+    cz = np.zeros((n, n), np.complex128)
+    cu = np.zeros(n, np.complex128)
+    ci = np.zeros(n, np.complex128)
+    lib().CalculateByFreq(0, n, 1, float(freq_mhz), _ptr(t.x, C.c_float), _ptr(t.y, C.c_float),
+                          _ptr(t.z, C.c_float), _ptr(t.rad, C.c_float), _ptr(t.ipc, C.c_int), _ptr(t.ips, C.c_int),
+                          _ptr(t.ise, C.c_int), t.nsmax, 1, 1.0, 1.0, 1.0, 1.0, iload, float(phas), int(ngap),
+                          float(ampl), cz.ctypes.data, cu.ctypes.data, ci.ctypes.data, rank)
+    return cu[ngap - 1] / ci[ngap - 1], ci, cz.T.copy()
+
+
+class Server:
+    """Continuous-batching request server (wm_server_*)."""
+
+    def __init__(self, ctx: Context, prec="f32", hbm_budget: int = 0, max_inflight: int = 0):
+        o = ServerOpts()
+        lib().wm_server_opts_default(C.byref(o))
+        o.prec = PREC_NAMES[prec] if isinstance(prec, str) else int(prec)
+        o.hbm_budget, o.max_inflight = hbm_budget, max_inflight
+        h = C.c_void_p()
+        _check(lib().wm_server_create(ctx.h, C.byref(o), C.byref(h)))
+        self.h, self.ctx = h, ctx
+        self.meta = {}
+
+    def submit(self, geom: Geometry, freqs_mhz, gaps, ampl: float = 1.0, phas: float = 0.0) -> int:
+        f = np.ascontiguousarray(np.atleast_1d(freqs_mhz), np.float32)
+        g = np.ascontiguousarray(np.atleast_1d(gaps), np.int32)
+        rid = C.c_int64()
+        _check(lib().wm_server_submit(self.h, geom.h, len(f), _ptr(f, C.c_float), len(g), _ptr(g, C.c_int),
+                                      float(ampl), float(phas), C.byref(rid)))
+        self.meta[rid.value] = (len(f), len(g), geom.npls)
+        return rid.value
+
+    def step(self) -> int:
+        d = C.c_int()
+        _check(lib().wm_server_step(self.h, 0, C.byref(d)))
+        return d.value
+
+    def status(self):
+        q, r, c = C.c_int(), C.c_int(), C.c_int()
+        used = C.c_size_t()
+        _check(lib().wm_server_status(self.h, C.byref(q), C.byref(r), C.byref(c), C.byref(used)))
+        return q.value, r.value, c.value, used.value
+
+    def result(self, rid: int):
+        nf, nr, n = self.meta.pop(rid)
+        I = np.zeros((nf, nr, n), np.complex128)
+        zin = np.zeros((nf, nr), np.complex128)
+        st = (FreqStats * nf)()
+        lat = C.c_double()
+        _check(lib().wm_server_result(self.h, rid, I.ctypes.data, zin.ctypes.data, st, C.byref(lat)))
+        return zin, I, [dict(iters=s.iters, info=s.info, fell_back=s.fell_back, dx_rel=s.dx_rel) for s in st], lat.value
+
+    def close(self):
+        if self.h:
+            lib().wm_server_destroy(self.h)
+            self.h = None

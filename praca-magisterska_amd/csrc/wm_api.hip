@@ -1,0 +1,675 @@
+// C ABI of libwiremom: context, geometry, fill, LU, sweeps and the drop-in
+// entry points with the reference program's names.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "wm_front.h"
+#include "wm_internal.h"
+
+namespace wm {
+
+static thread_local char g_err[1024] = "";
+
+void set_error(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+}
+
+int DevBuf::ensure(size_t need)
+{
+    if (bytes >= need && ptr) return WM_OK;
+    if (ptr) {
+        (void)hipDeviceSynchronize();
+        (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+    const size_t sz = std::max(need, (size_t)4096);
+    if (hipMalloc(&ptr, sz) != hipSuccess) {
+        ptr = nullptr;
+        set_error("device allocation of %zu bytes failed", sz);
+        return WM_ERR_NOMEM;
+    }
+    bytes = sz;
+    return WM_OK;
+}
+
+void DevBuf::release()
+{
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+}
+
+int check_device_error(wm_ctx *ctx, hipStream_t s)
+{
+    int h = 0;
+    WM_HIP_TRY(hipMemcpyAsync(&h, ctx->d_err, sizeof(int), hipMemcpyDeviceToHost, s));
+    WM_HIP_TRY(hipStreamSynchronize(s));
+    if (h != 0) {
+        const int zero = 0;
+        (void)hipMemcpy(ctx->d_err, &zero, sizeof(int), hipMemcpyHostToDevice);
+        set_error("a bounded device-side wait expired (code %d)", h);
+        return WM_ERR_SPIN;
+    }
+    return WM_OK;
+}
+
+}  // namespace wm
+
+using namespace wm;
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kCspeed = 2.99792448E+8;  // "wersja referencyjna/Wire_MoM.cu":5
+
+double wavenumber(float freq_mhz)  // Wire_MoM.cu:338-340 (frequency in MHz, float32)
+{
+    const double ambda = kCspeed / (double)freq_mhz * 1E-6;
+    return (kPi + kPi) / ambda;
+}
+
+int round_up(int v, int m) { return (v + m - 1) / m * m; }
+
+// Smith (1962) complex division.
+wm_c128 cdiv(wm_c128 a, wm_c128 b)
+{
+    wm_c128 q;
+    if (std::fabs(b.re) >= std::fabs(b.im)) {
+        const double t = b.im / b.re, d = b.re + b.im * t;
+        q.re = (a.re + a.im * t) / d;
+        q.im = (a.im - a.re * t) / d;
+    } else {
+        const double t = b.re / b.im, d = b.im + b.re * t;
+        q.re = (a.re * t + a.im) / d;
+        q.im = (a.im * t - a.re) / d;
+    }
+    return q;
+}
+
+__global__ void k_pad_identity(double2 *A, int lda, int64_t stride, int n0, int n)
+{
+    // sets rows/columns n0..n-1 of an n x n matrix to the identity
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.y;
+    const int64_t total = (int64_t)n * n;
+    if (idx >= total) return;
+    const int i = (int)(idx % n), j = (int)(idx / n);
+    if (i < n0 && j < n0) return;
+    A[(size_t)b * stride + i + (size_t)j * lda] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+}
+
+__global__ void k_zero_c128(double2 *p, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = make_double2(0.0, 0.0);
+}
+
+std::mutex g_default_mu;
+wm_ctx *g_default_ctx = nullptr;
+
+wm_ctx *default_ctx()
+{
+    std::lock_guard<std::mutex> lk(g_default_mu);
+    if (!g_default_ctx) {
+        int dev = 0;
+        if (const char *e = std::getenv("WM_DEVICE")) dev = std::atoi(e);
+        if (wm_init(dev, &g_default_ctx) != WM_OK) {
+            std::fprintf(stderr, "libwiremom: %s\n", wm_last_error());
+            g_default_ctx = nullptr;
+        }
+    }
+    return g_default_ctx;
+}
+
+struct DevMem {
+    void *p = nullptr;
+    ~DevMem()
+    {
+        if (p) (void)hipFree(p);
+    }
+    int alloc(size_t bytes)
+    {
+        if (hipMalloc(&p, bytes) != hipSuccess) {
+            p = nullptr;
+            set_error("device allocation of %zu bytes failed", bytes);
+            return WM_ERR_NOMEM;
+        }
+        return WM_OK;
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+const char *wm_last_error(void) { return g_err; }
+const char *wm_version(void) { return "libwiremom 0.1 (gfx950)"; }
+
+int wm_init(int device, wm_ctx **out)
+{
+    if (!out) { set_error("wm_init: null out"); return WM_ERR_ARG; }
+    int ndev = 0;
+    WM_HIP_TRY(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) {
+        set_error("wm_init: device %d not present (%d devices)", device, ndev);
+        return WM_ERR_ARG;
+    }
+    WM_HIP_TRY(hipSetDevice(device));
+    auto *c = new wm_ctx();
+    c->device = device;
+    hipDeviceProp_t p;
+    WM_HIP_TRY(hipGetDeviceProperties(&p, device));
+    c->num_cu = p.multiProcessorCount;
+    c->hbm_bytes = p.totalGlobalMem;
+    c->max_lds = p.sharedMemPerBlock;
+    std::snprintf(c->name, sizeof c->name, "%s (%s)", p.name, p.gcnArchName);
+    WM_HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    WM_HIP_TRY(hipMalloc(&c->d_sync, sizeof(int) * wm_ctx::kSyncCap));
+    WM_HIP_TRY(hipMalloc(&c->d_err, sizeof(int)));
+    WM_HIP_TRY(hipMemset(c->d_sync, 0, sizeof(int) * wm_ctx::kSyncCap));
+    WM_HIP_TRY(hipMemset(c->d_err, 0, sizeof(int)));
+    *out = c;
+    return WM_OK;
+}
+
+int wm_shutdown(wm_ctx *c)
+{
+    if (!c) return WM_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    c->ws_fill.release();
+    c->ws_lu.release();
+    c->ws_solve.release();
+    c->ws_resid.release();
+    c->ws_sweep.release();
+    (void)hipFree(c->d_sync);
+    (void)hipFree(c->d_err);
+    (void)hipStreamDestroy(c->stream);
+    if (c == g_default_ctx) g_default_ctx = nullptr;
+    delete c;
+    return WM_OK;
+}
+
+int wm_sync(wm_ctx *c)
+{
+    if (!c) return WM_ERR_ARG;
+    WM_HIP_TRY(hipStreamSynchronize(c->stream));
+    return check_device_error(c, c->stream);
+}
+
+int wm_device_info(wm_ctx *c, char *name, int name_len, int *num_cu, size_t *hbm)
+{
+    if (!c) return WM_ERR_ARG;
+    if (name && name_len > 0) std::snprintf(name, name_len, "%s", c->name);
+    if (num_cu) *num_cu = c->num_cu;
+    if (hbm) *hbm = c->hbm_bytes;
+    return WM_OK;
+}
+
+int wm_dev_alloc(wm_ctx *c, size_t bytes, void **ptr)
+{
+    if (!c || !ptr) return WM_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    if (hipMalloc(ptr, bytes) != hipSuccess) {
+        *ptr = nullptr;
+        set_error("wm_dev_alloc: %zu bytes failed", bytes);
+        return WM_ERR_NOMEM;
+    }
+    return WM_OK;
+}
+
+int wm_dev_free(wm_ctx *c, void *ptr)
+{
+    if (!c) return WM_ERR_ARG;
+    WM_HIP_TRY(hipStreamSynchronize(c->stream));
+    if (ptr) WM_HIP_TRY(hipFree(ptr));
+    return WM_OK;
+}
+
+int wm_memcpy_h2d(wm_ctx *c, void *dst, const void *src, size_t bytes)
+{
+    if (!c) return WM_ERR_ARG;
+    WM_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    WM_HIP_TRY(hipStreamSynchronize(c->stream));
+    return WM_OK;
+}
+
+int wm_memcpy_d2h(wm_ctx *c, void *dst, const void *src, size_t bytes)
+{
+    if (!c) return WM_ERR_ARG;
+    WM_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    WM_HIP_TRY(hipStreamSynchronize(c->stream));
+    return WM_OK;
+}
+
+// ---- geometry -------------------------------------------------------------------
+int wm_geom_create(wm_ctx *c, const float *x, const float *y, const float *z, const float *rad, const int *ipc,
+                   const int *ips, const int *ise, int nsmax, int npls, wm_geom **out)
+{
+    if (!c || !x || !y || !z || !rad || !ipc || !ips || !ise || !out || npls <= 0 || nsmax < npls) {
+        set_error("wm_geom_create: bad arguments");
+        return WM_ERR_ARG;
+    }
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    (void)hipSetDevice(c->device);
+    const int ld = round_up(npls, 128);
+    std::vector<double> obs((size_t)12 * ld, 0.0), src((size_t)16 * ld, 0.0);
+    std::vector<int> okey((size_t)2 * ld, -1), skey((size_t)2 * ld, -2);
+    std::map<long long, int> keys;  // unordered node pair -> segment key
+    auto seg_key = [&](int s) {
+        const int a = ise[s - 1], b = ise[s - 1 + nsmax];
+        const long long k = (long long)std::min(a, b) << 32 | (unsigned)std::max(a, b);
+        auto it = keys.find(k);
+        if (it != keys.end()) return it->second;
+        const int id = (int)keys.size();
+        keys.emplace(k, id);
+        return id;
+    };
+    auto node = [&](int id, double r[3]) { r[0] = x[id - 1]; r[1] = y[id - 1]; r[2] = z[id - 1]; };
+    for (int p = 0; p < npls; ++p) {
+        const int mp = std::abs(ipc[p]);
+        const int s1 = ips[p], s2 = ips[p + nsmax];
+        if (mp < 1 || s1 < 1 || s2 < 1 || s1 > nsmax || s2 > nsmax) {
+            set_error("wm_geom_create: basis %d has invalid topology", p);
+            return WM_ERR_GEOMETRY;
+        }
+        double P[3], F1[3], F2[3];
+        node(mp, P);
+        node(ise[s1 - 1], F1);             // first node of segment s1
+        node(ise[s2 - 1 + nsmax], F2);     // second node of segment s2
+        for (int d = 0; d < 3; ++d) {
+            const double rc1 = 0.5 * (P[d] + F1[d]);
+            const double rc2 = 0.5 * (F2[d] + P[d]);
+            obs[(size_t)d * ld + p] = rc1;
+            obs[(size_t)(3 + d) * ld + p] = P[d] - rc1;
+            obs[(size_t)(6 + d) * ld + p] = rc2;
+            obs[(size_t)(9 + d) * ld + p] = rc2 - P[d];
+        }
+        const int k1 = seg_key(s1), k2 = seg_key(s2);
+        okey[p] = k1;
+        okey[ld + p] = k2;
+        skey[2 * p] = k1;
+        skey[2 * p + 1] = k2;
+        // source half 1: first node of s1 -> peak; half 2: peak -> second node of s2
+        double *r = &src[(size_t)p * 16];
+        const double *A[2] = {F1, P}, *B[2] = {P, F2};
+        const int segs[2] = {s1, s2};
+        for (int g = 0; g < 2; ++g) {
+            const double rdx = B[g][0] - A[g][0], rdy = B[g][1] - A[g][1], rdz = B[g][2] - A[g][2];
+            const double dl = std::sqrt(rdx * rdx + rdy * rdy + rdz * rdz);
+            if (!(dl > 0.0) || !(rad[segs[g] - 1] > 0.f)) {
+                set_error("wm_geom_create: basis %d half %d has zero length or radius", p, g + 1);
+                return WM_ERR_GEOMETRY;
+            }
+            r[8 * g + 0] = A[g][0]; r[8 * g + 1] = A[g][1]; r[8 * g + 2] = A[g][2];
+            r[8 * g + 3] = rdx / dl; r[8 * g + 4] = rdy / dl; r[8 * g + 5] = rdz / dl;
+            r[8 * g + 6] = dl;
+            r[8 * g + 7] = (double)rad[segs[g] - 1];
+        }
+    }
+    for (int p = npls; p < ld; ++p) {  // padding: finite dummy data, overwritten by the identity
+        double *r = &src[(size_t)p * 16];
+        for (int g = 0; g < 2; ++g) { r[8 * g + 3] = 1.0; r[8 * g + 6] = 1.0; r[8 * g + 7] = 1.0; }
+        for (int d = 0; d < 12; ++d) obs[(size_t)d * ld + p] = 1.0e3;
+    }
+    auto *g = new wm_geom();
+    g->ctx = c;
+    g->npls = npls;
+    g->ld = ld;
+    g->h_src = src;
+    if (hipMalloc(&g->d_obs, obs.size() * sizeof(double)) != hipSuccess ||
+        hipMalloc(&g->d_src, src.size() * sizeof(double)) != hipSuccess ||
+        hipMalloc(&g->d_obs_key, okey.size() * sizeof(int)) != hipSuccess ||
+        hipMalloc(&g->d_src_key, skey.size() * sizeof(int)) != hipSuccess) {
+        wm_geom_destroy(g);
+        set_error("wm_geom_create: device allocation failed");
+        return WM_ERR_NOMEM;
+    }
+    WM_HIP_TRY(hipMemcpy(g->d_obs, obs.data(), obs.size() * sizeof(double), hipMemcpyHostToDevice));
+    WM_HIP_TRY(hipMemcpy(g->d_src, src.data(), src.size() * sizeof(double), hipMemcpyHostToDevice));
+    WM_HIP_TRY(hipMemcpy(g->d_obs_key, okey.data(), okey.size() * sizeof(int), hipMemcpyHostToDevice));
+    WM_HIP_TRY(hipMemcpy(g->d_src_key, skey.data(), skey.size() * sizeof(int), hipMemcpyHostToDevice));
+    *out = g;
+    return WM_OK;
+}
+
+int wm_geom_load(wm_ctx *c, const char *geo_path, const char *wirefile, wm_geom **out)
+{
+    GeoInput gi;
+    int rc = geo_read(geo_path, gi);
+    if (rc != WM_OK) {
+        set_error("wm_geom_load: cannot read %s", geo_path ? geo_path : "(null)");
+        return rc;
+    }
+    Topology t;
+    std::string text;
+    const bool want = wirefile && wirefile[0];
+    rc = configwire_build(gi, 1, kNsmax, kNwmax, t, want ? &text : nullptr);
+    if (rc != WM_OK) {
+        set_error("wm_geom_load: segmentation of %s failed (capacity or topology)", geo_path);
+        return rc;
+    }
+    if (want) {
+        FILE *f = std::fopen(wirefile, "wb");
+        if (!f) { set_error("cannot write %s", wirefile); return WM_ERR_IO; }
+        std::fwrite(text.data(), 1, text.size(), f);
+        std::fclose(f);
+    }
+    return wm_geom_create(c, t.x.data(), t.y.data(), t.z.data(), t.rad.data(), t.ipc.data(), t.ips.data(),
+                          t.ise.data(), t.nsmax, t.npls, out);
+}
+
+int wm_geom_destroy(wm_geom *g)
+{
+    if (!g) return WM_OK;
+    if (g->d_obs) (void)hipFree(g->d_obs);
+    if (g->d_src) (void)hipFree(g->d_src);
+    if (g->d_obs_key) (void)hipFree(g->d_obs_key);
+    if (g->d_src_key) (void)hipFree(g->d_src_key);
+    delete g;
+    return WM_OK;
+}
+
+int wm_geom_npls(const wm_geom *g) { return g ? g->npls : WM_ERR_ARG; }
+int wm_geom_ld(const wm_geom *g) { return g ? g->ld : WM_ERR_ARG; }
+
+// ---- fill / LU / solve on device buffers -------------------------------------------
+int wm_zfill(wm_ctx *c, const wm_geom *g, int nfreq, const double *ka, wm_c128 *Z, int64_t strideZ, wm_c64 *Z32,
+             int64_t strideZ32)
+{
+    if (!c || !g) return WM_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    (void)hipSetDevice(c->device);
+    return fill_launch(c, g, nfreq, ka, Z, strideZ, Z32, strideZ32, c->stream);
+}
+
+int wm_getrf_batched(wm_ctx *c, int prec, int n, void *A, int lda, int64_t strideA, int *ipiv, int64_t strideP,
+                     int batch, int *info)
+{
+    if (!c) return WM_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    (void)hipSetDevice(c->device);
+    DevMem dinfo;
+    WM_TRY(dinfo.alloc(sizeof(int) * std::max(batch, 1)));
+    WM_TRY(getrf_batched(c, prec, n, A, lda, strideA, ipiv, strideP, batch, (int *)dinfo.p, c->stream));
+    std::vector<int> h(batch);
+    WM_HIP_TRY(hipMemcpyAsync(h.data(), dinfo.p, sizeof(int) * batch, hipMemcpyDeviceToHost, c->stream));
+    WM_TRY(check_device_error(c, c->stream));
+    int first = 0;
+    for (int b = 0; b < batch; ++b) {
+        if (info) info[b] = h[b];
+        if (h[b] && !first) first = h[b];
+    }
+    return first;
+}
+
+int wm_getrs_batched(wm_ctx *c, int prec, int n, int nrhs, const void *A, int lda, int64_t strideA, const int *ipiv,
+                     int64_t strideP, wm_c128 *B, int ldb, int64_t strideB, int batch)
+{
+    if (!c) return WM_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    (void)hipSetDevice(c->device);
+    WM_TRY(getrs_batched(c, prec, n, nrhs, A, lda, strideA, ipiv, strideP, B, ldb, strideB, batch, c->stream));
+    return check_device_error(c, c->stream);
+}
+
+void wm_solve_opts_default(wm_solve_opts *o)
+{
+    if (!o) return;
+    o->prec = WM_PREC_F64;
+    o->max_iter = 30;
+    o->tol = 1e-14;
+    o->batch = 0;
+    o->hbm_fraction = 0.8;
+    o->fallback_f64 = 1;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int wm_solve_sweep(wm_ctx *c, const wm_geom *g, const wm_solve_opts *opts, int nfreq, const float *freq_mhz, int nrhs,
+                   const int *gaps, float ampl, float phas, wm_c128 *I, wm_c128 *zin, wm_freq_stats *st)
+{
+    if (!c || !g || nfreq <= 0 || !freq_mhz || nrhs <= 0 || !gaps) {
+        set_error("wm_solve_sweep: bad arguments");
+        return WM_ERR_ARG;
+    }
+    wm_solve_opts o;
+    wm_solve_opts_default(&o);
+    if (opts) o = *opts;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    (void)hipSetDevice(c->device);
+    const int N = g->npls;
+    std::vector<WorkItem> items(nfreq);
+    for (int f = 0; f < nfreq; ++f) {
+        WorkItem &w = items[f];
+        w.g = g;
+        w.freq = freq_mhz[f];
+        w.nrhs = nrhs;
+        w.gaps = gaps;
+        w.ampl = ampl;
+        w.phas = phas;
+        w.I = I ? I + (size_t)f * nrhs * N : nullptr;
+        w.zin = zin ? zin + (size_t)f * nrhs : nullptr;
+        w.st = st ? st + f : nullptr;
+    }
+    int batch = o.batch > 0 ? o.batch : auto_batch(c, g->ld, nrhs, o.prec, o.hbm_fraction);
+    int status = WM_OK;
+    for (int f0 = 0; f0 < nfreq; f0 += batch) {
+        const int rc = solve_items(c, o, items.data() + f0, std::min(batch, nfreq - f0), c->stream);
+        if (rc < 0) return rc;
+        if (rc > 0 && status == WM_OK) status = rc;
+    }
+    return status;
+}
+
+// ---- drop-in entry points ------------------------------------------------------------------
+
+void geoinp(int nnmax, int nwmax, int ntmax, int *nnod, float *xnod, float *ynod, float *znod, int *nw, int *iwn,
+            float *wrad, int *nsw, int *nt, int *iwt)
+{
+    (void)ntmax;
+    (void)iwt;
+    const char *path = std::getenv("WM_GEO_PATH");
+    if (!path || !path[0]) path = "dipol.geo";
+    std::printf(" Nazwa pliku geo: %s\n", path);
+    GeoInput g;
+    if (geo_read(path, g) != WM_OK) {
+        std::printf(" Nie udalo sie otworzyc pliku!\n");
+        std::exit(0);
+    }
+    if (g.nnod > nnmax || g.nw > nwmax) {
+        std::printf(" Przekroczona liczba wezlow lub drucikow w pliku geo\n");
+        std::exit(0);
+    }
+    *nnod = g.nnod;
+    *nw = g.nw;
+    *nt = 0;
+    for (int i = 1; i <= g.nnod; ++i) { xnod[i] = g.xnod[i]; ynod[i] = g.ynod[i]; znod[i] = g.znod[i]; }
+    for (int i = 1; i <= g.nw; ++i) {
+        iwn[i] = g.iwn[i];
+        iwn[i + g.nw] = g.iwn[i + g.nw];
+        wrad[i] = g.wrad[i];
+        nsw[i] = g.nsw[i];
+    }
+}
+
+void configwire(int *npnt, int nnod, float *xnod, float *ynod, float *znod, float *x, float *y, float *z, int nw,
+                int *iwn, float *wrad, int *nsw, int *ise, int *isp, int *ips, int *ipc, int *iwe, float *rad,
+                int *nplsw, int *nseg, int nsmax, int nwmax, int ienv)
+{
+    GeoInput g;
+    g.nnod = nnod;
+    g.nw = nw;
+    g.xnod.assign(xnod, xnod + nnod + 1);
+    g.ynod.assign(ynod, ynod + nnod + 1);
+    g.znod.assign(znod, znod + nnod + 1);
+    g.iwn.assign(iwn, iwn + 2 * nw + 1);
+    g.wrad.assign(wrad, wrad + nw + 1);
+    g.nsw.assign(nsw, nsw + nw + 1);
+    Topology t;
+    std::string text;
+    const int rc = configwire_build(g, ienv, nsmax, nwmax, t, &text);
+    if (FILE *f = std::fopen("configwire.txt", "wb")) {
+        std::fwrite(text.data(), 1, text.size(), f);
+        std::fclose(f);
+    }
+    if (rc != WM_OK) {
+        std::printf("\n Number of segments or basis functions exceeds the limits\n");
+        std::exit(0);
+    }
+    *npnt = t.npnt;
+    *nseg = t.nseg;
+    *nplsw = t.npls;
+    std::memcpy(x, t.x.data(), sizeof(float) * nsmax);
+    std::memcpy(y, t.y.data(), sizeof(float) * nsmax);
+    std::memcpy(z, t.z.data(), sizeof(float) * nsmax);
+    std::memcpy(rad, t.rad.data(), sizeof(float) * nsmax);
+    std::memcpy(ipc, t.ipc.data(), sizeof(int) * nsmax);
+    std::memcpy(ise, t.ise.data(), sizeof(int) * 2 * nsmax);
+    std::memcpy(isp, t.isp.data(), sizeof(int) * 2 * nsmax);
+    std::memcpy(ips, t.ips.data(), sizeof(int) * 2 * nsmax);
+    std::memcpy(iwe, t.iwe.data(), sizeof(int) * 2 * nsmax);
+}
+
+void config(int *npnt, int nnod, float *xnod, float *ynod, float *znod, float *x, float *y, float *z, int nw, int *iwn,
+            float *wrad, int *nsw, int *ise, int *isp, int *ips, int *ipc, int *iwe, float *rad, int *npls, int *nplsw,
+            int *nseg, int nsmax, int nwmax, int ienv)
+{
+    configwire(npnt, nnod, xnod, ynod, znod, x, y, z, nw, iwn, wrad, nsw, ise, isp, ips, ipc, iwe, rad, nplsw, nseg,
+               nsmax, nwmax, ienv);
+    *npls = *nplsw;
+}
+
+static int fill_to_host(wm_c128 *cz, float *x, float *y, float *z, float *rad, int *ipc, int *ips, int *ise,
+                        int nsmax, double ka, int npls, int ienv)
+{
+    if (ienv != 1) {
+        set_error("only free space (ienv = 1) is available");
+        return WM_ERR_ARG;
+    }
+    wm_ctx *c = default_ctx();
+    if (!c) return WM_ERR_HIP;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    wm_geom *g = nullptr;
+    WM_TRY(wm_geom_create(c, x, y, z, rad, ipc, ips, ise, nsmax, npls, &g));
+    std::unique_ptr<wm_geom, int (*)(wm_geom *)> guard(g, wm_geom_destroy);
+    DevMem Zd;
+    WM_TRY(Zd.alloc((size_t)g->ld * g->ld * sizeof(double2)));
+    WM_TRY(fill_launch(c, g, 1, &ka, Zd.p, 0, nullptr, 0, c->stream));
+    WM_HIP_TRY(hipMemcpy2DAsync(cz, (size_t)npls * sizeof(wm_c128), Zd.p, (size_t)g->ld * sizeof(double2),
+                                (size_t)npls * sizeof(wm_c128), npls, hipMemcpyDeviceToHost, c->stream));
+    WM_HIP_TRY(hipStreamSynchronize(c->stream));
+    return WM_OK;
+}
+
+void zmatrix(wm_c128 *cz, float *x, float *y, float *z, int *ipc, int *ips, int *ise, float *rad, int npls, double ka,
+             int nsmax, int ienv, float sgnx, float sgny, float sgnz, float sgnenv)
+{
+    (void)sgnx; (void)sgny; (void)sgnz; (void)sgnenv;
+    if (fill_to_host(cz, x, y, z, rad, ipc, ips, ise, nsmax, ka, npls, ienv) != WM_OK)
+        std::fprintf(stderr, "zmatrix: %s\n", wm_last_error());
+}
+
+void zmatrix_gpu(wm_c128 *cz_h, float *x_h, float *y_h, float *z_h, float *rad_h, int *ipc_h, int *ips_h, int *ise_h,
+                 int nsmax, double ka, int npls, int ienv, float sgnx, float sgny, float sgnz, float sgnenv)
+{
+    (void)sgnx; (void)sgny; (void)sgnz; (void)sgnenv;
+    if (fill_to_host(cz_h, x_h, y_h, z_h, rad_h, ipc_h, ips_h, ise_h, nsmax, ka, npls, ienv) != WM_OK)
+        std::fprintf(stderr, "zmatrix_gpu: %s\n", wm_last_error());
+}
+
+void zmatrix_gpu_rank(wm_c128 *cz_h, float *x_h, float *y_h, float *z_h, float *rad_h, int *ipc_h, int *ips_h,
+                      int *ise_h, int nsmax, double ka, int npls, int ienv, float sgnx, float sgny, float sgnz,
+                      float sgnenv, int rank)
+{
+    (void)rank;
+    zmatrix_gpu(cz_h, x_h, y_h, z_h, rad_h, ipc_h, ips_h, ise_h, nsmax, ka, npls, ienv, sgnx, sgny, sgnz, sgnenv);
+}
+
+static int lu_solve_host(int n, int nrhs, wm_c128 *ci, const wm_c128 *cz)
+{
+    wm_ctx *c = default_ctx();
+    if (!c) return WM_ERR_HIP;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    const int ld = round_up(n, 128);
+    const hipStream_t s = c->stream;
+    DevMem A, B, P, Inf;
+    WM_TRY(A.alloc((size_t)ld * ld * sizeof(double2)));
+    WM_TRY(B.alloc((size_t)ld * nrhs * sizeof(double2)));
+    WM_TRY(P.alloc((size_t)ld * sizeof(int)));
+    WM_TRY(Inf.alloc(sizeof(int)));
+    WM_HIP_TRY(hipMemcpy2DAsync(A.p, (size_t)ld * sizeof(double2), cz, (size_t)n * sizeof(wm_c128),
+                                (size_t)n * sizeof(wm_c128), n, hipMemcpyHostToDevice, s));
+    const int64_t tot = (int64_t)ld * ld;
+    hipLaunchKernelGGL(k_pad_identity, dim3((unsigned)((tot + 255) / 256), 1), dim3(256), 0, s, (double2 *)A.p, ld,
+                       (int64_t)0, n, ld);
+    hipLaunchKernelGGL(k_zero_c128, dim3((unsigned)((ld * nrhs + 255) / 256)), dim3(256), 0, s, (double2 *)B.p,
+                       (int64_t)ld * nrhs);
+    WM_HIP_TRY(hipMemcpy2DAsync(B.p, (size_t)ld * sizeof(double2), ci, (size_t)n * sizeof(wm_c128),
+                                (size_t)n * sizeof(wm_c128), nrhs, hipMemcpyHostToDevice, s));
+    WM_TRY(getrf_batched(c, WM_PREC_F64, ld, A.p, ld, tot, (int *)P.p, ld, 1, (int *)Inf.p, s));
+    WM_TRY(getrs_batched(c, WM_PREC_F64, ld, nrhs, A.p, ld, tot, (int *)P.p, ld, B.p, ld, (int64_t)ld * nrhs, 1, s));
+    int info = 0;
+    WM_HIP_TRY(hipMemcpyAsync(&info, Inf.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    WM_HIP_TRY(hipMemcpy2DAsync(ci, (size_t)n * sizeof(wm_c128), B.p, (size_t)ld * sizeof(double2),
+                                (size_t)n * sizeof(wm_c128), nrhs, hipMemcpyDeviceToHost, s));
+    WM_TRY(check_device_error(c, s));
+    return info;
+}
+
+void LU_cuBLAS(int n, int nrhs, wm_c128 *ci_cu, wm_c128 *cu_cu, wm_c128 *cz, int batchSize, int rank)
+{
+    (void)cu_cu;
+    (void)rank;
+    if (batchSize != 1) {
+        std::fprintf(stderr, "LU_cuBLAS: batchSize must be 1\n");
+        return;
+    }
+    const int rc = lu_solve_host(n, nrhs, ci_cu, cz);
+    if (rc > 0) std::printf("FACTORIZATION ERROR\n");
+    else if (rc < 0) std::fprintf(stderr, "LU_cuBLAS: %s\n", wm_last_error());
+}
+
+void CalculateByFreq(int i, int npls, int nfreq, float freq, float *x, float *y, float *z, float *rad, int *ipc,
+                     int *ips, int *ise, const int nsmax, int ienv, float sgnx, float sgny, float sgnz, float sgnenv,
+                     int iload, float phas, int ngap, float ampl, wm_c128 *cz, wm_c128 *cu_cu, wm_c128 *ci_cu, int rank)
+{
+    (void)i;
+    (void)nfreq;
+    const double ka = wavenumber(freq);
+    zmatrix_gpu(cz, x, y, z, rad, ipc, ips, ise, nsmax, ka, npls, ienv, sgnx, sgny, sgnz, sgnenv);
+    if (iload == 1) {  // hard-wired loads of the reference ("wersja gpu/Wire_MoM.cu":371-388)
+        auto add = [&](int nl, double re, double im) {
+            if (nl >= 0 && nl < npls) { cz[nl + (size_t)nl * npls].re += re; cz[nl + (size_t)nl * npls].im += im; }
+        };
+        for (int nl = 592 - 1; nl <= 2581 - 1; ++nl) add(nl, 1.e0, 2. * kPi * freq * 4.5);
+        add(2613 - 1, 30., 0.);
+        add(2582 - 1, 1.e12, 0.);
+    }
+    for (int q = 0; q < npls; ++q) cu_cu[q] = wm_c128{0.0, 0.0};
+    const double arg = kPi * phas / 180.;
+    cu_cu[ngap - 1] = wm_c128{(double)ampl * std::cos(arg), (double)ampl * std::sin(arg)};
+    for (int q = 0; q < npls; ++q) ci_cu[q] = cu_cu[q];
+    LU_cuBLAS(npls, 1, ci_cu, cu_cu, cz, 1, rank);
+    const wm_c128 zs = cdiv(cu_cu[ngap - 1], ci_cu[ngap - 1]);
+    std::printf("\n Process - %d Freq = %.4g\t%.4g\t%.4g\n", rank, (double)freq, zs.re, zs.im);
+    std::fflush(stdout);
+}
+
+}  // extern "C"

@@ -1,0 +1,300 @@
+// Batch solver: fill, factor and solve a set of (geometry, frequency, feeds)
+// items of one padded size on one stream.  Used by wm_solve_sweep (one
+// geometry, many frequencies) and by the request server (many requests).
+//
+// Replaces the per-frequency body of the reference's loop, CalculateByFreq
+// ("wersja gpu/Wire_MoM.cu":355-413): wavenumber, fill, delta-gap excitation,
+// LU solve, Zin = V/I at the feed.
+//
+// Precision forms:
+//   WM_PREC_F64    : complex128 fill, complex128 LU and solve.
+//   WM_PREC_F32    : complex128 fill kept for residuals, complex64 copy
+//                    factored on FP32 matrix cores; FP64 iterative
+//                    refinement x += LU32^{-1}(b - Z x) until
+//                    ||dx||/||x|| <= tol or the correction stalls at the
+//                    FP64 floor; items that do not converge are refactored in
+//                    FP64 (fallback).
+// Each item's arithmetic depends only on that item (per-item convergence
+// mask, fixed reduction splits), so an item's result is the same alone or
+// in any batch.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "wm_internal.h"
+
+namespace wm {
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+
+__global__ void k_zero2(double2 *p, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = make_double2(0.0, 0.0);
+}
+
+// Delta-gap right-hand sides: entries (index, value) per (item, rhs).
+__global__ void k_set_rhs(double2 *B, int64_t strideB, int ldb, int nr, const int *idx, const double2 *val, int count)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count * nr) return;
+    const int b = t / nr, r = t % nr;
+    const int i = idx[t];
+    if (i >= 0) B[(size_t)b * strideB + (size_t)r * ldb + i] = val[t];
+}
+
+// X += D for active items; max |D|, |X| per (item, rhs) as ordered bit patterns.
+__global__ void k_update(double2 *X, const double2 *D, int n, int ldb, int64_t strideB, int nr, const int *active,
+                         unsigned long long *mx)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int br = blockIdx.y, b = br / nr, r = br % nr;
+    if (!active[b]) return;
+    double dm = 0.0, xm = 0.0;
+    if (i < n) {
+        const size_t o = (size_t)b * strideB + (size_t)r * ldb + i;
+        const double2 d = D[o];
+        double2 x = X[o];
+        x.x += d.x;
+        x.y += d.y;
+        X[o] = x;
+        dm = hypot(d.x, d.y);
+        xm = hypot(x.x, x.y);
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+        dm = fmax(dm, __shfl_xor(dm, off));
+        xm = fmax(xm, __shfl_xor(xm, off));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(&mx[2 * br], (unsigned long long)__double_as_longlong(dm));
+        atomicMax(&mx[2 * br + 1], (unsigned long long)__double_as_longlong(xm));
+    }
+}
+
+__global__ void k_zero_u64(unsigned long long *p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+wm_c128 cdiv(wm_c128 a, wm_c128 b)  // Smith (1962)
+{
+    wm_c128 q;
+    if (std::fabs(b.re) >= std::fabs(b.im)) {
+        const double t = b.im / b.re, d = b.re + b.im * t;
+        q.re = (a.re + a.im * t) / d;
+        q.im = (a.im - a.re * t) / d;
+    } else {
+        const double t = b.re / b.im, d = b.im + b.re * t;
+        q.re = (a.re * t + a.im) / d;
+        q.im = (a.im * t - a.re) / d;
+    }
+    return q;
+}
+
+}  // namespace
+
+double wavenumber_mhz(float freq_mhz)  // "wersja referencyjna/Wire_MoM.cu":5-6,338-340
+{
+    const double ambda = 2.99792448E+8 / (double)freq_mhz * 1E-6;
+    return (kPi + kPi) / ambda;
+}
+
+size_t item_bytes(int ld, int nrhs, int prec)
+{
+    const size_t nn = (size_t)ld * ld;
+    size_t b = nn * sizeof(double2);
+    if (prec != WM_PREC_F64) b += nn * sizeof(float2);
+    b += (size_t)ld * nrhs * sizeof(double2) * 4 + (size_t)ld * sizeof(int) + 1024;
+    return b;
+}
+
+int auto_batch(wm_ctx *ctx, int ld, int nrhs, int prec, double frac)
+{
+    size_t freeb = 0, total = 0;
+    if (hipMemGetInfo(&freeb, &total) != hipSuccess) return 1;
+    if (!(frac > 0 && frac <= 1)) frac = 0.8;
+    const size_t have = (size_t)(frac * (double)(freeb + ctx->ws_sweep.bytes));
+    int b = (int)std::max<size_t>(1, have / item_bytes(ld, nrhs, prec));
+    return std::min(b, 64);
+}
+
+int solve_items(wm_ctx *c, const wm_solve_opts &o, WorkItem *items, int count, hipStream_t s)
+{
+    if (count <= 0) return WM_OK;
+    const int n = items[0].g->ld;
+    int nr = 0;
+    for (int b = 0; b < count; ++b) {
+        if (items[b].g->ld != n) {
+            set_error("solve_items: items of different sizes in one batch");
+            return WM_ERR_ARG;
+        }
+        for (int r = 0; r < items[b].nrhs; ++r)
+            if (items[b].gaps[r] < 1 || items[b].gaps[r] > items[b].g->npls) {
+                set_error("feed basis %d outside 1..%d", items[b].gaps[r], items[b].g->npls);
+                return WM_ERR_ARG;
+            }
+        nr = std::max(nr, items[b].nrhs);
+    }
+    const bool mixed = o.prec != WM_PREC_F64;
+    if (o.prec == WM_PREC_BF16X3) {
+        set_error("precision BF16X3 is not available in this build");
+        return WM_ERR_ARG;
+    }
+    const size_t nn = (size_t)n * n;
+    const int64_t sZ = (int64_t)nn, sV = (int64_t)n * nr;
+    size_t off = 0;
+    auto carve = [&](size_t bytes) { const size_t o0 = off; off += (bytes + 255) & ~(size_t)255; return o0; };
+    const size_t oZ = carve(nn * sizeof(double2) * count);
+    const size_t oZ32 = carve(mixed ? nn * sizeof(float2) * count : 0);
+    const size_t szV = (size_t)sV * count * sizeof(double2);
+    const size_t oB = carve(szV), oX = carve(szV), oD = carve(mixed ? szV : 0), oR = carve(mixed ? szV : 0);
+    const size_t oP = carve((size_t)n * count * sizeof(int)), oI = carve(count * sizeof(int));
+    const size_t oM = carve((size_t)2 * count * nr * sizeof(unsigned long long));
+    const size_t oA = carve(count * sizeof(int));
+    const size_t oG = carve((size_t)count * nr * (sizeof(int) + sizeof(double2)));
+    WM_TRY(c->ws_sweep.ensure(off));
+    char *base = (char *)c->ws_sweep.ptr;
+    double2 *Z = (double2 *)(base + oZ);
+    float2 *Z32 = mixed ? (float2 *)(base + oZ32) : nullptr;
+    double2 *B = (double2 *)(base + oB), *X = (double2 *)(base + oX);
+    double2 *D = (double2 *)(base + oD), *R = (double2 *)(base + oR);
+    int *ipiv = (int *)(base + oP), *dinfo = (int *)(base + oI), *dact = (int *)(base + oA);
+    unsigned long long *mx = (unsigned long long *)(base + oM);
+    int *gidx = (int *)(base + oG);
+    double2 *gval = (double2 *)(base + oG + (size_t)count * nr * sizeof(int));
+
+    // excitation: ampl*exp(j*pi*phas/180) at each feed ("wersja referencyjna/Wire_MoM.cu":307-309)
+    std::vector<int> hidx((size_t)count * nr, -1);
+    std::vector<double2> hval((size_t)count * nr, make_double2(0.0, 0.0));
+    for (int b = 0; b < count; ++b) {
+        const double arg = kPi * (double)items[b].phas / 180.;
+        const double2 e = make_double2((double)items[b].ampl * std::cos(arg), (double)items[b].ampl * std::sin(arg));
+        for (int r = 0; r < items[b].nrhs; ++r) {
+            hidx[(size_t)b * nr + r] = items[b].gaps[r] - 1;
+            hval[(size_t)b * nr + r] = e;
+        }
+    }
+    WM_HIP_TRY(hipMemcpyAsync(gidx, hidx.data(), hidx.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    WM_HIP_TRY(hipMemcpyAsync(gval, hval.data(), hval.size() * sizeof(double2), hipMemcpyHostToDevice, s));
+    const int64_t nv = sV * count;
+    hipLaunchKernelGGL(k_zero2, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, s, B, nv);
+    hipLaunchKernelGGL(k_set_rhs, dim3((count * nr + 255) / 256), dim3(256), 0, s, B, sV, n, nr, gidx, gval, count);
+
+    // fill, one launch group per run of items sharing a geometry
+    for (int b0 = 0; b0 < count;) {
+        int b1 = b0 + 1;
+        while (b1 < count && items[b1].g == items[b0].g) ++b1;
+        std::vector<double> ka(b1 - b0);
+        for (int b = b0; b < b1; ++b) ka[b - b0] = wavenumber_mhz(items[b].freq);
+        WM_TRY(fill_launch(c, items[b0].g, b1 - b0, ka.data(), Z + (size_t)b0 * sZ, sZ,
+                           mixed ? (void *)(Z32 + (size_t)b0 * sZ) : nullptr, sZ, s));
+        b0 = b1;
+    }
+
+    std::vector<int> hinfo(count, 0), iters(count, 0), fell(count, 0);
+    std::vector<double> dxr(count, 0.0);
+    if (!mixed) {
+        WM_TRY(getrf_batched(c, WM_PREC_F64, n, Z, n, sZ, ipiv, n, count, dinfo, s));
+        WM_HIP_TRY(hipMemcpyAsync(X, B, szV, hipMemcpyDeviceToDevice, s));
+        WM_TRY(getrs_batched(c, WM_PREC_F64, n, nr, Z, n, sZ, ipiv, n, X, n, sV, count, s));
+        WM_HIP_TRY(hipMemcpyAsync(hinfo.data(), dinfo, sizeof(int) * count, hipMemcpyDeviceToHost, s));
+        WM_TRY(check_device_error(c, s));
+    } else {
+        WM_TRY(getrf_batched(c, WM_PREC_F32, n, Z32, n, sZ, ipiv, n, count, dinfo, s));
+        WM_HIP_TRY(hipMemcpyAsync(hinfo.data(), dinfo, sizeof(int) * count, hipMemcpyDeviceToHost, s));
+        hipLaunchKernelGGL(k_zero2, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, s, X, nv);
+        WM_HIP_TRY(hipMemcpyAsync(R, B, szV, hipMemcpyDeviceToDevice, s));
+        WM_TRY(check_device_error(c, s));
+        std::vector<int> active(count, 1);
+        for (int b = 0; b < count; ++b)
+            if (hinfo[b] != 0) active[b] = 0;  // singular in FP32: straight to the FP64 fallback
+        std::vector<double> prev((size_t)count * nr, 1e300);
+        std::vector<unsigned long long> hm((size_t)2 * count * nr);
+        std::vector<int> conv(count, 0);
+        for (int it = 0; it < o.max_iter; ++it) {
+            if (std::none_of(active.begin(), active.end(), [](int a) { return a != 0; })) break;
+            WM_HIP_TRY(hipMemcpyAsync(dact, active.data(), sizeof(int) * count, hipMemcpyHostToDevice, s));
+            WM_HIP_TRY(hipMemcpyAsync(D, R, szV, hipMemcpyDeviceToDevice, s));
+            WM_TRY(getrs_batched(c, WM_PREC_F32, n, nr, Z32, n, sZ, ipiv, n, D, n, sV, count, s));
+            hipLaunchKernelGGL(k_zero_u64, dim3((2 * count * nr + 255) / 256), dim3(256), 0, s, mx, 2 * count * nr);
+            hipLaunchKernelGGL(k_update, dim3((n + 255) / 256, count * nr), dim3(256), 0, s, X, D, n, n, sV, nr, dact,
+                               mx);
+            WM_TRY(residual_batched(c, n, nr, Z, n, sZ, X, B, n, sV, R, count, s));
+            WM_HIP_TRY(hipMemcpyAsync(hm.data(), mx, hm.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+            WM_TRY(check_device_error(c, s));
+            for (int b = 0; b < count; ++b) {
+                if (!active[b]) continue;
+                double worst = 0.0;
+                bool stalled = true;
+                for (int r = 0; r < items[b].nrhs; ++r) {
+                    double dm, xm;
+                    std::memcpy(&dm, &hm[2 * ((size_t)b * nr + r)], 8);
+                    std::memcpy(&xm, &hm[2 * ((size_t)b * nr + r) + 1], 8);
+                    const double rel = xm > 0 ? dm / xm : 0.0;
+                    worst = std::max(worst, rel);
+                    double &p = prev[(size_t)b * nr + r];
+                    if (!(rel > 0.5 * p && p < 1e-9)) stalled = false;
+                    p = rel;
+                }
+                dxr[b] = worst;
+                iters[b] = it + 1;
+                if (worst <= o.tol || stalled) {
+                    active[b] = 0;
+                    conv[b] = 1;
+                }
+            }
+        }
+        // items that did not converge (or were singular in FP32): FP64 refactorization
+        for (int b = 0; b < count; ++b) {
+            if (conv[b]) continue;
+            if (!o.fallback_f64) {
+                set_error("refinement did not converge (item %d, ||dx||/||x|| = %.3g)", b, dxr[b]);
+                return WM_ERR_NOCONV;
+            }
+            WM_TRY(getrf_batched(c, WM_PREC_F64, n, Z + (size_t)b * sZ, n, sZ, ipiv + (size_t)b * n, n, 1, dinfo + b, s));
+            WM_HIP_TRY(hipMemcpyAsync(X + (size_t)b * sV, B + (size_t)b * sV, (size_t)sV * sizeof(double2),
+                                      hipMemcpyDeviceToDevice, s));
+            WM_TRY(getrs_batched(c, WM_PREC_F64, n, nr, Z + (size_t)b * sZ, n, sZ, ipiv + (size_t)b * n, n,
+                                 X + (size_t)b * sV, n, sV, 1, s));
+            WM_HIP_TRY(hipMemcpyAsync(&hinfo[b], dinfo + b, sizeof(int), hipMemcpyDeviceToHost, s));
+            fell[b] = 1;
+        }
+        WM_TRY(check_device_error(c, s));
+    }
+
+    std::vector<double2> hX((size_t)sV * count);
+    WM_HIP_TRY(hipMemcpyAsync(hX.data(), X, szV, hipMemcpyDeviceToHost, s));
+    WM_TRY(check_device_error(c, s));
+    int status = WM_OK;
+    for (int b = 0; b < count; ++b) {
+        const WorkItem &w = items[b];
+        const int N = w.g->npls;
+        if (hinfo[b] != 0 && status == WM_OK) {
+            set_error("zero pivot at column %d (%g MHz)", hinfo[b], (double)w.freq);
+            status = hinfo[b];
+        }
+        for (int r = 0; r < w.nrhs; ++r) {
+            const double2 *xv = &hX[(size_t)b * sV + (size_t)r * n];
+            if (w.I)
+                for (int i = 0; i < N; ++i) w.I[(size_t)r * N + i] = wm_c128{xv[i].x, xv[i].y};
+            if (w.zin) {
+                const double2 e = hval[(size_t)b * nr + r];
+                w.zin[r] = cdiv(wm_c128{e.x, e.y}, wm_c128{xv[w.gaps[r] - 1].x, xv[w.gaps[r] - 1].y});
+            }
+        }
+        if (w.st) {
+            w.st->iters = iters[b];
+            w.st->info = hinfo[b];
+            w.st->fell_back = fell[b];
+            w.st->backward_err = -1.0;
+            w.st->dx_rel = dxr[b];
+        }
+    }
+    return status;
+}
+
+}  // namespace wm

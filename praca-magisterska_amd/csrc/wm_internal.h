@@ -1,0 +1,123 @@
+// Internal declarations shared by the libwiremom sources (host side).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstddef>
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/wiremom.h"
+
+namespace wm {
+
+void set_error(const char *fmt, ...);
+
+#define WM_HIP_TRY(expr)                                                                 \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess) {                                                          \
+            ::wm::set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_),       \
+                            __FILE__, __LINE__);                                         \
+            return WM_ERR_HIP;                                                           \
+        }                                                                                \
+    } while (0)
+
+#define WM_TRY(expr)                 \
+    do {                             \
+        int r_ = (expr);             \
+        if (r_ != WM_OK) return r_;  \
+    } while (0)
+
+// Device buffer that only grows; reused across calls on one context.
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t need);
+    void release();
+};
+
+// Device-side error word written by kernels with bounded waits.
+enum DevErr : int { DEV_OK = 0, DEV_SPIN_TIMEOUT = 1 };
+
+}  // namespace wm
+
+struct wm_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int num_cu = 0;
+    size_t hbm_bytes = 0;
+    size_t max_lds = 0;
+    char name[256] = {0};
+    std::recursive_mutex mu;
+    // small persistent device scratch: sync counters and the error word
+    int *d_sync = nullptr;  // kSyncCap counters
+    int *d_err = nullptr;   // 1 int
+    static const int kSyncCap = 4096;
+    wm::DevBuf ws_fill;     // per-frequency self-term integrals
+    wm::DevBuf ws_lu;       // panel exchange buffers
+    wm::DevBuf ws_solve;    // triangular-solve permutation, flags and staging
+    wm::DevBuf ws_resid;    // residual partial sums
+    wm::DevBuf ws_sweep;    // matrices and vectors of a sweep batch
+    int solve_epoch = 0;    // monotonic tag of triangular-solve flags
+};
+
+// Geometry as the fill kernel reads it.  Observation data per basis (SoA,
+// one lane per test basis) and source data per basis (AoS, wave-uniform).
+struct wm_geom {
+    wm_ctx *ctx = nullptr;
+    int npls = 0;     // N, unknowns
+    int ld = 0;       // padded size (multiple of 128)
+    double *d_obs = nullptr;   // 12 arrays of ld doubles: rc1xyz, lm1xyz, rc2xyz, lm2xyz
+    int *d_obs_key = nullptr;  // 2 arrays of ld ints: segment key of obs half 1, 2
+    double *d_src = nullptr;   // ld records of 16 doubles: rs1[3] dc1[3] dl1 an1 rs2[3] dc2[3] dl2 an2
+    int *d_src_key = nullptr;  // ld records of 2 ints
+    std::vector<double> h_src; // host copy of the source records (self-term precompute)
+};
+
+namespace wm {
+
+// Fill: nf frequencies, complex128 output (+ optional complex64 copy).
+int fill_launch(wm_ctx *ctx, const wm_geom *g, int nf, const double *ka, void *Z, int64_t strideZ,
+                void *Z32, int64_t strideZ32, hipStream_t s);
+
+// LU: batched blocked right-looking factorization with partial pivoting.
+int getrf_batched(wm_ctx *ctx, int prec, int n, void *A, int lda, int64_t strideA, int *ipiv,
+                  int64_t strideP, int batch, int *d_info, hipStream_t s);
+// Triangular solves with the LU factors; B complex128.
+int getrs_batched(wm_ctx *ctx, int prec, int n, int nrhs, const void *A, int lda, int64_t strideA,
+                  const int *ipiv, int64_t strideP, void *B, int ldb, int64_t strideB, int batch,
+                  hipStream_t s);
+// R = B - Z X for complex128 Z (n x n), X, B (n x nrhs); batched.
+int residual_batched(wm_ctx *ctx, int n, int nrhs, const void *Z, int ldz, int64_t strideZ,
+                     const void *X, const void *B, int ldb, int64_t strideB, void *R, int batch,
+                     hipStream_t s);
+
+int check_device_error(wm_ctx *ctx, hipStream_t s);
+
+// One (geometry, frequency, feeds) solve.  Items of one solve_items call
+// share the padded size ld; outputs are host pointers (may be null).
+struct WorkItem {
+    const wm_geom *g = nullptr;
+    float freq = 0.f;          // MHz
+    int nrhs = 0;
+    const int *gaps = nullptr; // 1-based feed bases, nrhs of them
+    float ampl = 1.f, phas = 0.f;
+    wm_c128 *I = nullptr;      // [nrhs][npls]
+    wm_c128 *zin = nullptr;    // [nrhs]
+    wm_freq_stats *st = nullptr;
+};
+
+// Fill, factor and solve a batch of items on stream s (synchronous).
+// Returns <0 on error, >0 (first zero pivot) if a matrix was singular.
+int solve_items(wm_ctx *ctx, const wm_solve_opts &o, WorkItem *items, int count, hipStream_t s);
+// Device bytes one item of size ld needs in solve_items.
+size_t item_bytes(int ld, int nrhs, int prec);
+// Items of size ld that fit the given share of free HBM (capped at 64).
+int auto_batch(wm_ctx *ctx, int ld, int nrhs, int prec, double hbm_fraction);
+double wavenumber_mhz(float freq_mhz);
+
+}  // namespace wm

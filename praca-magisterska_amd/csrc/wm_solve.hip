@@ -1,0 +1,366 @@
+// Triangular solves with LU factors, FP64 residuals and refinement updates.
+//
+// Replaces LAPACKE_zgetrs ("wersja referencyjna/Wire_MoM.cu":382) and
+// cublasZgetrsBatched ("wersja gpu/Wire_MoM.cu":477): B is permuted by the
+// pivot sequence, then L y = P b (unit lower) and U x = y are solved.
+//
+// Each triangular solve is one launch.  Workgroup i owns row block i (64
+// rows).  It subtracts the contributions of the blocks already solved, as
+// each one becomes ready (a flag per block, set after its values are
+// stored), and then solves its diagonal block inside one wave with lane
+// broadcasts.  A workgroup only ever waits on blocks owned by workgroups
+// with a smaller id in the same grid, which are dispatched before it, so the
+// chain always makes progress; every wait is bounded.  Arithmetic is FP64;
+// complex64 factors are widened on load.
+#include "wm_cplx.h"
+#include "wm_internal.h"
+
+namespace wm {
+
+namespace {
+
+constexpr int TB = 64;  // rows per block
+
+template <typename T>
+__device__ inline double2 ld2(const typename CT<T>::V *p)
+{
+    const typename CT<T>::V v = *p;
+    return make_double2((double)v.x, (double)v.y);
+}
+
+__device__ inline void cfms(double2 &acc, double2 a, double2 b)  // acc -= a*b
+{
+    acc.x -= a.x * b.x - a.y * b.y;
+    acc.y -= a.x * b.y + a.y * b.x;
+}
+
+// perm[t] = source row of position t after applying ipiv in order.
+__global__ void k_perm(const int *ipiv, int64_t strideP, int n, int *perm, int64_t strideQ)
+{
+    extern __shared__ int s_p[];
+    const int b = blockIdx.x;
+    const int *ip = ipiv + (size_t)b * strideP;
+    int *pq = perm + (size_t)b * strideQ;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s_p[i] = i;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 0; i < n; ++i) {
+            const int p = ip[i] - 1;
+            if (p != i) { const int t = s_p[i]; s_p[i] = s_p[p]; s_p[p] = t; }
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) pq[i] = s_p[i];
+}
+
+struct TrsvArgs {
+    const void *A;
+    int lda;
+    int64_t strideA;
+    int n, nrhs;
+    const int *perm;
+    int64_t strideQ;
+    const double2 *Bin;   // lower solve: right-hand sides (gathered through perm)
+    int ldb;
+    int64_t strideB;
+    double2 *Y;           // output (lower: y, upper: x in place of y)
+    int *flags;           // nblk per matrix
+    int epoch;
+    int *err;
+};
+
+// L y = P b.  NR right-hand sides per pass.
+template <typename T, int NR>
+__global__ __launch_bounds__(256) void k_trsv_lower(TrsvArgs a, int r_off)
+{
+    using V = typename CT<T>::V;
+    const int b = blockIdx.y, bi = blockIdx.x, tid = threadIdx.x, row = tid & 63, part = tid >> 6;
+    const int nblk = a.n / TB, r0 = bi * TB;
+    const V *A = (const V *)a.A + (size_t)b * a.strideA;
+    const int *perm = a.perm + (size_t)b * a.strideQ;
+    const double2 *Bin = a.Bin + (size_t)b * a.strideB + (size_t)r_off * a.ldb;
+    double2 *Y = a.Y + (size_t)b * a.strideB + (size_t)r_off * a.ldb;
+    int *flags = a.flags + (size_t)b * nblk;
+
+    double2 acc[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) acc[r] = make_double2(0.0, 0.0);
+    if (part == 0) {
+        const int src = perm[r0 + row];
+#pragma unroll
+        for (int r = 0; r < NR; ++r)
+            if (r < a.nrhs - r_off) acc[r] = Bin[src + (size_t)r * a.ldb];
+    }
+    for (int jb = 0; jb < bi; ++jb) {
+        wait_flag(flags + jb, a.epoch, a.err);
+        const int c0 = jb * TB + part * 16;
+#pragma unroll 4
+        for (int c = 0; c < 16; ++c) {
+            const double2 l = ld2<T>(A + (r0 + row) + (size_t)(c0 + c) * a.lda);
+#pragma unroll
+            for (int r = 0; r < NR; ++r)
+                if (r < a.nrhs - r_off) cfms(acc[r], l, Y[(c0 + c) + (size_t)r * a.ldb]);
+        }
+    }
+    __shared__ double2 s_acc[3][TB][NR];
+    if (part > 0)
+#pragma unroll
+        for (int r = 0; r < NR; ++r) s_acc[part - 1][row][r] = acc[r];
+    __syncthreads();
+    if (part == 0) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r)
+#pragma unroll
+            for (int p = 0; p < 3; ++p) { acc[r].x += s_acc[p][row][r].x; acc[r].y += s_acc[p][row][r].y; }
+        // diagonal block, unit lower, lane `row` holds row r0+row
+        for (int j = 0; j < TB; ++j) {
+            const double2 l = ld2<T>(A + (r0 + row) + (size_t)(r0 + j) * a.lda);
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                const double yr = __shfl(acc[r].x, j), yi = __shfl(acc[r].y, j);
+                if (row > j) cfms(acc[r], l, make_double2(yr, yi));
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < NR; ++r)
+            if (r < a.nrhs - r_off) Y[(r0 + row) + (size_t)r * a.ldb] = acc[r];
+    }
+    set_flag(flags + bi, a.epoch);
+}
+
+// U x = y in place of y.  Workgroup g solves row block nblk-1-g.
+template <typename T, int NR>
+__global__ __launch_bounds__(256) void k_trsv_upper(TrsvArgs a, int r_off)
+{
+    using V = typename CT<T>::V;
+    const int b = blockIdx.y, tid = threadIdx.x, row = tid & 63, part = tid >> 6;
+    const int nblk = a.n / TB, bi = nblk - 1 - (int)blockIdx.x, r0 = bi * TB;
+    const V *A = (const V *)a.A + (size_t)b * a.strideA;
+    double2 *Y = a.Y + (size_t)b * a.strideB + (size_t)r_off * a.ldb;
+    int *flags = a.flags + (size_t)b * nblk;
+
+    double2 acc[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) acc[r] = make_double2(0.0, 0.0);
+    if (part == 0) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r)
+            if (r < a.nrhs - r_off) acc[r] = Y[(r0 + row) + (size_t)r * a.ldb];
+    }
+    for (int jb = nblk - 1; jb > bi; --jb) {
+        wait_flag(flags + jb, a.epoch, a.err);
+        const int c0 = jb * TB + part * 16;
+#pragma unroll 4
+        for (int c = 0; c < 16; ++c) {
+            const double2 u = ld2<T>(A + (r0 + row) + (size_t)(c0 + c) * a.lda);
+#pragma unroll
+            for (int r = 0; r < NR; ++r)
+                if (r < a.nrhs - r_off) cfms(acc[r], u, Y[(c0 + c) + (size_t)r * a.ldb]);
+        }
+    }
+    __shared__ double2 s_acc[3][TB][NR];
+    if (part > 0)
+#pragma unroll
+        for (int r = 0; r < NR; ++r) s_acc[part - 1][row][r] = acc[r];
+    __syncthreads();
+    if (part == 0) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r)
+#pragma unroll
+            for (int p = 0; p < 3; ++p) { acc[r].x += s_acc[p][row][r].x; acc[r].y += s_acc[p][row][r].y; }
+        for (int j = TB - 1; j >= 0; --j) {
+            const double2 u = ld2<T>(A + (r0 + row) + (size_t)(r0 + j) * a.lda);
+            // lane j divides by its diagonal; everyone else eliminates
+            double2 q[NR];
+            const double2 rinv = crecip(u.x, u.y);  // used by lane j only
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                if (row == j) {
+                    const double2 x = acc[r];
+                    acc[r] = make_double2(x.x * rinv.x - x.y * rinv.y, x.x * rinv.y + x.y * rinv.x);
+                }
+                q[r] = make_double2(__shfl(acc[r].x, j), __shfl(acc[r].y, j));
+                if (row < j) cfms(acc[r], u, q[r]);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < NR; ++r)
+            if (r < a.nrhs - r_off) Y[(r0 + row) + (size_t)r * a.ldb] = acc[r];
+    }
+    set_flag(flags + bi, a.epoch);
+}
+
+// Partial products P[split][row][r] = sum_{j in split} Z[row][j] X[j][r].
+template <int NR>
+__global__ __launch_bounds__(256) void k_resid_part(const double2 *Z, int ldz, int64_t strideZ, int n, int nrhs,
+                                                    const double2 *X, int ldb, int64_t strideB, int cols_per_split,
+                                                    int nsplit, double2 *P, int r_off)
+{
+    const int b = blockIdx.z, split = blockIdx.y, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int row = blockIdx.x * 64 + lane;
+    const double2 *Zb = Z + (size_t)b * strideZ;
+    const double2 *Xb = X + (size_t)b * strideB + (size_t)r_off * ldb;
+    const int cw = cols_per_split / 4;
+    const int c0 = split * cols_per_split + wave * cw;
+    double2 acc[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) acc[r] = make_double2(0.0, 0.0);
+#pragma unroll 4
+    for (int c = 0; c < cw; ++c) {
+        const int col = c0 + c;
+        const double2 z = Zb[row + (size_t)col * ldz];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            if (r < nrhs - r_off) {
+                const double2 x = Xb[col + (size_t)r * ldb];
+                acc[r].x += z.x * x.x - z.y * x.y;
+                acc[r].y += z.x * x.y + z.y * x.x;
+            }
+        }
+    }
+    __shared__ double2 s[3][64][NR];
+    if (wave > 0)
+#pragma unroll
+        for (int r = 0; r < NR; ++r) s[wave - 1][lane][r] = acc[r];
+    __syncthreads();
+    if (wave == 0) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+#pragma unroll
+            for (int p = 0; p < 3; ++p) { acc[r].x += s[p][lane][r].x; acc[r].y += s[p][lane][r].y; }
+            P[(((size_t)b * nsplit + split) * NR + r) * n + row] = acc[r];
+        }
+    }
+}
+
+// R = B - sum_split P, in split order.
+template <int NR>
+__global__ void k_resid_final(const double2 *P, int nsplit, int n, int nrhs, const double2 *B, int ldb,
+                              int64_t strideB, double2 *R, int r_off, int batch)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.y;
+    if (i >= n) return;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        if (r >= nrhs - r_off) break;
+        double2 s = make_double2(0.0, 0.0);
+        for (int sp = 0; sp < nsplit; ++sp) {
+            const double2 p = P[(((size_t)b * nsplit + sp) * NR + r) * n + i];
+            s.x += p.x;
+            s.y += p.y;
+        }
+        const size_t o = (size_t)b * strideB + (size_t)(r_off + r) * ldb + i;
+        R[o] = make_double2(B[o].x - s.x, B[o].y - s.y);
+    }
+}
+
+template <typename T, int NR>
+void launch_trsv(const TrsvArgs &a, int batch, int r_off, bool upper, hipStream_t s)
+{
+    const dim3 grid(a.n / TB, batch);
+    if (upper) hipLaunchKernelGGL((k_trsv_upper<T, NR>), grid, dim3(256), 0, s, a, r_off);
+    else hipLaunchKernelGGL((k_trsv_lower<T, NR>), grid, dim3(256), 0, s, a, r_off);
+}
+
+template <typename T>
+void trsv_dispatch(const TrsvArgs &a, int batch, int r_off, bool upper, hipStream_t s)
+{
+    const int left = a.nrhs - r_off;
+    if (left >= 8) launch_trsv<T, 8>(a, batch, r_off, upper, s);
+    else if (left >= 4) launch_trsv<T, 4>(a, batch, r_off, upper, s);
+    else if (left >= 2) launch_trsv<T, 2>(a, batch, r_off, upper, s);
+    else launch_trsv<T, 1>(a, batch, r_off, upper, s);
+}
+
+__global__ void k_zero_flags(int *p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+}  // namespace
+
+// Workspace layout in ctx->ws_solve: perm [batch][n] ints, flags [2][batch][n/64] ints.
+int getrs_batched(wm_ctx *ctx, int prec, int n, int nrhs, const void *A, int lda, int64_t strideA,
+                  const int *ipiv, int64_t strideP, void *B, int ldb, int64_t strideB, int batch, hipStream_t s)
+{
+    if (n <= 0 || n % TB != 0 || nrhs <= 0 || batch <= 0 || ldb < n) {
+        set_error("getrs: bad arguments (n=%d multiple of %d, nrhs=%d, batch=%d)", n, TB, nrhs, batch);
+        return WM_ERR_ARG;
+    }
+    const int nblk = n / TB;
+    const size_t perm_bytes = (size_t)batch * n * sizeof(int);
+    const size_t flag_bytes = (size_t)2 * batch * nblk * sizeof(int);
+    const size_t y_bytes = (size_t)batch * strideB * sizeof(double2);
+    const size_t need = perm_bytes + flag_bytes + y_bytes + 1024;
+    const bool grew = ctx->ws_solve.bytes < need;
+    WM_TRY(ctx->ws_solve.ensure(need));
+    char *base = (char *)ctx->ws_solve.ptr;
+    int *perm = (int *)base;
+    int *flags = (int *)(base + perm_bytes);
+    double2 *Ytmp = (double2 *)(base + ((perm_bytes + flag_bytes + 255) & ~(size_t)255));
+    if (grew) {
+        const int nf = (int)(flag_bytes / sizeof(int));
+        hipLaunchKernelGGL(k_zero_flags, dim3((nf + 255) / 256), dim3(256), 0, s, flags, nf);
+        ctx->solve_epoch = 0;
+    }
+    if ((size_t)n * sizeof(int) > ctx->max_lds) {
+        set_error("getrs: n=%d exceeds the LDS permutation capacity", n);
+        return WM_ERR_ARG;
+    }
+    hipLaunchKernelGGL(k_perm, dim3(batch), dim3(256), n * sizeof(int), s, ipiv, strideP, n, perm, (int64_t)n);
+    // copy B to the temporary (lower solve reads B, writes Y; upper solves Y in place)
+    WM_HIP_TRY(hipMemcpy2DAsync(Ytmp, strideB * sizeof(double2), B, strideB * sizeof(double2),
+                                strideB * sizeof(double2), batch, hipMemcpyDeviceToDevice, s));
+    for (int r0 = 0; r0 < nrhs; r0 += 8) {
+        TrsvArgs a;
+        a.A = A; a.lda = lda; a.strideA = strideA; a.n = n; a.nrhs = nrhs;
+        a.perm = perm; a.strideQ = n; a.ldb = ldb; a.strideB = strideB; a.err = ctx->d_err;
+        // lower: Bin = Ytmp (copy of B), output into B
+        a.Bin = Ytmp; a.Y = (double2 *)B; a.flags = flags;
+        a.epoch = ++ctx->solve_epoch;
+        if (prec == WM_PREC_F64) trsv_dispatch<double>(a, batch, r0, false, s);
+        else trsv_dispatch<float>(a, batch, r0, false, s);
+        a.Bin = nullptr; a.flags = flags + (size_t)batch * nblk;
+        if (prec == WM_PREC_F64) trsv_dispatch<double>(a, batch, r0, true, s);
+        else trsv_dispatch<float>(a, batch, r0, true, s);
+    }
+    WM_HIP_TRY(hipGetLastError());
+    return WM_OK;
+}
+
+int residual_batched(wm_ctx *ctx, int n, int nrhs, const void *Z, int ldz, int64_t strideZ, const void *X,
+                     const void *B, int ldb, int64_t strideB, void *R, int batch, hipStream_t s)
+{
+    if (n % 64 != 0) {
+        set_error("residual: n=%d not a multiple of 64", n);
+        return WM_ERR_ARG;
+    }
+    // column splits depend on n only, so an item's rounding never depends on the batch
+    int nsplit = 1;
+    while (nsplit < 16 && n % (nsplit * 2 * 64) == 0 && n / (nsplit * 2) >= 256) nsplit *= 2;
+    const int cps = n / nsplit;
+    const int nrp = nrhs >= 8 ? 8 : nrhs >= 4 ? 4 : nrhs >= 2 ? 2 : 1;
+    DevBuf &partial = ctx->ws_resid;
+    const size_t need = (size_t)batch * nsplit * nrp * n * sizeof(double2);
+    WM_TRY(partial.ensure(need));
+    for (int r0 = 0; r0 < nrhs; r0 += 8) {
+        const int left = nrhs - r0;
+        const dim3 g1(n / 64, nsplit, batch), g2((n + 255) / 256, batch);
+#define WM_RESID(NR)                                                                                        \
+    hipLaunchKernelGGL(k_resid_part<NR>, g1, dim3(256), 0, s, (const double2 *)Z, ldz, strideZ, n, nrhs,    \
+                       (const double2 *)X, ldb, strideB, cps, nsplit, (double2 *)partial.ptr, r0);          \
+    hipLaunchKernelGGL(k_resid_final<NR>, g2, dim3(256), 0, s, (const double2 *)partial.ptr, nsplit, n, nrhs, \
+                       (const double2 *)B, ldb, strideB, (double2 *)R, r0, batch)
+        if (left >= 8) { WM_RESID(8); }
+        else if (left >= 4) { WM_RESID(4); }
+        else if (left >= 2) { WM_RESID(2); }
+        else { WM_RESID(1); }
+#undef WM_RESID
+    }
+    WM_HIP_TRY(hipGetLastError());
+    return WM_OK;
+}
+
+}  // namespace wm

@@ -1,0 +1,191 @@
+"""Pins the CPU oracle (oracle/wm_oracle.c) against the reference's own data and
+against what the mathematics fixes.  CPU only."""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _dipole(G, d, nseg, length=25.0, radius=0.0005):
+    p = os.path.join(d, f"dipole_{nseg}_{length}_{radius}.geo")
+    G.dipole(p, nseg, length, radius)
+    return p
+
+
+def _fmt4(v):
+    # the reference prints Zin with a sticky std::setprecision(4) (default float format)
+    return "%.4g" % v
+
+
+def test_configwire_matches_reference_files(oracle, geometries, geo_dir):
+    """configwire.txt written by the oracle is byte-identical to the files the reference shipped."""
+    kat = json.load(open(os.path.join(GOLD, "configwire_kat.json")))
+    assert len(kat) == 4
+    for k in kat:
+        p = _dipole(geometries, geo_dir, k["nseg"], 25.0, 0.0005)
+        out = os.path.join(geo_dir, f"cw_{k['nseg']}.txt")
+        t = oracle.load(p, wirefile=out)
+        data = open(out, "rb").read()
+        assert t.npls == k["npls"]
+        assert len(data) == k["bytes"]
+        assert hashlib.sha256(data).hexdigest() == k["sha256"], k["variant"]
+
+
+def test_zin_known_answers(oracle, geometries, geo_dir):
+    """Zin of the 25 m dipole (feed basis 100) equals every logged 4-s.f. value for N <= 1024."""
+    cases = json.load(open(os.path.join(GOLD, "zin_kat.json")))["cases"]
+    assert len(cases) == 308
+    small = [c for c in cases if c["nseg"] <= 1024]
+    assert len(small) == 36
+    topo = {}
+    bad = []
+    for c in small:
+        if c["nseg"] not in topo:
+            topo[c["nseg"]] = oracle.load(_dipole(geometries, geo_dir, c["nseg"]))
+        zin, _, _ = oracle.solve_freq(topo[c["nseg"]], float(c["freq_mhz"]), gaps=(100,))
+        z = zin[0]
+        if (_fmt4(z.real), _fmt4(z.imag)) != (c["re"], c["im"]):
+            bad.append((c, z))
+    assert not bad, bad
+
+
+def test_gauss_legendre_rules_are_textbook():
+    """The reference's quadrature constants are the 2- and 4-point Gauss-Legendre rules on [0, b]."""
+    x2, w2 = np.polynomial.legendre.leggauss(2)
+    assert abs(x2[1] / 2 - 0.288675134594812882) < 1e-16
+    assert abs(w2[0] / 2 - 0.5) < 1e-16
+    x4, w4 = np.polynomial.legendre.leggauss(4)
+    assert abs(x4[3] / 2 - 0.43056815579702629) < 1e-16
+    assert abs(x4[2] / 2 - 0.16999052179242813) < 1e-16
+    assert abs(w4[3] / 2 - 0.17392742256872693) < 1e-16
+    assert abs(w4[2] / 2 - 0.32607257743127307) < 1e-16
+
+
+def test_elliptic_k_polynomial_matches_ellipk(oracle):
+    """delick is the A&S 17.3.34 polynomial: |K_poly - K| <= 2e-8 against scipy's exact K."""
+    from scipy.special import ellipk
+    for m in np.concatenate([np.linspace(0.0, 0.999, 200), 1 - np.logspace(-3, -8, 40)]):
+        bet = np.sqrt(m)
+        assert abs(oracle.delick(bet) - ellipk(m)) <= 2.2e-8, m
+
+
+def test_self_integral_against_adaptive_quadrature(oracle):
+    """The singularity-extracted self term equals the integral of
+    w(s) [ (exp(-jk rho) - 1)/rho + beta K(beta)/(pi a) ] over the segment: the
+    log term integrated by the 4-point rules and the analytic term cancel
+    (Zmatrix.cpp:279-323, 695-728).  Brute force with scipy quad + exact K.
+
+    What is left is the reference's own 4-point quadrature error on a smooth
+    feature of width ~a, so the gap must shrink as dl/a falls: <= 1e-4 at
+    dl/a <= 10 and monotone in dl/a.  A dropped analytic term or a sign error
+    is off by O(1) or more at every aspect ratio."""
+    from scipy.integrate import quad
+    from scipy.special import ellipk
+
+    def rel_err(ifun, ka, dl, a):
+        def w(s):
+            return s / dl if ifun == 1 else (dl - s) / dl if ifun == 2 else 1.0
+
+        def f(s, part):
+            r = 0.5 * dl - s
+            rho = np.hypot(a, r)
+            bet = 2 * a / np.sqrt(4 * a * a + r * r)
+            v = (np.exp(-1j * ka * rho) - 1) / rho + bet * ellipk(bet * bet) / (np.pi * a)
+            return w(s) * (v.real if part == 0 else v.imag)
+
+        halves = [(0, dl / 2), (dl / 2, dl)]
+        ref = complex(sum(quad(f, lo, hi, args=(0,), limit=400)[0] for lo, hi in halves),
+                      sum(quad(f, lo, hi, args=(1,), limit=400)[0] for lo, hi in halves))
+        got = oracle.self_integral(ifun, ka, dl, a)
+        return abs(got - ref) / abs(ref)
+
+    for ifun in (1, 2, 3):
+        for ka, dl, a in [(0.3, 0.04, 0.004), (1.0, 0.02, 0.002), (1.0, 0.01, 0.002)]:
+            assert rel_err(ifun, ka, dl, a) <= 1e-4, (ifun, ka, dl, a)
+        errs = [rel_err(ifun, 2.0, 0.05, a) for a in (0.0005, 0.001, 0.0025, 0.005, 0.01)]
+        assert all(e2 < e1 for e1, e2 in zip(errs, errs[1:])), errs
+        assert errs[0] < 0.05, errs
+
+
+def test_uniform_wire_is_toeplitz_and_symmetric(oracle, geometries, geo_dir):
+    """Z of a straight uniform wire is symmetric Toeplitz up to float32 node rounding."""
+    t = oracle.load(_dipole(geometries, geo_dir, 64, 1.0, 0.001))
+    Z = oracle.zmatrix(t, oracle.wavenumber(150.0))
+    scale = np.abs(Z).max()
+    assert np.abs(Z - Z.T).max() <= 1e-5 * scale
+    for d in range(0, 10):
+        diag = np.diagonal(Z, d)
+        assert np.abs(diag - diag[len(diag) // 2]).max() <= 1e-5 * scale
+
+
+def test_half_wave_dipole_impedance(oracle, geometries, geo_dir):
+    """A thin half-wave dipole has Zin near 73 + j42 ohm (thin-wire limit): physical sanity of the fill."""
+    t = oracle.load(_dipole(geometries, geo_dir, 200, 1.0, 1e-4))
+    f = 2.99792448e8 / 2.0 / 1e6  # length = lambda/2
+    zin, _, _ = oracle.solve_freq(t, f, gaps=(100,))
+    assert 60 < zin[0].real < 100 and 20 < zin[0].imag < 70, zin
+
+
+def test_lu_matches_lapack(oracle):
+    """zgetrf/zgetrs: same pivots as LAPACK (scipy), factors and solution to rounding."""
+    from scipy.linalg import lapack
+    rng = np.random.RandomState(0)
+    for n in (1, 7, 64, 130):
+        A = rng.randn(n, n) + 1j * rng.randn(n, n)
+        LU, piv, info = oracle.zgetrf(A)
+        lu2, piv2, info2 = lapack.zgetrf(A)
+        assert info == info2 == 0
+        assert np.array_equal(piv, piv2 + 1)
+        assert np.abs(LU - lu2).max() <= 1e-10 * np.abs(lu2).max()
+        b = rng.randn(n, 2) + 1j * rng.randn(n, 2)
+        x = oracle.zgetrs(LU, piv, b)
+        assert np.abs(A @ x - b).max() <= 1e-11 * np.abs(A).max() * np.abs(x).max() * n
+
+
+def test_lu_zero_pivot_info(oracle):
+    """An exactly zero column gives LAPACK-style info (1-based column)."""
+    A = np.eye(5, dtype=complex)
+    A[:, 2] = 0
+    _, _, info = oracle.zgetrf(A)
+    assert info == 3
+
+
+def _two_wire_dipole(d, kind, nhalf=64, L=8.0, a=0.001):
+    """Straight dipole along z built from two wires meeting at z=0, joined by
+    connection `kind` (1: start-start, 2: start-end, 3: end-start, 4: end-end)."""
+    import wiremom.geometries as G
+    bot, mid, top = (0, 0, -L / 2), (0, 0, 0.0), (0, 0, L / 2)
+    nodes = [bot, mid, top]
+    w = {1: [(2, 3), (2, 1)], 2: [(1, 2), (2, 3)], 3: [(2, 3), (1, 2)], 4: [(1, 2), (3, 2)]}[kind]
+    wires = [(a_, b_, a, nhalf) for a_, b_ in w]
+    p = os.path.join(d, f"two_wire_{kind}.geo")
+    G.write_geo(p, nodes, wires)
+    return p
+
+
+@pytest.mark.parametrize("kind", [1, 2, 3, 4])
+def test_junction_connections_reduce_to_single_wire(oracle, geometries, geo_dir, kind, wm):
+    """Each junction kind (Configuration.cpp connections 1-4) joining two collinear
+    wires gives the same Zin and the same current magnitudes along the wire as
+    one straight wire with the same segmentation: the junction basis must span
+    the joint, with either orientation (Zin = V/I does not depend on it)."""
+    L, nh = 8.0, 64  # node steps L/(2 nh) = 1/16 m are exact in float32
+    single = oracle.load(_dipole(geometries, geo_dir, 2 * nh, L, 0.001))
+    two = oracle.load(_two_wire_dipole(geo_dir, kind, nh, L, 0.001))
+    assert two.npls == single.npls == 2 * nh - 1
+    pk1 = single.z[np.abs(single.ipc[:single.npls]) - 1]
+    pk2 = two.z[np.abs(two.ipc[:two.npls]) - 1]
+    order1, order2 = np.argsort(pk1), np.argsort(pk2)
+    assert np.array_equal(pk1[order1], pk2[order2])
+    for zf in (-1.5, 0.0, 2.25):  # feed positions (peak-node z), 0.0 is the junction basis
+        g1 = geometries.nearest_basis(single, (0, 0, zf))
+        g2 = geometries.nearest_basis(two, (0, 0, zf))
+        z1, I1, _ = oracle.solve_freq(single, 30.0, gaps=(g1,))
+        z2, I2, _ = oracle.solve_freq(two, 30.0, gaps=(g2,))
+        assert abs(z1[0] - z2[0]) <= 1e-9 * abs(z1[0]), (kind, zf, z1, z2)
+        a1, a2 = np.abs(I1[order1, 0]), np.abs(I2[order2, 0])
+        assert np.abs(a1 - a2).max() <= 1e-9 * a1.max(), (kind, zf)

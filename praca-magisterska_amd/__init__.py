@@ -394,7 +394,9 @@ def LU_cuBLAS(Z: np.ndarray, b: np.ndarray) -> np.ndarray:
     cz = np.ascontiguousarray(np.asarray(Z, np.complex128).T)  # column-major
     B = np.asarray(b, np.complex128)
     nr = 1 if B.ndim == 1 else B.shape[1]
-    ci = np.ascontiguousarray(B.reshape(n, nr).T)
+    # ci is the in/out buffer of the reference contract: always a fresh copy,
+    # never a view of the caller's array
+    ci = np.array(B.reshape(n, nr).T, dtype=np.complex128, order="C", copy=True)
     cu = np.zeros_like(ci)
     lib().LU_cuBLAS(n, nr, ci.ctypes.data, cu.ctypes.data, cz.ctypes.data, 1, 0)
     x = ci.T
@@ -449,12 +451,13 @@ class Server:
         return q.value, r.value, c.value, used.value
 
     def result(self, rid: int):
-        nf, nr, n = self.meta.pop(rid)
+        nf, nr, n = self.meta[rid]
         I = np.zeros((nf, nr, n), np.complex128)
         zin = np.zeros((nf, nr), np.complex128)
         st = (FreqStats * nf)()
         lat = C.c_double()
         _check(lib().wm_server_result(self.h, rid, I.ctypes.data, zin.ctypes.data, st, C.byref(lat)))
+        del self.meta[rid]
         return zin, I, [dict(iters=s.iters, info=s.info, fell_back=s.fell_back, dx_rel=s.dx_rel) for s in st], lat.value
 
     def close(self):

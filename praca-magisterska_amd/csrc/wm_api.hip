@@ -53,11 +53,38 @@ void DevBuf::release()
     bytes = 0;
 }
 
+int copy_h2d(void *dst, const void *src, size_t bytes, hipStream_t s)
+{
+    WM_HIP_TRY(hipStreamSynchronize(s));
+    WM_HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return WM_OK;
+}
+
+int copy_d2h(void *dst, const void *src, size_t bytes, hipStream_t s)
+{
+    WM_HIP_TRY(hipStreamSynchronize(s));
+    WM_HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return WM_OK;
+}
+
+int copy2d_h2d(void *dst, size_t dpitch, const void *src, size_t spitch, size_t width, size_t height, hipStream_t s)
+{
+    WM_HIP_TRY(hipStreamSynchronize(s));
+    WM_HIP_TRY(hipMemcpy2D(dst, dpitch, src, spitch, width, height, hipMemcpyHostToDevice));
+    return WM_OK;
+}
+
+int copy2d_d2h(void *dst, size_t dpitch, const void *src, size_t spitch, size_t width, size_t height, hipStream_t s)
+{
+    WM_HIP_TRY(hipStreamSynchronize(s));
+    WM_HIP_TRY(hipMemcpy2D(dst, dpitch, src, spitch, width, height, hipMemcpyDeviceToHost));
+    return WM_OK;
+}
+
 int check_device_error(wm_ctx *ctx, hipStream_t s)
 {
     int h = 0;
-    WM_HIP_TRY(hipMemcpyAsync(&h, ctx->d_err, sizeof(int), hipMemcpyDeviceToHost, s));
-    WM_HIP_TRY(hipStreamSynchronize(s));
+    WM_TRY(copy_d2h(&h, ctx->d_err, sizeof(int), s));
     if (h != 0) {
         const int zero = 0;
         (void)hipMemcpy(ctx->d_err, &zero, sizeof(int), hipMemcpyHostToDevice);
@@ -194,6 +221,7 @@ int wm_shutdown(wm_ctx *c)
     c->ws_fill.release();
     c->ws_lu.release();
     c->ws_solve.release();
+    c->ws_flags.release();
     c->ws_resid.release();
     c->ws_sweep.release();
     (void)hipFree(c->d_sync);
@@ -243,17 +271,13 @@ int wm_dev_free(wm_ctx *c, void *ptr)
 int wm_memcpy_h2d(wm_ctx *c, void *dst, const void *src, size_t bytes)
 {
     if (!c) return WM_ERR_ARG;
-    WM_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
-    WM_HIP_TRY(hipStreamSynchronize(c->stream));
-    return WM_OK;
+    return copy_h2d(dst, src, bytes, c->stream);
 }
 
 int wm_memcpy_d2h(wm_ctx *c, void *dst, const void *src, size_t bytes)
 {
     if (!c) return WM_ERR_ARG;
-    WM_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
-    WM_HIP_TRY(hipStreamSynchronize(c->stream));
-    return WM_OK;
+    return copy_d2h(dst, src, bytes, c->stream);
 }
 
 // ---- geometry -------------------------------------------------------------------
@@ -407,8 +431,8 @@ int wm_getrf_batched(wm_ctx *c, int prec, int n, void *A, int lda, int64_t strid
     WM_TRY(dinfo.alloc(sizeof(int) * std::max(batch, 1)));
     WM_TRY(getrf_batched(c, prec, n, A, lda, strideA, ipiv, strideP, batch, (int *)dinfo.p, c->stream));
     std::vector<int> h(batch);
-    WM_HIP_TRY(hipMemcpyAsync(h.data(), dinfo.p, sizeof(int) * batch, hipMemcpyDeviceToHost, c->stream));
     WM_TRY(check_device_error(c, c->stream));
+    WM_TRY(copy_d2h(h.data(), dinfo.p, sizeof(int) * batch, c->stream));
     int first = 0;
     for (int b = 0; b < batch; ++b) {
         if (info) info[b] = h[b];
@@ -572,10 +596,8 @@ static int fill_to_host(wm_c128 *cz, float *x, float *y, float *z, float *rad, i
     DevMem Zd;
     WM_TRY(Zd.alloc((size_t)g->ld * g->ld * sizeof(double2)));
     WM_TRY(fill_launch(c, g, 1, &ka, Zd.p, 0, nullptr, 0, c->stream));
-    WM_HIP_TRY(hipMemcpy2DAsync(cz, (size_t)npls * sizeof(wm_c128), Zd.p, (size_t)g->ld * sizeof(double2),
-                                (size_t)npls * sizeof(wm_c128), npls, hipMemcpyDeviceToHost, c->stream));
-    WM_HIP_TRY(hipStreamSynchronize(c->stream));
-    return WM_OK;
+    return copy2d_d2h(cz, (size_t)npls * sizeof(wm_c128), Zd.p, (size_t)g->ld * sizeof(double2),
+                      (size_t)npls * sizeof(wm_c128), npls, c->stream);
 }
 
 void zmatrix(wm_c128 *cz, float *x, float *y, float *z, int *ipc, int *ips, int *ise, float *rad, int npls, double ka,
@@ -614,22 +636,22 @@ static int lu_solve_host(int n, int nrhs, wm_c128 *ci, const wm_c128 *cz)
     WM_TRY(B.alloc((size_t)ld * nrhs * sizeof(double2)));
     WM_TRY(P.alloc((size_t)ld * sizeof(int)));
     WM_TRY(Inf.alloc(sizeof(int)));
-    WM_HIP_TRY(hipMemcpy2DAsync(A.p, (size_t)ld * sizeof(double2), cz, (size_t)n * sizeof(wm_c128),
-                                (size_t)n * sizeof(wm_c128), n, hipMemcpyHostToDevice, s));
+    WM_TRY(copy2d_h2d(A.p, (size_t)ld * sizeof(double2), cz, (size_t)n * sizeof(wm_c128),
+                      (size_t)n * sizeof(wm_c128), n, s));
     const int64_t tot = (int64_t)ld * ld;
     hipLaunchKernelGGL(k_pad_identity, dim3((unsigned)((tot + 255) / 256), 1), dim3(256), 0, s, (double2 *)A.p, ld,
                        (int64_t)0, n, ld);
     hipLaunchKernelGGL(k_zero_c128, dim3((unsigned)((ld * nrhs + 255) / 256)), dim3(256), 0, s, (double2 *)B.p,
                        (int64_t)ld * nrhs);
-    WM_HIP_TRY(hipMemcpy2DAsync(B.p, (size_t)ld * sizeof(double2), ci, (size_t)n * sizeof(wm_c128),
-                                (size_t)n * sizeof(wm_c128), nrhs, hipMemcpyHostToDevice, s));
+    WM_TRY(copy2d_h2d(B.p, (size_t)ld * sizeof(double2), ci, (size_t)n * sizeof(wm_c128),
+                      (size_t)n * sizeof(wm_c128), nrhs, s));
     WM_TRY(getrf_batched(c, WM_PREC_F64, ld, A.p, ld, tot, (int *)P.p, ld, 1, (int *)Inf.p, s));
     WM_TRY(getrs_batched(c, WM_PREC_F64, ld, nrhs, A.p, ld, tot, (int *)P.p, ld, B.p, ld, (int64_t)ld * nrhs, 1, s));
     int info = 0;
-    WM_HIP_TRY(hipMemcpyAsync(&info, Inf.p, sizeof(int), hipMemcpyDeviceToHost, s));
-    WM_HIP_TRY(hipMemcpy2DAsync(ci, (size_t)n * sizeof(wm_c128), B.p, (size_t)ld * sizeof(double2),
-                                (size_t)n * sizeof(wm_c128), nrhs, hipMemcpyDeviceToHost, s));
     WM_TRY(check_device_error(c, s));
+    WM_TRY(copy_d2h(&info, Inf.p, sizeof(int), s));
+    WM_TRY(copy2d_d2h(ci, (size_t)n * sizeof(wm_c128), B.p, (size_t)ld * sizeof(double2),
+                      (size_t)n * sizeof(wm_c128), nrhs, s));
     return info;
 }
 

@@ -178,8 +178,8 @@ int solve_items(wm_ctx *c, const wm_solve_opts &o, WorkItem *items, int count, h
             hval[(size_t)b * nr + r] = e;
         }
     }
-    WM_HIP_TRY(hipMemcpyAsync(gidx, hidx.data(), hidx.size() * sizeof(int), hipMemcpyHostToDevice, s));
-    WM_HIP_TRY(hipMemcpyAsync(gval, hval.data(), hval.size() * sizeof(double2), hipMemcpyHostToDevice, s));
+    WM_TRY(copy_h2d(gidx, hidx.data(), hidx.size() * sizeof(int), s));
+    WM_TRY(copy_h2d(gval, hval.data(), hval.size() * sizeof(double2), s));
     const int64_t nv = sV * count;
     hipLaunchKernelGGL(k_zero2, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, s, B, nv);
     hipLaunchKernelGGL(k_set_rhs, dim3((count * nr + 255) / 256), dim3(256), 0, s, B, sV, n, nr, gidx, gval, count);
@@ -201,11 +201,12 @@ int solve_items(wm_ctx *c, const wm_solve_opts &o, WorkItem *items, int count, h
         WM_TRY(getrf_batched(c, WM_PREC_F64, n, Z, n, sZ, ipiv, n, count, dinfo, s));
         WM_HIP_TRY(hipMemcpyAsync(X, B, szV, hipMemcpyDeviceToDevice, s));
         WM_TRY(getrs_batched(c, WM_PREC_F64, n, nr, Z, n, sZ, ipiv, n, X, n, sV, count, s));
-        WM_HIP_TRY(hipMemcpyAsync(hinfo.data(), dinfo, sizeof(int) * count, hipMemcpyDeviceToHost, s));
         WM_TRY(check_device_error(c, s));
+        WM_TRY(copy_d2h(hinfo.data(), dinfo, sizeof(int) * count, s));
     } else {
         WM_TRY(getrf_batched(c, WM_PREC_F32, n, Z32, n, sZ, ipiv, n, count, dinfo, s));
-        WM_HIP_TRY(hipMemcpyAsync(hinfo.data(), dinfo, sizeof(int) * count, hipMemcpyDeviceToHost, s));
+        WM_TRY(check_device_error(c, s));
+        WM_TRY(copy_d2h(hinfo.data(), dinfo, sizeof(int) * count, s));
         hipLaunchKernelGGL(k_zero2, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, s, X, nv);
         WM_HIP_TRY(hipMemcpyAsync(R, B, szV, hipMemcpyDeviceToDevice, s));
         WM_TRY(check_device_error(c, s));
@@ -217,15 +218,15 @@ int solve_items(wm_ctx *c, const wm_solve_opts &o, WorkItem *items, int count, h
         std::vector<int> conv(count, 0);
         for (int it = 0; it < o.max_iter; ++it) {
             if (std::none_of(active.begin(), active.end(), [](int a) { return a != 0; })) break;
-            WM_HIP_TRY(hipMemcpyAsync(dact, active.data(), sizeof(int) * count, hipMemcpyHostToDevice, s));
+            WM_TRY(copy_h2d(dact, active.data(), sizeof(int) * count, s));
             WM_HIP_TRY(hipMemcpyAsync(D, R, szV, hipMemcpyDeviceToDevice, s));
             WM_TRY(getrs_batched(c, WM_PREC_F32, n, nr, Z32, n, sZ, ipiv, n, D, n, sV, count, s));
             hipLaunchKernelGGL(k_zero_u64, dim3((2 * count * nr + 255) / 256), dim3(256), 0, s, mx, 2 * count * nr);
             hipLaunchKernelGGL(k_update, dim3((n + 255) / 256, count * nr), dim3(256), 0, s, X, D, n, n, sV, nr, dact,
                                mx);
             WM_TRY(residual_batched(c, n, nr, Z, n, sZ, X, B, n, sV, R, count, s));
-            WM_HIP_TRY(hipMemcpyAsync(hm.data(), mx, hm.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
             WM_TRY(check_device_error(c, s));
+            WM_TRY(copy_d2h(hm.data(), mx, hm.size() * sizeof(unsigned long long), s));
             for (int b = 0; b < count; ++b) {
                 if (!active[b]) continue;
                 double worst = 0.0;
@@ -260,15 +261,15 @@ int solve_items(wm_ctx *c, const wm_solve_opts &o, WorkItem *items, int count, h
                                       hipMemcpyDeviceToDevice, s));
             WM_TRY(getrs_batched(c, WM_PREC_F64, n, nr, Z + (size_t)b * sZ, n, sZ, ipiv + (size_t)b * n, n,
                                  X + (size_t)b * sV, n, sV, 1, s));
-            WM_HIP_TRY(hipMemcpyAsync(&hinfo[b], dinfo + b, sizeof(int), hipMemcpyDeviceToHost, s));
+            WM_TRY(copy_d2h(&hinfo[b], dinfo + b, sizeof(int), s));
             fell[b] = 1;
         }
         WM_TRY(check_device_error(c, s));
     }
 
     std::vector<double2> hX((size_t)sV * count);
-    WM_HIP_TRY(hipMemcpyAsync(hX.data(), X, szV, hipMemcpyDeviceToHost, s));
     WM_TRY(check_device_error(c, s));
+    WM_TRY(copy_d2h(hX.data(), X, szV, s));
     int status = WM_OK;
     for (int b = 0; b < count; ++b) {
         const WorkItem &w = items[b];

@@ -59,7 +59,8 @@ struct wm_ctx {
     static const int kSyncCap = 4096;
     wm::DevBuf ws_fill;     // per-frequency self-term integrals
     wm::DevBuf ws_lu;       // panel exchange buffers
-    wm::DevBuf ws_solve;    // triangular-solve permutation, flags and staging
+    wm::DevBuf ws_solve;    // triangular-solve permutation and staging
+    wm::DevBuf ws_flags;    // triangular-solve completion flags (epoch tags only)
     wm::DevBuf ws_resid;    // residual partial sums
     wm::DevBuf ws_sweep;    // matrices and vectors of a sweep batch
     int solve_epoch = 0;    // monotonic tag of triangular-solve flags
@@ -97,6 +98,14 @@ int residual_batched(wm_ctx *ctx, int n, int nrhs, const void *Z, int ldz, int64
                      hipStream_t s);
 
 int check_device_error(wm_ctx *ctx, hipStream_t s);
+
+// Copies between device memory and pageable host memory.  Both wait for the
+// stream first and copy synchronously, so they are ordered after every kernel
+// already queued on s and complete on return.
+int copy_h2d(void *dst, const void *src, size_t bytes, hipStream_t s);
+int copy_d2h(void *dst, const void *src, size_t bytes, hipStream_t s);
+int copy2d_h2d(void *dst, size_t dpitch, const void *src, size_t spitch, size_t width, size_t height, hipStream_t s);
+int copy2d_d2h(void *dst, size_t dpitch, const void *src, size_t spitch, size_t width, size_t height, hipStream_t s);
 
 // One (geometry, frequency, feeds) solve.  Items of one solve_items call
 // share the padded size ld; outputs are host pointers (may be null).

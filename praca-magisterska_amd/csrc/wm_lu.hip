@@ -375,7 +375,8 @@ __global__ __launch_bounds__(256) void k_swap_trsm(typename CT<T>::V *A, int lda
 // Workgroup tile 64 x 64, four waves each 32 x 32 = 2 x 2 MFMA 16x16 tiles.
 // MFMA operand roles (transposed problem): srcA = B^T (row jj = column of C),
 // srcB = A^T (column ii = row of C).  Lane l of a 16x16x4 f64 MFMA holds
-// srcA[l%16][l/16], srcB[l/16][l%16] and D[4*(l/16)+v][l%16].
+// srcA[l%16][l/16], srcB[l/16][l%16] and D[l/16 + 4*v][l%16] (measured on
+// gfx950; the f32 16x16 layout groups rows 4*(l/16)+v instead).
 constexpr int G64_T = 64, G64_KC = 16;
 
 __global__ __launch_bounds__(256) void k_gemm_f64(double2 *Abase, int lda, int64_t strideA, int n, int k)
@@ -442,7 +443,7 @@ __global__ __launch_bounds__(256) void k_gemm_f64(double2 *Abase, int lda, int64
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int gi = i0 + wi + ti * 16 + lane % 16;
-                const int gj = j0 + wj + tj * 16 + 4 * (lane / 16) + v;
+                const int gj = j0 + wj + tj * 16 + (lane / 16) + 4 * v;
                 if (gi < mrows && gj < mrows) {
                     double2 c = C[gi + (size_t)gj * lda];
                     c.x -= accR[tj][ti][v];
@@ -625,3 +626,64 @@ int getrf_batched(wm_ctx *ctx, int prec, int n, void *A, int lda, int64_t stride
 }
 
 }  // namespace wm
+
+// ---------------------------------------------------------------------------
+// Internal test hooks (not part of include/wiremom.h): run one step of the
+// blocked factorization on a device matrix so tests can check each kernel
+// against numpy.  A is n x n (lda = n), complex128 for prec 0, complex64 else.
+extern "C" int wmx_gemm_step(wm_ctx *ctx, int prec, int n, void *A, int k)
+{
+    using namespace wm;
+    const int m2 = n - k - NB;
+    if (m2 <= 0 || n % NB || k % NB) return WM_ERR_ARG;
+    if (prec == WM_PREC_F64) {
+        const int t = (m2 + G64_T - 1) / G64_T;
+        hipLaunchKernelGGL(k_gemm_f64, dim3(t, t, 1), dim3(256), 0, ctx->stream, (double2 *)A, n, (int64_t)0, n, k);
+    } else {
+        const int t = (m2 + G32_T - 1) / G32_T;
+        hipLaunchKernelGGL(k_gemm_f32, dim3(t, t, 1), dim3(256), 0, ctx->stream, (float2 *)A, n, (int64_t)0, n, k);
+    }
+    WM_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return WM_OK;
+}
+
+// One panel factorization at column k followed by the interchanges and U12.
+extern "C" int wmx_panel_step(wm_ctx *ctx, int prec, int n, void *A, int *ipiv, int k, int do_swap)
+{
+    using namespace wm;
+    if (n % NB || k % NB || k + NB > n) return WM_ERR_ARG;
+    int *info = nullptr;
+    WM_HIP_TRY(hipMalloc(&info, sizeof(int)));
+    WM_HIP_TRY(hipMemset(info, 0, sizeof(int)));
+    WM_HIP_TRY(hipMemset(ctx->d_sync, 0, sizeof(int)));
+    auto run = [&](auto tag, auto *Ap, int TPRv) -> int {
+        using T = decltype(tag);
+        using V = typename CT<T>::V;
+        const int RW = 256 / TPRv;
+        const int pmax = (n + RW - 1) / RW;
+        const XLayout<V> X(pmax);
+        WM_TRY(ctx->ws_lu.ensure(X.mat_bytes));
+        PanelArgs<T> pa;
+        pa.A = (V *)Ap; pa.lda = n; pa.strideA = 0; pa.n = n; pa.k = k;
+        pa.P = (n - k + RW - 1) / RW; pa.pmax = pmax;
+        pa.ipiv = ipiv; pa.strideP = 0; pa.info = info;
+        pa.cnt = ctx->d_sync; pa.err = ctx->d_err; pa.xbuf = (char *)ctx->ws_lu.ptr;
+        constexpr int CB = (sizeof(T) == 8) ? 16 : 32;
+        if (sizeof(T) == 8)
+            hipLaunchKernelGGL((k_panel<double, 2>), dim3(pa.P, 1), dim3(256), 0, ctx->stream, *(PanelArgs<double> *)&pa);
+        else
+            hipLaunchKernelGGL((k_panel<float, 1>), dim3(pa.P, 1), dim3(256), 0, ctx->stream, *(PanelArgs<float> *)&pa);
+        if (do_swap) {
+            const int ntl = (k + CB - 1) / CB, ntr = (n - k - NB + CB - 1) / CB;
+            if (ntl + ntr > 0)
+                hipLaunchKernelGGL((k_swap_trsm<T, CB>), dim3(ntl + ntr, 1), dim3(256), 0, ctx->stream, (V *)Ap, n,
+                                   (int64_t)0, n, k, pmax, (char *)ctx->ws_lu.ptr, ntl, ctx->d_sync);
+        }
+        return WM_OK;
+    };
+    int rc = (prec == WM_PREC_F64) ? run(double(0), (double2 *)A, 2) : run(float(0), (float2 *)A, 1);
+    WM_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    (void)hipFree(info);
+    if (rc != WM_OK) return rc;
+    return check_device_error(ctx, ctx->stream);
+}

@@ -263,14 +263,18 @@ void launch_trsv(const TrsvArgs &a, int batch, int r_off, bool upper, hipStream_
     else hipLaunchKernelGGL((k_trsv_lower<T, NR>), grid, dim3(256), 0, s, a, r_off);
 }
 
+// Right-hand sides handled by one pass starting at r_off: 8, 4, 2 or 1.
+inline int rhs_chunk(int left) { return left >= 8 ? 8 : left >= 4 ? 4 : left >= 2 ? 2 : 1; }
+
 template <typename T>
 void trsv_dispatch(const TrsvArgs &a, int batch, int r_off, bool upper, hipStream_t s)
 {
-    const int left = a.nrhs - r_off;
-    if (left >= 8) launch_trsv<T, 8>(a, batch, r_off, upper, s);
-    else if (left >= 4) launch_trsv<T, 4>(a, batch, r_off, upper, s);
-    else if (left >= 2) launch_trsv<T, 2>(a, batch, r_off, upper, s);
-    else launch_trsv<T, 1>(a, batch, r_off, upper, s);
+    switch (rhs_chunk(a.nrhs - r_off)) {
+        case 8: launch_trsv<T, 8>(a, batch, r_off, upper, s); break;
+        case 4: launch_trsv<T, 4>(a, batch, r_off, upper, s); break;
+        case 2: launch_trsv<T, 2>(a, batch, r_off, upper, s); break;
+        default: launch_trsv<T, 1>(a, batch, r_off, upper, s); break;
+    }
 }
 
 __global__ void k_zero_flags(int *p, int n)
@@ -290,18 +294,25 @@ int getrs_batched(wm_ctx *ctx, int prec, int n, int nrhs, const void *A, int lda
         return WM_ERR_ARG;
     }
     const int nblk = n / TB;
-    const size_t perm_bytes = (size_t)batch * n * sizeof(int);
-    const size_t flag_bytes = (size_t)2 * batch * nblk * sizeof(int);
+    const size_t perm_bytes = ((size_t)batch * n * sizeof(int) + 255) & ~(size_t)255;
     const size_t y_bytes = (size_t)batch * strideB * sizeof(double2);
-    const size_t need = perm_bytes + flag_bytes + y_bytes + 1024;
-    const bool grew = ctx->ws_solve.bytes < need;
-    WM_TRY(ctx->ws_solve.ensure(need));
+    WM_TRY(ctx->ws_solve.ensure(perm_bytes + y_bytes));
     char *base = (char *)ctx->ws_solve.ptr;
     int *perm = (int *)base;
-    int *flags = (int *)(base + perm_bytes);
-    double2 *Ytmp = (double2 *)(base + ((perm_bytes + flag_bytes + 255) & ~(size_t)255));
-    if (grew) {
-        const int nf = (int)(flag_bytes / sizeof(int));
+    double2 *Ytmp = (double2 *)(base + perm_bytes);
+    // Completion flags live in their own buffer and only ever hold epoch tags
+    // of this context, so a wait can never be satisfied by stale data.  The
+    // buffer is zeroed whenever it is (re)allocated.
+    const size_t flag_bytes = (size_t)2 * batch * nblk * sizeof(int);
+    if (ctx->ws_flags.bytes < flag_bytes || !ctx->ws_flags.ptr) {
+        WM_TRY(ctx->ws_flags.ensure(flag_bytes));
+        const int nf = (int)(ctx->ws_flags.bytes / sizeof(int));
+        hipLaunchKernelGGL(k_zero_flags, dim3((nf + 255) / 256), dim3(256), 0, s, (int *)ctx->ws_flags.ptr, nf);
+        ctx->solve_epoch = 0;
+    }
+    int *flags = (int *)ctx->ws_flags.ptr;
+    if (ctx->solve_epoch > (1 << 30)) {  // keep the tags positive: restart from a zeroed buffer
+        const int nf = (int)(ctx->ws_flags.bytes / sizeof(int));
         hipLaunchKernelGGL(k_zero_flags, dim3((nf + 255) / 256), dim3(256), 0, s, flags, nf);
         ctx->solve_epoch = 0;
     }
@@ -313,7 +324,7 @@ int getrs_batched(wm_ctx *ctx, int prec, int n, int nrhs, const void *A, int lda
     // copy B to the temporary (lower solve reads B, writes Y; upper solves Y in place)
     WM_HIP_TRY(hipMemcpy2DAsync(Ytmp, strideB * sizeof(double2), B, strideB * sizeof(double2),
                                 strideB * sizeof(double2), batch, hipMemcpyDeviceToDevice, s));
-    for (int r0 = 0; r0 < nrhs; r0 += 8) {
+    for (int r0 = 0; r0 < nrhs; r0 += rhs_chunk(nrhs - r0)) {
         TrsvArgs a;
         a.A = A; a.lda = lda; a.strideA = strideA; a.n = n; a.nrhs = nrhs;
         a.perm = perm; a.strideQ = n; a.ldb = ldb; a.strideB = strideB; a.err = ctx->d_err;
@@ -345,17 +356,17 @@ int residual_batched(wm_ctx *ctx, int n, int nrhs, const void *Z, int ldz, int64
     DevBuf &partial = ctx->ws_resid;
     const size_t need = (size_t)batch * nsplit * nrp * n * sizeof(double2);
     WM_TRY(partial.ensure(need));
-    for (int r0 = 0; r0 < nrhs; r0 += 8) {
-        const int left = nrhs - r0;
+    for (int r0 = 0; r0 < nrhs; r0 += rhs_chunk(nrhs - r0)) {
+        const int left = rhs_chunk(nrhs - r0);
         const dim3 g1(n / 64, nsplit, batch), g2((n + 255) / 256, batch);
 #define WM_RESID(NR)                                                                                        \
     hipLaunchKernelGGL(k_resid_part<NR>, g1, dim3(256), 0, s, (const double2 *)Z, ldz, strideZ, n, nrhs,    \
                        (const double2 *)X, ldb, strideB, cps, nsplit, (double2 *)partial.ptr, r0);          \
     hipLaunchKernelGGL(k_resid_final<NR>, g2, dim3(256), 0, s, (const double2 *)partial.ptr, nsplit, n, nrhs, \
                        (const double2 *)B, ldb, strideB, (double2 *)R, r0, batch)
-        if (left >= 8) { WM_RESID(8); }
-        else if (left >= 4) { WM_RESID(4); }
-        else if (left >= 2) { WM_RESID(2); }
+        if (left == 8) { WM_RESID(8); }
+        else if (left == 4) { WM_RESID(4); }
+        else if (left == 2) { WM_RESID(2); }
         else { WM_RESID(1); }
 #undef WM_RESID
     }

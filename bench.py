@@ -1,0 +1,186 @@
+"""Benchmark: freq-solves/s (fill + LU + solve) at N = 8192 on one MI355X.
+
+Workload (the lightning-protection structure): 4 down-conductors 20 m tall, a
+10 x 10 m roof mesh at 1 m pitch and a ground ring, radius 5 mm, segmented to
+N = 8192 unknowns, delta-gap feed just below the top corner, 64 frequencies
+log-spaced over 10 kHz - 10 MHz.  Synthetic geometry generated in-process.
+
+One step = one sweep over the 64 frequencies: fill, factor, solve and Zin for
+every frequency, with inputs (geometry) resident in HBM.  `value` is the
+mixed-precision path (FP32 matrix-core LU + FP64 iterative refinement, or the
+precision given by --prec); the plain FP64 GPU path is timed beside it and
+every Zin is compared with it.
+
+Also reported: the roofline of the dominant kernel class (HIP events on the
+launch stream, algorithmic flops per launch, MI355X datasheet peaks) and the
+CPU oracle timed on a bounded sample of the same workload on the host cores.
+
+    python bench.py [--steps K] [--warmup W] [--prec f32|f64] [--nfreq F] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# MI355X datasheet peaks (dense)
+PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3, "bf16x3": 2516.6 / 3.0}
+PEAK_FP64_VALU = 78.6
+PEAK_HBM_GBS = 8000.0
+REF_BEST_PUBLISHED = 0.148  # freq-solves/s at N=8192, reference's best variant (GPU fill + MKL LU, 6 procs)
+
+
+def _load():
+    import __graft_entry__ as ge
+    return ge._load_package()
+
+
+def cpu_baseline(geo: str, freq: float, threads: int):
+    """Oracle (C restatement of the reference CPU path) on a bounded sample:
+    the fill of 1024 of the 8192 columns (scaled x8) and an FP64 LU + solve of
+    a 4096 x 4096 complex system (scaled x8 for N^3)."""
+    from oracle import oracle as O
+    O.set_threads(threads)
+    t = O.load(geo)
+    ka = O.wavenumber(freq)
+    n = t.npls
+    cols = 1024
+    t0 = time.perf_counter()
+    O.zmatrix(t, ka, cols=(0, cols))
+    t_fill = (time.perf_counter() - t0) * n / cols
+    m = 4096
+    rng = np.random.RandomState(0)
+    A = rng.randn(m, m) + 1j * rng.randn(m, m)
+    t0 = time.perf_counter()
+    LU, piv, _ = O.zgetrf(A)
+    O.zgetrs(LU, piv, np.ones(m, complex))
+    t_lu = (time.perf_counter() - t0) * (n / m) ** 3
+    per = t_fill + t_lu
+    return {"value": 1.0 / per, "unit": "freq-solves/s", "cores": threads, "kind": "port",
+            "sample": f"oracle fill of {cols}/{n} columns (x{n // cols}) + zgetrf/zgetrs at N={m} (x{(n / m) ** 3:.0f}); "
+                      f"fill {t_fill:.1f} s + LU {t_lu:.1f} s per freq-solve (extrapolated)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--prec", default="f32")
+    ap.add_argument("--nfreq", type=int, default=64)
+    ap.add_argument("--batch", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported "
+              "(frequencies would shard across GPUs with no exchange, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    wm = _load()
+    import wiremom.geometries as G
+    tmp = tempfile.mkdtemp(prefix="wm_bench_")
+    geo = os.path.join(tmp, "c3.geo")
+    n_target = 8192
+    G.lightning(geo, n_target)
+    freqs = G.freqs_c3()[: a.nfreq]
+    ctx = wm.Context(0)
+    name, ncu, hbm = ctx.info()
+    g = wm.Geometry.load(ctx, geo, os.path.join(tmp, "configwire.txt"))
+    t = wm.configwire(geo, cwd=tmp)
+    gap = G.nearest_basis(t, (0.0, 0.0, 19.9))
+
+    def sweep(prec):
+        return wm.solve_sweep(g, freqs, [gap], prec, batch=a.batch)
+
+    for _ in range(a.warmup):
+        sweep(a.prec)
+    ctx.sync()
+    ctx.prof_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        zin, I, st = sweep(a.prec)
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    prof = {k: ctx.prof_get(k) for k in ("fill", "panel", "swap_trsm", "gemm", "trsv", "resid")}
+    ctx.prof_enable(False)
+    value = a.steps * len(freqs) / dt
+
+    # plain FP64 GPU path on the same sweep (the drop-in precision and the accuracy reference)
+    if a.prec != "f64":
+        t0 = time.perf_counter()
+        z64, _, _ = sweep("f64")
+        dt64 = time.perf_counter() - t0
+        fp64 = len(freqs) / dt64
+        rel = np.abs(zin[:, 0] - z64[:, 0]) / np.abs(z64[:, 0])
+    else:
+        fp64, rel = value, np.zeros(len(freqs))
+
+    # roofline of the dominant kernel class (largest event time)
+    peak_mfma = PEAK_TFLOPS.get(a.prec, 157.3)
+    dom = max(prof, key=lambda k: prof[k][0])
+    ms, nl, fl, by = prof[dom]
+    if dom in ("gemm", "panel", "swap_trsm"):
+        ach = fl / (ms * 1e-3) / 1e12
+        roof = {"bound": "mfma", "achieved": ach, "peak": peak_mfma, "unit": "TFLOP/s", "frac": ach / peak_mfma}
+    elif dom == "fill":
+        ach = fl / (ms * 1e-3) / 1e12
+        roof = {"bound": "mfma", "achieved": ach, "peak": PEAK_FP64_VALU, "unit": "TFLOP/s",
+                "frac": ach / PEAK_FP64_VALU}
+    else:
+        ach = by / (ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": ach / PEAK_HBM_GBS}
+    roof.update({"kernel": dom, "launches": nl, "avg_launch_us": 1e3 * ms / max(nl, 1), "traffic": None})
+
+    # whole-solve roofline fraction: t_ideal / t_measured per freq-solve
+    N = g.ld
+    iters = float(np.mean([s["iters"] for s in st]))
+    t_ideal = (8 / 3 * N ** 3 / (peak_mfma * 1e12) + 220 * N ** 2 / (PEAK_FP64_VALU * 1e12)
+               + (16 * N ** 2 + iters * 24 * N ** 2) / (PEAK_HBM_GBS * 1e9))
+    solve_frac = t_ideal / (dt / (a.steps * len(freqs)))
+
+    cpu = None
+    if not a.no_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        cpu = cpu_baseline(geo, float(freqs[-1]), threads)
+
+    out = {
+        "metric": "freq-solves/s (fill+LU) at N=8192",
+        "value": value,
+        "unit": "freq-solves/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * dt / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"f32": "c64 LU + f64 IR", "f64": "c128", "bf16x3": "bf16x3 LU + f64 IR"}.get(a.prec, a.prec),
+        "data": "synthetic lightning-protection geometry (generated), 64 log-spaced frequencies",
+        "config": {"workload": "lightning structure N=8192, 64 freqs 10 kHz-10 MHz, 1 delta-gap feed",
+                   "N": int(g.npls), "ld": int(g.ld), "nfreq": len(freqs), "prec": a.prec,
+                   "parallelism": "1 GPU, frequencies batched"},
+        "fp64_solves_per_s": fp64,
+        "speedup_vs_fp64": value / fp64,
+        "zin_rel_err_vs_fp64_max": float(rel.max()),
+        "ir_iters_mean": iters,
+        "fell_back": int(sum(s["fell_back"] for s in st)),
+        "solve_roofline_frac": solve_frac,
+        "kernel_ms": {k: round(v[0] / a.steps, 3) for k, v in prof.items()},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "reference_best_published_solves_per_s": REF_BEST_PUBLISHED,
+        "device": f"{name} x{ncu} CU",
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -66,6 +66,15 @@ int wm_sync(wm_ctx *ctx);
 /* Device name, CU count and HBM bytes. */
 int wm_device_info(wm_ctx *ctx, char *name, int name_len, int *num_cu, size_t *hbm_bytes);
 
+/* Profiler: when enabled, every launch of the timed kernel classes ("fill",
+ * "panel", "swap_trsm", "gemm", "trsv", "resid") is bracketed by HIP events on
+ * its own stream, and its algorithmic flops and bytes are recorded.
+ * wm_prof_enable(ctx, 1) also clears earlier records.  wm_prof_get returns
+ * the summed event time (ms), the launch count, and the summed algorithmic
+ * flops and bytes of one class. */
+int wm_prof_enable(wm_ctx *ctx, int on);
+int wm_prof_get(wm_ctx *ctx, const char *kernel, double *ms, int64_t *launches, double *flops, double *bytes);
+
 /* Device memory helpers so that callers need no HIP headers. */
 int wm_dev_alloc(wm_ctx *ctx, size_t bytes, void **ptr);
 int wm_dev_free(wm_ctx *ctx, void *ptr);

@@ -28,7 +28,8 @@ NNMAX, NTMAX, NWMAX, NPWMAX = 3000, 5000, 10000, 20480
 NSMAX = NWMAX + NPWMAX
 
 EXPORTS = [
-    "wm_last_error", "wm_version", "wm_init", "wm_shutdown", "wm_sync", "wm_device_info", "wm_dev_alloc",
+    "wm_last_error", "wm_version", "wm_init", "wm_shutdown", "wm_sync", "wm_device_info", "wm_prof_enable",
+    "wm_prof_get", "wm_dev_alloc",
     "wm_dev_free", "wm_memcpy_h2d", "wm_memcpy_d2h", "wm_geom_create", "wm_geom_load", "wm_geom_destroy",
     "wm_geom_npls", "wm_geom_ld", "wm_zfill", "wm_getrf_batched", "wm_getrs_batched", "wm_solve_opts_default",
     "wm_solve_sweep", "wm_server_opts_default", "wm_server_create", "wm_server_submit", "wm_server_step",
@@ -77,6 +78,8 @@ def lib() -> C.CDLL:
         L.wm_init.argtypes = [C.c_int, _p(vp)]
         L.wm_shutdown.argtypes = [vp]
         L.wm_sync.argtypes = [vp]
+        L.wm_prof_enable.argtypes = [vp, C.c_int]
+        L.wm_prof_get.argtypes = [vp, C.c_char_p, dp, _p(C.c_int64), dp, dp]
         L.wm_device_info.argtypes = [vp, C.c_char_p, C.c_int, ip, _p(C.c_size_t)]
         L.wm_dev_alloc.argtypes = [vp, C.c_size_t, _p(vp)]
         L.wm_dev_free.argtypes = [vp, vp]
@@ -236,6 +239,16 @@ class Context:
 
     def sync(self):
         _check(lib().wm_sync(self.h))
+
+    def prof_enable(self, on: bool = True):
+        _check(lib().wm_prof_enable(self.h, int(on)))
+
+    def prof_get(self, kernel: str):
+        """(ms, launches, algorithmic flops, algorithmic bytes) of one kernel class."""
+        ms, fl, by = C.c_double(), C.c_double(), C.c_double()
+        n = C.c_int64()
+        _check(lib().wm_prof_get(self.h, kernel.encode(), C.byref(ms), C.byref(n), C.byref(fl), C.byref(by)))
+        return ms.value, n.value, fl.value, by.value
 
     def info(self):
         name = C.create_string_buffer(256)

@@ -81,6 +81,39 @@ int copy2d_d2h(void *dst, size_t dpitch, const void *src, size_t spitch, size_t 
     return WM_OK;
 }
 
+static hipEvent_t prof_event(wm_ctx *ctx)
+{
+    if (!ctx->prof.pool.empty()) {
+        hipEvent_t e = ctx->prof.pool.back();
+        ctx->prof.pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+void prof_begin(wm_ctx *ctx, int k, hipStream_t s)
+{
+    if (!ctx->prof.on) return;
+    ProfRec r;
+    r.a = prof_event(ctx);
+    r.b = nullptr;
+    r.flops = r.bytes = 0;
+    (void)hipEventRecord(r.a, s);
+    ctx->prof.rec[k].push_back(r);
+}
+
+void prof_end(wm_ctx *ctx, int k, hipStream_t s, double flops, double bytes)
+{
+    if (!ctx->prof.on || ctx->prof.rec[k].empty()) return;
+    ProfRec &r = ctx->prof.rec[k].back();
+    r.b = prof_event(ctx);
+    r.flops = flops;
+    r.bytes = bytes;
+    (void)hipEventRecord(r.b, s);
+}
+
 int check_device_error(wm_ctx *ctx, hipStream_t s)
 {
     int h = 0;
@@ -230,6 +263,52 @@ int wm_shutdown(wm_ctx *c)
     if (c == g_default_ctx) g_default_ctx = nullptr;
     delete c;
     return WM_OK;
+}
+
+static const char *kClassNames[K_NCLASS] = {"fill", "panel", "swap_trsm", "gemm", "trsv", "resid"};
+
+int wm_prof_enable(wm_ctx *c, int on)
+{
+    if (!c) return WM_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    WM_HIP_TRY(hipStreamSynchronize(c->stream));
+    for (auto &v : c->prof.rec) {
+        for (auto &r : v) {
+            if (r.a) c->prof.pool.push_back(r.a);
+            if (r.b) c->prof.pool.push_back(r.b);
+        }
+        v.clear();
+    }
+    c->prof.on = on != 0;
+    return WM_OK;
+}
+
+int wm_prof_get(wm_ctx *c, const char *kernel, double *ms, int64_t *launches, double *flops, double *bytes)
+{
+    if (!c || !kernel) return WM_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    WM_HIP_TRY(hipStreamSynchronize(c->stream));
+    for (int k = 0; k < K_NCLASS; ++k) {
+        if (std::strcmp(kernel, kClassNames[k]) != 0) continue;
+        double t = 0, f = 0, b = 0;
+        int64_t n = 0;
+        for (auto &r : c->prof.rec[k]) {
+            if (!r.b) continue;
+            float e = 0;
+            WM_HIP_TRY(hipEventElapsedTime(&e, r.a, r.b));
+            t += e;
+            f += r.flops;
+            b += r.bytes;
+            ++n;
+        }
+        if (ms) *ms = t;
+        if (launches) *launches = n;
+        if (flops) *flops = f;
+        if (bytes) *bytes = b;
+        return WM_OK;
+    }
+    set_error("wm_prof_get: unknown kernel class %s", kernel);
+    return WM_ERR_ARG;
 }
 
 int wm_sync(wm_ctx *c)

@@ -250,12 +250,16 @@ int fill_launch(wm_ctx *ctx, const wm_geom *g, int nf, const double *ka, void *Z
                            fa, self);
         double2 *Zf = (double2 *)Z + (size_t)f0 * strideZ;
         float2 *Z32f = Z32 ? (float2 *)Z32 + (size_t)f0 * strideZ32 : nullptr;
+        // algorithmic count per frequency: 220 N^2 FP64 operations, 16 N^2 bytes written
+        const double nn = (double)g->npls * g->npls;
+        prof_begin(ctx, K_FILL, s);
         switch (F) {
             case 1: launch_fill_f<1>(grid, s, g, self, fa, Zf, strideZ, Z32f, strideZ32); break;
             case 2: launch_fill_f<2>(grid, s, g, self, fa, Zf, strideZ, Z32f, strideZ32); break;
             case 3: launch_fill_f<3>(grid, s, g, self, fa, Zf, strideZ, Z32f, strideZ32); break;
             default: launch_fill_f<4>(grid, s, g, self, fa, Zf, strideZ, Z32f, strideZ32); break;
         }
+        prof_end(ctx, K_FILL, s, 220.0 * nn * F, (Z32 ? 24.0 : 16.0) * nn * F);
         WM_HIP_TRY(hipGetLastError());
     }
     return WM_OK;

@@ -45,6 +45,20 @@ enum DevErr : int { DEV_OK = 0, DEV_SPIN_TIMEOUT = 1 };
 
 }  // namespace wm
 
+namespace wm {
+// Kernel classes timed by the optional profiler (events on the launch stream).
+enum KClass { K_FILL = 0, K_PANEL, K_SWAP, K_GEMM, K_TRSV, K_RESID, K_NCLASS };
+struct ProfRec {
+    hipEvent_t a, b;
+    double flops, bytes;
+};
+struct Prof {
+    bool on = false;
+    std::vector<ProfRec> rec[K_NCLASS];
+    std::vector<hipEvent_t> pool;
+};
+}  // namespace wm
+
 struct wm_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -64,6 +78,7 @@ struct wm_ctx {
     wm::DevBuf ws_resid;    // residual partial sums
     wm::DevBuf ws_sweep;    // matrices and vectors of a sweep batch
     int solve_epoch = 0;    // monotonic tag of triangular-solve flags
+    wm::Prof prof;          // per-kernel-class event timing (off by default)
 };
 
 // Geometry as the fill kernel reads it.  Observation data per basis (SoA,
@@ -98,6 +113,11 @@ int residual_batched(wm_ctx *ctx, int n, int nrhs, const void *Z, int ldz, int64
                      hipStream_t s);
 
 int check_device_error(wm_ctx *ctx, hipStream_t s);
+
+// Profiler: bracket one launch of class k on stream s with events and record
+// its algorithmic flops and bytes.  No-ops unless ctx->prof.on.
+void prof_begin(wm_ctx *ctx, int k, hipStream_t s);
+void prof_end(wm_ctx *ctx, int k, hipStream_t s, double flops, double bytes);
 
 // Copies between device memory and pageable host memory.  Both wait for the
 // stream first and copy synchronously, so they are ordered after every kernel

@@ -586,21 +586,31 @@ int getrf_impl(wm_ctx *ctx, int n, typename CT<T>::V *A, int lda, int64_t stride
             pa.P = (n - k + RW - 1) / RW; pa.pmax = pmax;
             pa.ipiv = ipb; pa.strideP = strideP; pa.info = infob;
             pa.cnt = ctx->d_sync; pa.err = ctx->d_err; pa.xbuf = xbuf;
+            const double mr = n - k, m2 = n - k - NB, esz = 2.0 * sizeof(T);
+            // panel: per column j, (mr-j) scalings and (mr-j)(NB-j-1) complex multiply-adds
+            prof_begin(ctx, K_PANEL, s);
             hipLaunchKernelGGL((k_panel<T, TPR>), dim3(pa.P, nb_), dim3(256), 0, s, pa);
+            prof_end(ctx, K_PANEL, s, nb_ * 8.0 * (mr * NB * NB / 2.0 - (double)NB * NB * NB / 6.0),
+                     nb_ * 2.0 * mr * NB * esz);
             const int ntl = (k + CB - 1) / CB;
             const int ntr = (n - k - NB + CB - 1) / CB;
-            if (ntl + ntr > 0)
+            if (ntl + ntr > 0) {
+                prof_begin(ctx, K_SWAP, s);
                 hipLaunchKernelGGL((k_swap_trsm<T, CB>), dim3(ntl + ntr, nb_), dim3(256), 0, s, Ab, lda, strideA, n,
                                    k, pmax, xbuf, ntl, ctx->d_sync);
-            const int m2 = n - k - NB;
+                prof_end(ctx, K_SWAP, s, nb_ * 4.0 * NB * NB * m2, nb_ * 4.0 * NB * n * esz);
+            }
             if (m2 > 0) {
+                // trailing update: 8 M^2 NB real flops; C read and written, L21 and U12 read once
+                prof_begin(ctx, K_GEMM, s);
                 if (sizeof(T) == 8) {
-                    const int t = (m2 + G64_T - 1) / G64_T;
+                    const int t = (int)(m2 + G64_T - 1) / G64_T;
                     hipLaunchKernelGGL(k_gemm_f64, dim3(t, t, nb_), dim3(256), 0, s, (double2 *)Ab, lda, strideA, n, k);
                 } else {
-                    const int t = (m2 + G32_T - 1) / G32_T;
+                    const int t = (int)(m2 + G32_T - 1) / G32_T;
                     hipLaunchKernelGGL(k_gemm_f32, dim3(t, t, nb_), dim3(256), 0, s, (float2 *)Ab, lda, strideA, n, k);
                 }
+                prof_end(ctx, K_GEMM, s, nb_ * 8.0 * m2 * m2 * NB, nb_ * (2.0 * m2 * m2 + 2.0 * m2 * NB) * esz);
             }
         }
         WM_HIP_TRY(hipGetLastError());

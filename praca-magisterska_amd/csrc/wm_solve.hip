@@ -331,11 +331,18 @@ int getrs_batched(wm_ctx *ctx, int prec, int n, int nrhs, const void *A, int lda
         // lower: Bin = Ytmp (copy of B), output into B
         a.Bin = Ytmp; a.Y = (double2 *)B; a.flags = flags;
         a.epoch = ++ctx->solve_epoch;
+        const int nr = rhs_chunk(nrhs - r0);
+        const double esz = prec == WM_PREC_F64 ? 16.0 : 8.0;
+        const double fl = batch * 4.0 * (double)n * n * nr, by = batch * 0.5 * (double)n * n * esz;
+        prof_begin(ctx, K_TRSV, s);
         if (prec == WM_PREC_F64) trsv_dispatch<double>(a, batch, r0, false, s);
         else trsv_dispatch<float>(a, batch, r0, false, s);
+        prof_end(ctx, K_TRSV, s, fl, by);
         a.Bin = nullptr; a.flags = flags + (size_t)batch * nblk;
+        prof_begin(ctx, K_TRSV, s);
         if (prec == WM_PREC_F64) trsv_dispatch<double>(a, batch, r0, true, s);
         else trsv_dispatch<float>(a, batch, r0, true, s);
+        prof_end(ctx, K_TRSV, s, fl, by);
     }
     WM_HIP_TRY(hipGetLastError());
     return WM_OK;
@@ -364,10 +371,12 @@ int residual_batched(wm_ctx *ctx, int n, int nrhs, const void *Z, int ldz, int64
                        (const double2 *)X, ldb, strideB, cps, nsplit, (double2 *)partial.ptr, r0);          \
     hipLaunchKernelGGL(k_resid_final<NR>, g2, dim3(256), 0, s, (const double2 *)partial.ptr, nsplit, n, nrhs, \
                        (const double2 *)B, ldb, strideB, (double2 *)R, r0, batch)
+        prof_begin(ctx, K_RESID, s);
         if (left == 8) { WM_RESID(8); }
         else if (left == 4) { WM_RESID(4); }
         else if (left == 2) { WM_RESID(2); }
         else { WM_RESID(1); }
+        prof_end(ctx, K_RESID, s, batch * 8.0 * (double)n * n * left, batch * 16.0 * (double)n * n);
 #undef WM_RESID
     }
     WM_HIP_TRY(hipGetLastError());

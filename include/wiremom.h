@@ -121,8 +121,9 @@ int wm_getrs_batched(wm_ctx *ctx, int prec, int n, int nrhs, const void *A, int 
 /* ---- whole solves ------------------------------------------------------------ */
 typedef struct {
     int prec;              /* WM_PREC_* */
-    int max_iter;          /* refinement cap (mixed precision), default 30 */
-    double tol;            /* stop when ||dx||/||x|| <= tol, default 1e-14 */
+    int max_iter;          /* refinement cap (mixed precision), default 20; an item whose
+                              contraction rate predicts more is refactored in FP64 early */
+    double tol;            /* stop when ||dx||/||x|| <= tol, default 1e-12 */
     int batch;             /* frequencies in flight, 0 = auto by HBM budget */
     double hbm_fraction;   /* share of free HBM the sweep may use, default 0.8 */
     int fallback_f64;      /* refactor in FP64 if refinement fails, default 1 */

@@ -341,7 +341,7 @@ class Geometry:
 
 
 def solve_sweep(geom: Geometry, freqs_mhz, gaps, prec="f64", ampl: float = 1.0, phas: float = 0.0,
-                batch: int = 0, max_iter: int = 30, tol: float = 1e-14, fallback: bool = True):
+                batch: int = 0, max_iter: int = 20, tol: float = 1e-12, fallback: bool = True):
     """Whole solves (wm_solve_sweep).  Returns (zin[nf, nrhs], I[nf, nrhs, N], stats list)."""
     f = np.ascontiguousarray(np.atleast_1d(freqs_mhz), np.float32)
     g = np.ascontiguousarray(np.atleast_1d(gaps), np.int32)

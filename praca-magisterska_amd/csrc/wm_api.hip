@@ -255,6 +255,7 @@ int wm_shutdown(wm_ctx *c)
     c->ws_lu.release();
     c->ws_solve.release();
     c->ws_flags.release();
+    c->ws_perm.release();
     c->ws_resid.release();
     c->ws_sweep.release();
     (void)hipFree(c->d_sync);
@@ -534,8 +535,8 @@ void wm_solve_opts_default(wm_solve_opts *o)
 {
     if (!o) return;
     o->prec = WM_PREC_F64;
-    o->max_iter = 30;
-    o->tol = 1e-14;
+    o->max_iter = 20;
+    o->tol = 1e-12;
     o->batch = 0;
     o->hbm_fraction = 0.8;
     o->fallback_f64 = 1;

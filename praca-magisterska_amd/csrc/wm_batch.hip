@@ -46,13 +46,13 @@ __global__ void k_set_rhs(double2 *B, int64_t strideB, int ldb, int nr, const in
     if (i >= 0) B[(size_t)b * strideB + (size_t)r * ldb + i] = val[t];
 }
 
-// X += D for active items; max |D|, |X| per (item, rhs) as ordered bit patterns.
-__global__ void k_update(double2 *X, const double2 *D, int n, int ldb, int64_t strideB, int nr, const int *active,
+// X += D for the listed items; max |D|, |X| per (item, rhs) as ordered bit patterns.
+__global__ void k_update(double2 *X, const double2 *D, int n, int ldb, int64_t strideB, int nr, const int *bidx,
                          unsigned long long *mx)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int br = blockIdx.y, b = br / nr, r = br % nr;
-    if (!active[b]) return;
+    const int br = blockIdx.y, b = bidx[br / nr], r = br % nr;
+    const int bb = b * nr + r;
     double dm = 0.0, xm = 0.0;
     if (i < n) {
         const size_t o = (size_t)b * strideB + (size_t)r * ldb + i;
@@ -69,9 +69,18 @@ __global__ void k_update(double2 *X, const double2 *D, int n, int ldb, int64_t s
         xm = fmax(xm, __shfl_xor(xm, off));
     }
     if ((threadIdx.x & 63) == 0) {
-        atomicMax(&mx[2 * br], (unsigned long long)__double_as_longlong(dm));
-        atomicMax(&mx[2 * br + 1], (unsigned long long)__double_as_longlong(xm));
+        atomicMax(&mx[2 * bb], (unsigned long long)__double_as_longlong(dm));
+        atomicMax(&mx[2 * bb + 1], (unsigned long long)__double_as_longlong(xm));
     }
+}
+
+// Dst[b] = Src[b] for the listed items (n x nr complex128 per item).
+__global__ void k_copy_items(double2 *Dst, const double2 *Src, int64_t stride, int64_t len, const int *bidx)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= len) return;
+    const size_t o = (size_t)bidx[blockIdx.y] * stride + i;
+    Dst[o] = Src[o];
 }
 
 __global__ void k_zero_u64(unsigned long long *p, int n)
@@ -155,6 +164,7 @@ int solve_items(wm_ctx *c, const wm_solve_opts &o, WorkItem *items, int count, h
     const size_t oP = carve((size_t)n * count * sizeof(int)), oI = carve(count * sizeof(int));
     const size_t oM = carve((size_t)2 * count * nr * sizeof(unsigned long long));
     const size_t oA = carve(count * sizeof(int));
+    const size_t oQ = carve((size_t)n * count * sizeof(int));
     const size_t oG = carve((size_t)count * nr * (sizeof(int) + sizeof(double2)));
     WM_TRY(c->ws_sweep.ensure(off));
     char *base = (char *)c->ws_sweep.ptr;
@@ -163,6 +173,7 @@ int solve_items(wm_ctx *c, const wm_solve_opts &o, WorkItem *items, int count, h
     double2 *B = (double2 *)(base + oB), *X = (double2 *)(base + oX);
     double2 *D = (double2 *)(base + oD), *R = (double2 *)(base + oR);
     int *ipiv = (int *)(base + oP), *dinfo = (int *)(base + oI), *dact = (int *)(base + oA);
+    int *perm = (int *)(base + oQ);
     unsigned long long *mx = (unsigned long long *)(base + oM);
     int *gidx = (int *)(base + oG);
     double2 *gval = (double2 *)(base + oG + (size_t)count * nr * sizeof(int));
@@ -197,39 +208,41 @@ int solve_items(wm_ctx *c, const wm_solve_opts &o, WorkItem *items, int count, h
 
     std::vector<int> hinfo(count, 0), iters(count, 0), fell(count, 0);
     std::vector<double> dxr(count, 0.0);
+    auto upload_list = [&](const std::vector<int> &lst) { return copy_h2d(dact, lst.data(), lst.size() * sizeof(int), s); };
     if (!mixed) {
         WM_TRY(getrf_batched(c, WM_PREC_F64, n, Z, n, sZ, ipiv, n, count, dinfo, s));
+        WM_TRY(perm_batched(c, n, ipiv, n, perm, n, count, s));
         WM_HIP_TRY(hipMemcpyAsync(X, B, szV, hipMemcpyDeviceToDevice, s));
-        WM_TRY(getrs_batched(c, WM_PREC_F64, n, nr, Z, n, sZ, ipiv, n, X, n, sV, count, s));
+        WM_TRY(getrs_perm_batched(c, WM_PREC_F64, n, nr, Z, n, sZ, perm, n, X, n, sV, count, s));
         WM_TRY(check_device_error(c, s));
         WM_TRY(copy_d2h(hinfo.data(), dinfo, sizeof(int) * count, s));
     } else {
         WM_TRY(getrf_batched(c, WM_PREC_F32, n, Z32, n, sZ, ipiv, n, count, dinfo, s));
+        WM_TRY(perm_batched(c, n, ipiv, n, perm, n, count, s));
         WM_TRY(check_device_error(c, s));
         WM_TRY(copy_d2h(hinfo.data(), dinfo, sizeof(int) * count, s));
         hipLaunchKernelGGL(k_zero2, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, s, X, nv);
         WM_HIP_TRY(hipMemcpyAsync(R, B, szV, hipMemcpyDeviceToDevice, s));
-        WM_TRY(check_device_error(c, s));
-        std::vector<int> active(count, 1);
+        std::vector<int> active;
         for (int b = 0; b < count; ++b)
-            if (hinfo[b] != 0) active[b] = 0;  // singular in FP32: straight to the FP64 fallback
+            if (hinfo[b] == 0) active.push_back(b);  // singular in FP32: straight to the FP64 fallback
         std::vector<double> prev((size_t)count * nr, 1e300);
         std::vector<unsigned long long> hm((size_t)2 * count * nr);
         std::vector<int> conv(count, 0);
-        for (int it = 0; it < o.max_iter; ++it) {
-            if (std::none_of(active.begin(), active.end(), [](int a) { return a != 0; })) break;
-            WM_TRY(copy_h2d(dact, active.data(), sizeof(int) * count, s));
-            WM_HIP_TRY(hipMemcpyAsync(D, R, szV, hipMemcpyDeviceToDevice, s));
-            WM_TRY(getrs_batched(c, WM_PREC_F32, n, nr, Z32, n, sZ, ipiv, n, D, n, sV, count, s));
+        for (int it = 0; it < o.max_iter && !active.empty(); ++it) {
+            const int na = (int)active.size();
+            WM_TRY(upload_list(active));
+            // D = R for the active items, then D = LU32^{-1} D (FP64 arithmetic on widened factors)
+            hipLaunchKernelGGL(k_copy_items, dim3((unsigned)((sV + 255) / 256), na), dim3(256), 0, s, D, R, sV, sV, dact);
+            WM_TRY(getrs_perm_batched(c, WM_PREC_F32, n, nr, Z32, n, sZ, perm, n, D, n, sV, na, s, dact));
             hipLaunchKernelGGL(k_zero_u64, dim3((2 * count * nr + 255) / 256), dim3(256), 0, s, mx, 2 * count * nr);
-            hipLaunchKernelGGL(k_update, dim3((n + 255) / 256, count * nr), dim3(256), 0, s, X, D, n, n, sV, nr, dact,
-                               mx);
-            WM_TRY(residual_batched(c, n, nr, Z, n, sZ, X, B, n, sV, R, count, s));
+            hipLaunchKernelGGL(k_update, dim3((n + 255) / 256, na * nr), dim3(256), 0, s, X, D, n, n, sV, nr, dact, mx);
+            WM_TRY(residual_batched(c, n, nr, Z, n, sZ, X, B, n, sV, R, na, s, dact));
             WM_TRY(check_device_error(c, s));
             WM_TRY(copy_d2h(hm.data(), mx, hm.size() * sizeof(unsigned long long), s));
-            for (int b = 0; b < count; ++b) {
-                if (!active[b]) continue;
-                double worst = 0.0;
+            std::vector<int> still;
+            for (int b : active) {
+                double worst = 0.0, rate = 0.0;
                 bool stalled = true;
                 for (int r = 0; r < items[b].nrhs; ++r) {
                     double dm, xm;
@@ -238,33 +251,44 @@ int solve_items(wm_ctx *c, const wm_solve_opts &o, WorkItem *items, int count, h
                     const double rel = xm > 0 ? dm / xm : 0.0;
                     worst = std::max(worst, rel);
                     double &p = prev[(size_t)b * nr + r];
+                    rate = std::max(rate, p < 1e299 && p > 0 ? rel / p : 0.0);
                     if (!(rel > 0.5 * p && p < 1e-9)) stalled = false;
                     p = rel;
                 }
                 dxr[b] = worst;
                 iters[b] = it + 1;
                 if (worst <= o.tol || stalled) {
-                    active[b] = 0;
                     conv[b] = 1;
+                    continue;
                 }
+                // contraction too slow to reach tol within max_iter: stop now (FP64 fallback)
+                if (it >= 1) {
+                    const double need = rate > 0 && rate < 1 ? std::log(o.tol / worst) / std::log(rate) : 1e9;
+                    if (it + 1 + need > o.max_iter) continue;
+                }
+                still.push_back(b);
             }
+            active.swap(still);
         }
-        // items that did not converge (or were singular in FP32): FP64 refactorization
-        for (int b = 0; b < count; ++b) {
-            if (conv[b]) continue;
+        // items that did not converge (or were singular in FP32): one batched FP64 refactorization
+        std::vector<int> bad;
+        for (int b = 0; b < count; ++b)
+            if (!conv[b]) bad.push_back(b);
+        if (!bad.empty()) {
             if (!o.fallback_f64) {
-                set_error("refinement did not converge (item %d, ||dx||/||x|| = %.3g)", b, dxr[b]);
+                set_error("refinement did not converge for %zu of %d items", bad.size(), count);
                 return WM_ERR_NOCONV;
             }
-            WM_TRY(getrf_batched(c, WM_PREC_F64, n, Z + (size_t)b * sZ, n, sZ, ipiv + (size_t)b * n, n, 1, dinfo + b, s));
-            WM_HIP_TRY(hipMemcpyAsync(X + (size_t)b * sV, B + (size_t)b * sV, (size_t)sV * sizeof(double2),
-                                      hipMemcpyDeviceToDevice, s));
-            WM_TRY(getrs_batched(c, WM_PREC_F64, n, nr, Z + (size_t)b * sZ, n, sZ, ipiv + (size_t)b * n, n,
-                                 X + (size_t)b * sV, n, sV, 1, s));
-            WM_TRY(copy_d2h(&hinfo[b], dinfo + b, sizeof(int), s));
-            fell[b] = 1;
+            const int nb = (int)bad.size();
+            WM_TRY(upload_list(bad));
+            WM_TRY(getrf_batched(c, WM_PREC_F64, n, Z, n, sZ, ipiv, n, nb, dinfo, s, dact));
+            WM_TRY(perm_batched(c, n, ipiv, n, perm, n, nb, s, dact));
+            hipLaunchKernelGGL(k_copy_items, dim3((unsigned)((sV + 255) / 256), nb), dim3(256), 0, s, X, B, sV, sV, dact);
+            WM_TRY(getrs_perm_batched(c, WM_PREC_F64, n, nr, Z, n, sZ, perm, n, X, n, sV, nb, s, dact));
+            WM_TRY(check_device_error(c, s));
+            WM_TRY(copy_d2h(hinfo.data(), dinfo, sizeof(int) * count, s));
+            for (int b : bad) fell[b] = 1;
         }
-        WM_TRY(check_device_error(c, s));
     }
 
     std::vector<double2> hX((size_t)sV * count);

@@ -75,6 +75,7 @@ struct wm_ctx {
     wm::DevBuf ws_lu;       // panel exchange buffers
     wm::DevBuf ws_solve;    // triangular-solve permutation and staging
     wm::DevBuf ws_flags;    // triangular-solve completion flags (epoch tags only)
+    wm::DevBuf ws_perm;     // row permutations for getrs on raw pivots
     wm::DevBuf ws_resid;    // residual partial sums
     wm::DevBuf ws_sweep;    // matrices and vectors of a sweep batch
     int solve_epoch = 0;    // monotonic tag of triangular-solve flags
@@ -101,16 +102,25 @@ int fill_launch(wm_ctx *ctx, const wm_geom *g, int nf, const double *ka, void *Z
                 void *Z32, int64_t strideZ32, hipStream_t s);
 
 // LU: batched blocked right-looking factorization with partial pivoting.
+// d_bidx (device, optional): member i of the launch works on matrix d_bidx[i]
+// (the matrix of index m lives at base + m * stride); null means 0..batch-1.
 int getrf_batched(wm_ctx *ctx, int prec, int n, void *A, int lda, int64_t strideA, int *ipiv,
-                  int64_t strideP, int batch, int *d_info, hipStream_t s);
-// Triangular solves with the LU factors; B complex128.
+                  int64_t strideP, int batch, int *d_info, hipStream_t s, const int *d_bidx = nullptr);
+// Row permutation of each factorization: perm[t] = source row of position t.
+int perm_batched(wm_ctx *ctx, int n, const int *ipiv, int64_t strideP, int *perm, int64_t strideQ, int batch,
+                 hipStream_t s, const int *d_bidx = nullptr);
+// Triangular solves with the LU factors and a precomputed permutation; B complex128.
+int getrs_perm_batched(wm_ctx *ctx, int prec, int n, int nrhs, const void *A, int lda, int64_t strideA,
+                       const int *perm, int64_t strideQ, void *B, int ldb, int64_t strideB, int batch,
+                       hipStream_t s, const int *d_bidx = nullptr);
+// Same, computing the permutation from ipiv first.
 int getrs_batched(wm_ctx *ctx, int prec, int n, int nrhs, const void *A, int lda, int64_t strideA,
                   const int *ipiv, int64_t strideP, void *B, int ldb, int64_t strideB, int batch,
                   hipStream_t s);
 // R = B - Z X for complex128 Z (n x n), X, B (n x nrhs); batched.
 int residual_batched(wm_ctx *ctx, int n, int nrhs, const void *Z, int ldz, int64_t strideZ,
                      const void *X, const void *B, int ldb, int64_t strideB, void *R, int batch,
-                     hipStream_t s);
+                     hipStream_t s, const int *d_bidx = nullptr);
 
 int check_device_error(wm_ctx *ctx, hipStream_t s);
 

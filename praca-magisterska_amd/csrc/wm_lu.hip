@@ -79,6 +79,7 @@ struct PanelArgs {
     int *cnt;
     int *err;
     char *xbuf;
+    const int *bidx;  // launch member -> matrix index (null: identity)
 };
 
 // Argmax combine: larger value wins, equal values go to the lower row.
@@ -91,7 +92,8 @@ __global__ __launch_bounds__(256) void k_panel(PanelArgs<T> a)
     constexpr int W = NB / TPR;    // columns per thread
     constexpr int RPW = 64 / TPR;  // rows per wave
     constexpr int RW = 4 * RPW;    // rows per workgroup
-    const int b = blockIdx.y, w = blockIdx.x;
+    const int bl = blockIdx.y, w = blockIdx.x;          // bl: member of this launch
+    const int b = a.bidx ? a.bidx[bl] : bl;              // b: matrix index
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int chunk = (TPR == 1) ? 0 : lane / RPW;
     const int row = a.k + w * RW + wave * RPW + (lane % RPW);
@@ -99,7 +101,7 @@ __global__ __launch_bounds__(256) void k_panel(PanelArgs<T> a)
     const int k = a.k;
     V *Ab = a.A + (size_t)b * a.strideA;
     const XLayout<V> X(a.pmax);
-    char *xm = a.xbuf + (size_t)b * X.mat_bytes;
+    char *xm = a.xbuf + (size_t)bl * X.mat_bytes;
 
     T re[W], im[W];
 #pragma unroll
@@ -175,7 +177,7 @@ __global__ __launch_bounds__(256) void k_panel(PanelArgs<T> a)
             wrote = true;
         }
         if (wrote) __threadfence();
-        cluster_barrier(a.cnt + b, (j + 1) * a.P, a.err);
+        cluster_barrier(a.cnt + bl, (j + 1) * a.P, a.err);
 
         if (wave == 0) {
             double gv = -1.0;
@@ -282,17 +284,18 @@ __global__ __launch_bounds__(256) void k_panel(PanelArgs<T> a)
 // Row interchanges outside the panel and U12 = L11^{-1} A12.
 template <typename T, int CB>
 __global__ __launch_bounds__(256) void k_swap_trsm(typename CT<T>::V *A, int lda, int64_t strideA, int n, int k,
-                                                   int pmax, char *xbuf, int ntile_left, int *cnt)
+                                                   int pmax, char *xbuf, int ntile_left, int *cnt, const int *bidx)
 {
     using V = typename CT<T>::V;
     constexpr int RG = 256 / CB;        // row groups for the triangular solve
     constexpr int RPT = NB / RG;        // rows per thread in the solve
     constexpr int EPT = 2 * NB * CB / 256;  // permuted elements per thread (max)
-    const int b = blockIdx.y, tile = blockIdx.x, tid = threadIdx.x;
+    const int bl = blockIdx.y, tile = blockIdx.x, tid = threadIdx.x;
+    const int b = bidx ? bidx[bl] : bl;
     V *Ab = A + (size_t)b * strideA;
     const XLayout<V> X(pmax);
-    char *xm = xbuf + (size_t)b * X.mat_bytes;
-    if (tile == 0 && tid == 0) cnt[b] = 0;  // panel barrier counter, for the next panel
+    char *xm = xbuf + (size_t)bl * X.mat_bytes;
+    if (tile == 0 && tid == 0) cnt[bl] = 0;  // panel barrier counter, for the next panel
 
     const bool left = tile < ntile_left;
     const int c0 = left ? tile * CB : k + NB + (tile - ntile_left) * CB;
@@ -371,75 +374,127 @@ __global__ __launch_bounds__(256) void k_swap_trsm(typename CT<T>::V *A, int lda
 }
 
 // ---------------------------------------------------------------------------
-// Trailing update, complex128:  C (M x N) -= A (M x NB) * B (NB x N).
-// Workgroup tile 64 x 64, four waves each 32 x 32 = 2 x 2 MFMA 16x16 tiles.
-// MFMA operand roles (transposed problem): srcA = B^T (row jj = column of C),
-// srcB = A^T (column ii = row of C).  Lane l of a 16x16x4 f64 MFMA holds
-// srcA[l%16][l/16], srcB[l/16][l%16] and D[l/16 + 4*v][l%16] (measured on
-// gfx950; the f32 16x16 layout groups rows 4*(l/16)+v instead).
-constexpr int G64_T = 64, G64_KC = 16;
+// Trailing update  C (M x M) -= A (M x NB) * B (NB x M)  on the matrix cores.
+//
+// Both kernels issue the MFMA on the transposed problem, C^T = B^T A^T:
+// MFMA rows are columns of C and MFMA columns are rows of C, so each
+// accumulator register of a lane maps to a run of consecutive rows of one
+// column of C and the read-modify-write of C is coalesced.  Complex products
+// take four real MFMAs (re*re - im*im, re*im + im*re).  K = NB = 64 is walked
+// in chunks of 16 staged through LDS; the next chunk is fetched from global
+// memory into registers while the current one is multiplied.
+//
+// Tiles are ordered column by column and dealt to the XCDs in contiguous runs
+// (workgroup ids go to the 8 XCDs round-robin), so tiles that share a block
+// column of U12 share an L2.
 
-__global__ __launch_bounds__(256) void k_gemm_f64(double2 *Abase, int lda, int64_t strideA, int n, int k)
+// Tile of workgroup g out of T, grouped per XCD; >= T means "no tile".
+__device__ inline int xcd_tile(int g, int T)
 {
-    const int b = blockIdx.z;
+    const int per = (T + 7) / 8;
+    return (g % 8) * per + g / 8;
+}
+
+inline int xcd_grid(int T) { return 8 * ((T + 7) / 8); }
+
+// complex128: workgroup tile 128 (rows) x 64 (columns), four waves of 64 x 32,
+// v_mfma_f64_16x16x4f64.  Lane l holds srcA[l%16][l/16], srcB[l/16][l%16] and
+// D[l/16 + 4*v][l%16] (measured on gfx950).
+constexpr int GD_BM = 128, GD_BN = 64, GD_KC = 16;
+
+__global__ __launch_bounds__(256, 2) void k_gemm_f64(double2 *Abase, int lda, int64_t strideA, int n, int k,
+                                                     const int *bidx, int TI, int TJ)
+{
+    const int t = xcd_tile(blockIdx.x, TI * TJ);
+    if (t >= TI * TJ) return;
+    const int b = bidx ? bidx[blockIdx.z] : (int)blockIdx.z;
     double2 *M = Abase + (size_t)b * strideA;
     const int r0 = k + NB, mrows = n - r0;
-    const int i0 = blockIdx.x * G64_T, j0 = blockIdx.y * G64_T;
-    const double2 *A = M + r0 + (size_t)k * lda;    // L21
-    const double2 *B = M + k + (size_t)r0 * lda;    // U12
-    double2 *C = M + r0 + (size_t)r0 * lda;         // A22
+    const int i0 = (t % TI) * GD_BM, j0 = (t / TI) * GD_BN;
+    const double2 *A = M + r0 + (size_t)k * lda;  // L21
+    const double2 *B = M + k + (size_t)r0 * lda;  // U12
+    double2 *C = M + r0 + (size_t)r0 * lda;       // A22
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wi = (wave & 1) * 32, wj = (wave >> 1) * 32;
+    const int wi = (wave & 1) * 64, wj = (wave >> 1) * 32;
 
-    __shared__ double2 sA[G64_KC][G64_T + 1];  // [k][i]
-    __shared__ double2 sB[G64_KC][G64_T + 1];  // [k][j]
-    v4d accR[2][2], accI[2][2];
+    __shared__ double2 sA[GD_KC][GD_BM + 1];
+    __shared__ double2 sB[GD_KC][GD_BN + 1];
+    double2 pa[8], pb[4];
+    auto fetch = [&](int kc) {
 #pragma unroll
-    for (int x = 0; x < 2; ++x)
+        for (int e = 0; e < 8; ++e) {
+            const int idx = tid + 256 * e, i = idx % GD_BM, kk = idx / GD_BM, gi = i0 + i;
+            pa[e] = gi < mrows ? A[gi + (size_t)(kc + kk) * lda] : double2{0.0, 0.0};
+        }
 #pragma unroll
-        for (int y = 0; y < 2; ++y) { accR[x][y] = v4d{0, 0, 0, 0}; accI[x][y] = v4d{0, 0, 0, 0}; }
-
-    for (int kc = 0; kc < NB; kc += G64_KC) {
+        for (int e = 0; e < 4; ++e) {
+            const int idx = tid + 256 * e, kk = idx % GD_KC, j = idx / GD_KC, gj = j0 + j;
+            pb[e] = gj < mrows ? B[(kc + kk) + (size_t)gj * lda] : double2{0.0, 0.0};
+        }
+    };
+    auto stash = [&]() {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int idx = tid + 256 * e;
+            sA[idx / GD_BM][idx % GD_BM] = pa[e];
+        }
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             const int idx = tid + 256 * e;
-            {   // A chunk: 64 rows x 16 k, coalesced along rows
-                const int i = idx % G64_T, kk = idx / G64_T;
-                const int gi = i0 + i;
-                sA[kk][i] = gi < mrows ? A[gi + (size_t)(kc + kk) * lda] : double2{0.0, 0.0};
-            }
-            {   // B chunk: 16 k x 64 columns, coalesced along k
-                const int kk = idx % G64_KC, j = idx / G64_KC;
-                const int gj = j0 + j;
-                sB[kk][j] = gj < mrows ? B[(kc + kk) + (size_t)gj * lda] : double2{0.0, 0.0};
-            }
+            sB[idx % GD_KC][idx / GD_KC] = pb[e];
         }
-        __syncthreads();
+    };
+    v4d accR[2][4], accI[2][4];
 #pragma unroll
-        for (int ks = 0; ks < G64_KC; ks += 4) {
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y) { accR[x][y] = v4d{0, 0, 0, 0}; accI[x][y] = v4d{0, 0, 0, 0}; }
+
+    fetch(0);
+    stash();
+    __syncthreads();
+    for (int kc = 0; kc < NB; kc += GD_KC) {
+        if (kc + GD_KC < NB) fetch(kc + GD_KC);
+#pragma unroll
+        for (int ks = 0; ks < GD_KC; ks += 4) {
             const int kk = ks + lane / 16;
-            double2 bv[2], av[2];
+            double2 bv[2], av[4];
 #pragma unroll
-            for (int t = 0; t < 2; ++t) {
-                bv[t] = sB[kk][wj + t * 16 + lane % 16];
-                av[t] = sA[kk][wi + t * 16 + lane % 16];
-            }
+            for (int tj = 0; tj < 2; ++tj) bv[tj] = sB[kk][wj + tj * 16 + lane % 16];
+#pragma unroll
+            for (int ti = 0; ti < 4; ++ti) av[ti] = sA[kk][wi + ti * 16 + lane % 16];
+            // four passes over the tiles so that consecutive MFMAs never share an accumulator
 #pragma unroll
             for (int tj = 0; tj < 2; ++tj)
 #pragma unroll
-                for (int ti = 0; ti < 2; ++ti) {
+                for (int ti = 0; ti < 4; ++ti)
                     accR[tj][ti] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[tj].x, av[ti].x, accR[tj][ti], 0, 0, 0);
-                    accR[tj][ti] = __builtin_amdgcn_mfma_f64_16x16x4f64(-bv[tj].y, av[ti].y, accR[tj][ti], 0, 0, 0);
+#pragma unroll
+            for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+                for (int ti = 0; ti < 4; ++ti)
                     accI[tj][ti] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[tj].x, av[ti].y, accI[tj][ti], 0, 0, 0);
+#pragma unroll
+            for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+                for (int ti = 0; ti < 4; ++ti)
+                    accR[tj][ti] = __builtin_amdgcn_mfma_f64_16x16x4f64(-bv[tj].y, av[ti].y, accR[tj][ti], 0, 0, 0);
+#pragma unroll
+            for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+                for (int ti = 0; ti < 4; ++ti)
                     accI[tj][ti] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[tj].y, av[ti].x, accI[tj][ti], 0, 0, 0);
-                }
         }
-        __syncthreads();
+        if (kc + GD_KC < NB) {
+            __syncthreads();
+            stash();
+            __syncthreads();
+        }
     }
 #pragma unroll
     for (int tj = 0; tj < 2; ++tj)
 #pragma unroll
-        for (int ti = 0; ti < 2; ++ti)
+        for (int ti = 0; ti < 4; ++ti)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int gi = i0 + wi + ti * 16 + lane % 16;
@@ -453,25 +508,47 @@ __global__ __launch_bounds__(256) void k_gemm_f64(double2 *Abase, int lda, int64
             }
 }
 
-// Trailing update, complex64, v_mfma_f32_32x32x2f32.  Workgroup tile
-// 128 x 128, four waves each 64 x 64 = 2 x 2 MFMA 32x32 tiles.  Lane l holds
-// srcA[l%32][l/32], srcB[l/32][l%32] and D[8*(v/4)+4*(l/32)+v%4][l%32].
-constexpr int G32_T = 128, G32_KC = 16;
+// complex64: workgroup tile 128 x 128, four waves of 64 x 64 (2 x 2 MFMA
+// tiles), v_mfma_f32_32x32x2f32.  Lane l holds srcA[l%32][l/32],
+// srcB[l/32][l%32] and D[8*(v/4)+4*(l/32)+v%4][l%32].
+constexpr int GS_BM = 128, GS_BN = 128, GS_KC = 16;
 
-__global__ __launch_bounds__(256) void k_gemm_f32(float2 *Abase, int lda, int64_t strideA, int n, int k)
+__global__ __launch_bounds__(256, 2) void k_gemm_f32(float2 *Abase, int lda, int64_t strideA, int n, int k,
+                                                     const int *bidx, int TI, int TJ)
 {
-    const int b = blockIdx.z;
+    const int t = xcd_tile(blockIdx.x, TI * TJ);
+    if (t >= TI * TJ) return;
+    const int b = bidx ? bidx[blockIdx.z] : (int)blockIdx.z;
     float2 *M = Abase + (size_t)b * strideA;
     const int r0 = k + NB, mrows = n - r0;
-    const int i0 = blockIdx.x * G32_T, j0 = blockIdx.y * G32_T;
+    const int i0 = (t % TI) * GS_BM, j0 = (t / TI) * GS_BN;
     const float2 *A = M + r0 + (size_t)k * lda;
     const float2 *B = M + k + (size_t)r0 * lda;
     float2 *C = M + r0 + (size_t)r0 * lda;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wi = (wave & 1) * 64, wj = (wave >> 1) * 64;
 
-    __shared__ float2 sA[G32_KC][G32_T + 2];
-    __shared__ float2 sB[G32_KC][G32_T + 2];
+    __shared__ float2 sA[GS_KC][GS_BM + 2];
+    __shared__ float2 sB[GS_KC][GS_BN + 2];
+    float2 pa[8], pb[8];
+    // per-thread fixed offsets of the chunk elements it moves
+    const int ai = tid % GS_BM, ak = tid / GS_BM;  // A: rows ai, k = ak + 2e
+    const int bk = tid % GS_KC, bj = tid / GS_KC;  // B: k = bk, columns bj + 16e
+    const bool arow = i0 + ai < mrows;
+    auto fetch = [&](int kc) {
+        const float2 *pA = A + (i0 + ai) + (size_t)(kc + ak) * lda;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) pa[e] = arow ? pA[(size_t)(2 * e) * lda] : float2{0.f, 0.f};
+        const float2 *pB = B + (kc + bk) + (size_t)(j0 + bj) * lda;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) pb[e] = (j0 + bj + 16 * e < mrows) ? pB[(size_t)(16 * e) * lda] : float2{0.f, 0.f};
+    };
+    auto stash = [&]() {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) sA[ak + 2 * e][ai] = pa[e];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) sB[bk][bj + 16 * e] = pb[e];
+    };
     v16f accR[2][2], accI[2][2];
 #pragma unroll
     for (int x = 0; x < 2; ++x)
@@ -480,42 +557,46 @@ __global__ __launch_bounds__(256) void k_gemm_f32(float2 *Abase, int lda, int64_
 #pragma unroll
             for (int v = 0; v < 16; ++v) { accR[x][y][v] = 0.f; accI[x][y][v] = 0.f; }
 
-    for (int kc = 0; kc < NB; kc += G32_KC) {
+    fetch(0);
+    stash();
+    __syncthreads();
+    for (int kc = 0; kc < NB; kc += GS_KC) {
+        if (kc + GS_KC < NB) fetch(kc + GS_KC);
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const int idx = tid + 256 * e;
-            {
-                const int i = idx % G32_T, kk = idx / G32_T;
-                const int gi = i0 + i;
-                sA[kk][i] = gi < mrows ? A[gi + (size_t)(kc + kk) * lda] : float2{0.f, 0.f};
-            }
-            {
-                const int kk = idx % G32_KC, j = idx / G32_KC;
-                const int gj = j0 + j;
-                sB[kk][j] = gj < mrows ? B[(kc + kk) + (size_t)gj * lda] : float2{0.f, 0.f};
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int ks = 0; ks < G32_KC; ks += 2) {
+        for (int ks = 0; ks < GS_KC; ks += 2) {
             const int kk = ks + lane / 32;
             float2 bv[2], av[2];
 #pragma unroll
-            for (int t = 0; t < 2; ++t) {
-                bv[t] = sB[kk][wj + t * 32 + lane % 32];
-                av[t] = sA[kk][wi + t * 32 + lane % 32];
+            for (int q = 0; q < 2; ++q) {
+                bv[q] = sB[kk][wj + q * 32 + lane % 32];
+                av[q] = sA[kk][wi + q * 32 + lane % 32];
             }
 #pragma unroll
             for (int tj = 0; tj < 2; ++tj)
 #pragma unroll
-                for (int ti = 0; ti < 2; ++ti) {
+                for (int ti = 0; ti < 2; ++ti)
                     accR[tj][ti] = __builtin_amdgcn_mfma_f32_32x32x2f32(bv[tj].x, av[ti].x, accR[tj][ti], 0, 0, 0);
-                    accR[tj][ti] = __builtin_amdgcn_mfma_f32_32x32x2f32(-bv[tj].y, av[ti].y, accR[tj][ti], 0, 0, 0);
+#pragma unroll
+            for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+                for (int ti = 0; ti < 2; ++ti)
                     accI[tj][ti] = __builtin_amdgcn_mfma_f32_32x32x2f32(bv[tj].x, av[ti].y, accI[tj][ti], 0, 0, 0);
+#pragma unroll
+            for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+                for (int ti = 0; ti < 2; ++ti)
+                    accR[tj][ti] = __builtin_amdgcn_mfma_f32_32x32x2f32(-bv[tj].y, av[ti].y, accR[tj][ti], 0, 0, 0);
+#pragma unroll
+            for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+                for (int ti = 0; ti < 2; ++ti)
                     accI[tj][ti] = __builtin_amdgcn_mfma_f32_32x32x2f32(bv[tj].y, av[ti].x, accI[tj][ti], 0, 0, 0);
-                }
         }
-        __syncthreads();
+        if (kc + GS_KC < NB) {
+            __syncthreads();
+            stash();
+            __syncthreads();
+        }
     }
 #pragma unroll
     for (int tj = 0; tj < 2; ++tj)
@@ -534,10 +615,32 @@ __global__ __launch_bounds__(256) void k_gemm_f32(float2 *Abase, int lda, int64_
             }
 }
 
+// Launches the trailing update of step k for nb_ matrices.
+void launch_gemm(int esz, void *A, int lda, int64_t strideA, int n, int k, int nb_, const int *bidx, hipStream_t s)
+{
+    const int m2 = n - k - NB;
+    if (m2 <= 0) return;
+    if (esz == 8) {
+        const int TI = (m2 + GD_BM - 1) / GD_BM, TJ = (m2 + GD_BN - 1) / GD_BN;
+        hipLaunchKernelGGL(k_gemm_f64, dim3(xcd_grid(TI * TJ), 1, nb_), dim3(256), 0, s, (double2 *)A, lda, strideA,
+                           n, k, bidx, TI, TJ);
+    } else {
+        const int TI = (m2 + GS_BM - 1) / GS_BM, TJ = (m2 + GS_BN - 1) / GS_BN;
+        hipLaunchKernelGGL(k_gemm_f32, dim3(xcd_grid(TI * TJ), 1, nb_), dim3(256), 0, s, (float2 *)A, lda, strideA,
+                           n, k, bidx, TI, TJ);
+    }
+}
+
 __global__ void k_zero_int(int *p, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
+}
+
+__global__ void k_zero_idx(int *p, const int *idx, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[idx[i]] = 0;
 }
 
 template <typename T, int TPR>
@@ -553,7 +656,7 @@ int occupancy_panel(int *blocks_per_cu)
 
 template <typename T, int TPR>
 int getrf_impl(wm_ctx *ctx, int n, typename CT<T>::V *A, int lda, int64_t strideA, int *ipiv, int64_t strideP,
-               int batch, int *d_info, hipStream_t s)
+               int batch, int *d_info, const int *d_bidx, hipStream_t s)
 {
     using V = typename CT<T>::V;
     constexpr int RW = 256 / TPR;
@@ -576,16 +679,18 @@ int getrf_impl(wm_ctx *ctx, int n, typename CT<T>::V *A, int lda, int64_t stride
 
     for (int b0 = 0; b0 < batch; b0 += per_launch) {
         const int nb_ = (batch - b0 < per_launch) ? batch - b0 : per_launch;
-        V *Ab = A + (size_t)b0 * strideA;
-        int *ipb = ipiv + (size_t)b0 * strideP;
-        int *infob = d_info + b0;
+        // with an index table the bases stay put and members are looked up
+        V *Ab = d_bidx ? A : A + (size_t)b0 * strideA;
+        int *ipb = d_bidx ? ipiv : ipiv + (size_t)b0 * strideP;
+        int *infob = d_bidx ? d_info : d_info + b0;
+        const int *bidx = d_bidx ? d_bidx + b0 : nullptr;
         hipLaunchKernelGGL(k_zero_int, dim3((nb_ + 255) / 256), dim3(256), 0, s, ctx->d_sync, nb_);
         for (int k = 0; k < n; k += NB) {
             PanelArgs<T> pa;
             pa.A = Ab; pa.lda = lda; pa.strideA = strideA; pa.n = n; pa.k = k;
             pa.P = (n - k + RW - 1) / RW; pa.pmax = pmax;
             pa.ipiv = ipb; pa.strideP = strideP; pa.info = infob;
-            pa.cnt = ctx->d_sync; pa.err = ctx->d_err; pa.xbuf = xbuf;
+            pa.cnt = ctx->d_sync; pa.err = ctx->d_err; pa.xbuf = xbuf; pa.bidx = bidx;
             const double mr = n - k, m2 = n - k - NB, esz = 2.0 * sizeof(T);
             // panel: per column j, (mr-j) scalings and (mr-j)(NB-j-1) complex multiply-adds
             prof_begin(ctx, K_PANEL, s);
@@ -597,19 +702,13 @@ int getrf_impl(wm_ctx *ctx, int n, typename CT<T>::V *A, int lda, int64_t stride
             if (ntl + ntr > 0) {
                 prof_begin(ctx, K_SWAP, s);
                 hipLaunchKernelGGL((k_swap_trsm<T, CB>), dim3(ntl + ntr, nb_), dim3(256), 0, s, Ab, lda, strideA, n,
-                                   k, pmax, xbuf, ntl, ctx->d_sync);
+                                   k, pmax, xbuf, ntl, ctx->d_sync, bidx);
                 prof_end(ctx, K_SWAP, s, nb_ * 4.0 * NB * NB * m2, nb_ * 4.0 * NB * n * esz);
             }
             if (m2 > 0) {
                 // trailing update: 8 M^2 NB real flops; C read and written, L21 and U12 read once
                 prof_begin(ctx, K_GEMM, s);
-                if (sizeof(T) == 8) {
-                    const int t = (int)(m2 + G64_T - 1) / G64_T;
-                    hipLaunchKernelGGL(k_gemm_f64, dim3(t, t, nb_), dim3(256), 0, s, (double2 *)Ab, lda, strideA, n, k);
-                } else {
-                    const int t = (int)(m2 + G32_T - 1) / G32_T;
-                    hipLaunchKernelGGL(k_gemm_f32, dim3(t, t, nb_), dim3(256), 0, s, (float2 *)Ab, lda, strideA, n, k);
-                }
+                launch_gemm((int)sizeof(T), Ab, lda, strideA, n, k, nb_, bidx, s);
                 prof_end(ctx, K_GEMM, s, nb_ * 8.0 * m2 * m2 * NB, nb_ * (2.0 * m2 * m2 + 2.0 * m2 * NB) * esz);
             }
         }
@@ -621,16 +720,19 @@ int getrf_impl(wm_ctx *ctx, int n, typename CT<T>::V *A, int lda, int64_t stride
 }  // namespace
 
 int getrf_batched(wm_ctx *ctx, int prec, int n, void *A, int lda, int64_t strideA, int *ipiv, int64_t strideP,
-                  int batch, int *d_info, hipStream_t s)
+                  int batch, int *d_info, hipStream_t s, const int *d_bidx)
 {
     if (n <= 0 || n % NB != 0 || lda < n || batch <= 0 || !A || !ipiv || !d_info) {
         set_error("getrf: bad arguments (n=%d must be a positive multiple of %d, lda=%d, batch=%d)", n, NB, lda,
                   batch);
         return WM_ERR_ARG;
     }
-    hipLaunchKernelGGL(k_zero_int, dim3((batch + 255) / 256), dim3(256), 0, s, d_info, batch);
-    if (prec == WM_PREC_F64) return getrf_impl<double, 2>(ctx, n, (double2 *)A, lda, strideA, ipiv, strideP, batch, d_info, s);
-    if (prec == WM_PREC_F32) return getrf_impl<float, 1>(ctx, n, (float2 *)A, lda, strideA, ipiv, strideP, batch, d_info, s);
+    if (!d_bidx) hipLaunchKernelGGL(k_zero_int, dim3((batch + 255) / 256), dim3(256), 0, s, d_info, batch);
+    else hipLaunchKernelGGL(k_zero_idx, dim3((batch + 255) / 256), dim3(256), 0, s, d_info, d_bidx, batch);
+    if (prec == WM_PREC_F64)
+        return getrf_impl<double, 2>(ctx, n, (double2 *)A, lda, strideA, ipiv, strideP, batch, d_info, d_bidx, s);
+    if (prec == WM_PREC_F32)
+        return getrf_impl<float, 1>(ctx, n, (float2 *)A, lda, strideA, ipiv, strideP, batch, d_info, d_bidx, s);
     set_error("getrf: precision %d not available", prec);
     return WM_ERR_ARG;
 }
@@ -646,13 +748,7 @@ extern "C" int wmx_gemm_step(wm_ctx *ctx, int prec, int n, void *A, int k)
     using namespace wm;
     const int m2 = n - k - NB;
     if (m2 <= 0 || n % NB || k % NB) return WM_ERR_ARG;
-    if (prec == WM_PREC_F64) {
-        const int t = (m2 + G64_T - 1) / G64_T;
-        hipLaunchKernelGGL(k_gemm_f64, dim3(t, t, 1), dim3(256), 0, ctx->stream, (double2 *)A, n, (int64_t)0, n, k);
-    } else {
-        const int t = (m2 + G32_T - 1) / G32_T;
-        hipLaunchKernelGGL(k_gemm_f32, dim3(t, t, 1), dim3(256), 0, ctx->stream, (float2 *)A, n, (int64_t)0, n, k);
-    }
+    launch_gemm(prec == WM_PREC_F64 ? 8 : 4, A, n, 0, n, k, 1, nullptr, ctx->stream);
     WM_HIP_TRY(hipStreamSynchronize(ctx->stream));
     return WM_OK;
 }
@@ -677,7 +773,7 @@ extern "C" int wmx_panel_step(wm_ctx *ctx, int prec, int n, void *A, int *ipiv, 
         pa.A = (V *)Ap; pa.lda = n; pa.strideA = 0; pa.n = n; pa.k = k;
         pa.P = (n - k + RW - 1) / RW; pa.pmax = pmax;
         pa.ipiv = ipiv; pa.strideP = 0; pa.info = info;
-        pa.cnt = ctx->d_sync; pa.err = ctx->d_err; pa.xbuf = (char *)ctx->ws_lu.ptr;
+        pa.cnt = ctx->d_sync; pa.err = ctx->d_err; pa.xbuf = (char *)ctx->ws_lu.ptr; pa.bidx = nullptr;
         constexpr int CB = (sizeof(T) == 8) ? 16 : 32;
         if (sizeof(T) == 8)
             hipLaunchKernelGGL((k_panel<double, 2>), dim3(pa.P, 1), dim3(256), 0, ctx->stream, *(PanelArgs<double> *)&pa);
@@ -687,7 +783,8 @@ extern "C" int wmx_panel_step(wm_ctx *ctx, int prec, int n, void *A, int *ipiv, 
             const int ntl = (k + CB - 1) / CB, ntr = (n - k - NB + CB - 1) / CB;
             if (ntl + ntr > 0)
                 hipLaunchKernelGGL((k_swap_trsm<T, CB>), dim3(ntl + ntr, 1), dim3(256), 0, ctx->stream, (V *)Ap, n,
-                                   (int64_t)0, n, k, pmax, (char *)ctx->ws_lu.ptr, ntl, ctx->d_sync);
+                                   (int64_t)0, n, k, pmax, (char *)ctx->ws_lu.ptr, ntl, ctx->d_sync,
+                                   (const int *)nullptr);
         }
         return WM_OK;
     };

@@ -35,10 +35,10 @@ __device__ inline void cfms(double2 &acc, double2 a, double2 b)  // acc -= a*b
 }
 
 // perm[t] = source row of position t after applying ipiv in order.
-__global__ void k_perm(const int *ipiv, int64_t strideP, int n, int *perm, int64_t strideQ)
+__global__ void k_perm(const int *ipiv, int64_t strideP, int n, int *perm, int64_t strideQ, const int *bidx)
 {
     extern __shared__ int s_p[];
-    const int b = blockIdx.x;
+    const int b = bidx ? bidx[blockIdx.x] : (int)blockIdx.x;
     const int *ip = ipiv + (size_t)b * strideP;
     int *pq = perm + (size_t)b * strideQ;
     for (int i = threadIdx.x; i < n; i += blockDim.x) s_p[i] = i;
@@ -60,104 +60,124 @@ struct TrsvArgs {
     int n, nrhs;
     const int *perm;
     int64_t strideQ;
-    const double2 *Bin;   // lower solve: right-hand sides (gathered through perm)
+    const double2 *Bin;   // lower solve: right-hand sides (gathered through perm), per launch member
     int ldb;
     int64_t strideB;
     double2 *Y;           // output (lower: y, upper: x in place of y)
-    int *flags;           // nblk per matrix
+    int *flags;           // nblk per launch member
     int epoch;
     int *err;
+    const int *bidx;      // launch member -> matrix index (null: identity)
 };
 
-// L y = P b.  NR right-hand sides per pass.
-template <typename T, int NR>
-__global__ __launch_bounds__(256) void k_trsv_lower(TrsvArgs a, int r_off)
+__device__ inline double bcast(double v, int lane)  // value of `lane` to the whole wave
 {
-    using V = typename CT<T>::V;
-    const int b = blockIdx.y, bi = blockIdx.x, tid = threadIdx.x, row = tid & 63, part = tid >> 6;
-    const int nblk = a.n / TB, r0 = bi * TB;
-    const V *A = (const V *)a.A + (size_t)b * a.strideA;
-    const int *perm = a.perm + (size_t)b * a.strideQ;
-    const double2 *Bin = a.Bin + (size_t)b * a.strideB + (size_t)r_off * a.ldb;
-    double2 *Y = a.Y + (size_t)b * a.strideB + (size_t)r_off * a.ldb;
-    int *flags = a.flags + (size_t)b * nblk;
-
-    double2 acc[NR];
-#pragma unroll
-    for (int r = 0; r < NR; ++r) acc[r] = make_double2(0.0, 0.0);
-    if (part == 0) {
-        const int src = perm[r0 + row];
-#pragma unroll
-        for (int r = 0; r < NR; ++r)
-            if (r < a.nrhs - r_off) acc[r] = Bin[src + (size_t)r * a.ldb];
-    }
-    for (int jb = 0; jb < bi; ++jb) {
-        wait_flag(flags + jb, a.epoch, a.err);
-        const int c0 = jb * TB + part * 16;
-#pragma unroll 4
-        for (int c = 0; c < 16; ++c) {
-            const double2 l = ld2<T>(A + (r0 + row) + (size_t)(c0 + c) * a.lda);
-#pragma unroll
-            for (int r = 0; r < NR; ++r)
-                if (r < a.nrhs - r_off) cfms(acc[r], l, Y[(c0 + c) + (size_t)r * a.ldb]);
-        }
-    }
-    __shared__ double2 s_acc[3][TB][NR];
-    if (part > 0)
-#pragma unroll
-        for (int r = 0; r < NR; ++r) s_acc[part - 1][row][r] = acc[r];
-    __syncthreads();
-    if (part == 0) {
-#pragma unroll
-        for (int r = 0; r < NR; ++r)
-#pragma unroll
-            for (int p = 0; p < 3; ++p) { acc[r].x += s_acc[p][row][r].x; acc[r].y += s_acc[p][row][r].y; }
-        // diagonal block, unit lower, lane `row` holds row r0+row
-        for (int j = 0; j < TB; ++j) {
-            const double2 l = ld2<T>(A + (r0 + row) + (size_t)(r0 + j) * a.lda);
-#pragma unroll
-            for (int r = 0; r < NR; ++r) {
-                const double yr = __shfl(acc[r].x, j), yi = __shfl(acc[r].y, j);
-                if (row > j) cfms(acc[r], l, make_double2(yr, yi));
-            }
-        }
-#pragma unroll
-        for (int r = 0; r < NR; ++r)
-            if (r < a.nrhs - r_off) Y[(r0 + row) + (size_t)r * a.ldb] = acc[r];
-    }
-    set_flag(flags + bi, a.epoch);
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = __builtin_amdgcn_readlane((unsigned)(u & 0xffffffffu), lane);
+    const unsigned hi = __builtin_amdgcn_readlane((unsigned)(u >> 32), lane);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
-// U x = y in place of y.  Workgroup g solves row block nblk-1-g.
-template <typename T, int NR>
-__global__ __launch_bounds__(256) void k_trsv_upper(TrsvArgs a, int r_off)
+// Packed index of element (i, j) of the 64 x 64 diagonal block: strictly
+// lower part for L (i > j), upper part with the diagonal for U (i <= j).
+__device__ inline int dl_idx(int i, int j) { return i * (i - 1) / 2 + j; }
+__device__ inline int du_idx(int i, int j) { return j * (j + 1) / 2 + i; }
+
+// Triangular solve, one launch.  Workgroup g owns row block bi (lower: g,
+// upper: nblk-1-g); lower solves L y = P b, upper solves U x = y in place.
+// Waits only on blocks of workgroups with smaller ids (dispatched earlier).
+template <typename T, int NR, bool UPPER>
+__global__ __launch_bounds__(256) void k_trsv(TrsvArgs a, int r_off)
 {
     using V = typename CT<T>::V;
-    const int b = blockIdx.y, tid = threadIdx.x, row = tid & 63, part = tid >> 6;
-    const int nblk = a.n / TB, bi = nblk - 1 - (int)blockIdx.x, r0 = bi * TB;
+    const int bl = blockIdx.y, tid = threadIdx.x, row = tid & 63;
+    const int b = a.bidx ? a.bidx[bl] : bl;
+    const int part = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int nblk = a.n / TB;
+    const int bi = UPPER ? nblk - 1 - (int)blockIdx.x : (int)blockIdx.x, r0 = bi * TB;
     const V *A = (const V *)a.A + (size_t)b * a.strideA;
     double2 *Y = a.Y + (size_t)b * a.strideB + (size_t)r_off * a.ldb;
-    int *flags = a.flags + (size_t)b * nblk;
+    int *flags = a.flags + (size_t)bl * nblk;
+    const int nr = a.nrhs - r_off;
+
+    // diagonal block to LDS (independent of other blocks, so before any wait)
+    __shared__ double2 s_d[TB * (TB + 1) / 2];
+    __shared__ double2 s_rd[TB];
+    for (int e = tid; e < TB * TB; e += 256) {
+        const int i = e % TB, j = e / TB;
+        if (!UPPER && i > j) s_d[dl_idx(i, j)] = ld2<T>(A + (r0 + i) + (size_t)(r0 + j) * a.lda);
+        if (UPPER && i <= j) {
+            const double2 u = ld2<T>(A + (r0 + i) + (size_t)(r0 + j) * a.lda);
+            s_d[du_idx(i, j)] = u;
+            if (i == j) s_rd[i] = crecip(u.x, u.y);
+        }
+    }
 
     double2 acc[NR];
 #pragma unroll
     for (int r = 0; r < NR; ++r) acc[r] = make_double2(0.0, 0.0);
     if (part == 0) {
+        const int src = UPPER ? r0 + row : a.perm[(size_t)b * a.strideQ + r0 + row];
+        const double2 *in = UPPER ? Y : a.Bin + (size_t)bl * a.strideB + (size_t)r_off * a.ldb;
 #pragma unroll
         for (int r = 0; r < NR; ++r)
-            if (r < a.nrhs - r_off) acc[r] = Y[(r0 + row) + (size_t)r * a.ldb];
+            if (r < nr) acc[r] = in[src + (size_t)r * a.ldb];
     }
-    for (int jb = nblk - 1; jb > bi; --jb) {
-        wait_flag(flags + jb, a.epoch, a.err);
-        const int c0 = jb * TB + part * 16;
-#pragma unroll 4
+
+    // off-diagonal blocks in dependency order, next block prefetched
+    const int nb_off = UPPER ? nblk - 1 - bi : bi;
+    auto blk = [&](int q) { return UPPER ? nblk - 1 - q : q; };
+    __shared__ int s_ready;
+    int ready = 0;  // blocks q < ready are known to be solved
+    double2 lcur[16], lnext[16];
+    if (nb_off > 0) {
+        const int c0 = blk(0) * TB + part * 16;
+#pragma unroll
+        for (int c = 0; c < 16; ++c) lcur[c] = ld2<T>(A + (r0 + row) + (size_t)(c0 + c) * a.lda);
+    }
+    for (int q = 0; q < nb_off; ++q) {
+        if (q + 1 < nb_off) {
+            const int c1 = blk(q + 1) * TB + part * 16;
+#pragma unroll
+            for (int c = 0; c < 16; ++c) lnext[c] = ld2<T>(A + (r0 + row) + (size_t)(c1 + c) * a.lda);
+        }
+        if (q >= ready) {
+            // wave 0 polls up to 64 flags per round trip and records the ready run
+            if (tid < 64) {
+                long it = 0;
+                int run = 0;
+                for (;;) {
+                    const int qq = q + tid;
+                    int f = a.epoch;
+                    if (qq < nb_off) f = __hip_atomic_load(flags + blk(qq), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    const unsigned long long m = __ballot(f >= a.epoch);
+                    run = (~m == 0ull) ? 64 : (int)__builtin_ctzll(~m);  // leading ready lanes
+                    if (run > 0) break;
+                    if (__hip_atomic_load(a.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) { run = nb_off; break; }
+                    if (++it > kSpinLimit) {
+                        if (tid == 0) __hip_atomic_store(a.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        run = nb_off;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                if (tid == 0) s_ready = q + run;
+            }
+            __syncthreads();
+            ready = s_ready;
+        }
+        const int c0 = blk(q) * TB + part * 16;
+#pragma unroll
         for (int c = 0; c < 16; ++c) {
-            const double2 u = ld2<T>(A + (r0 + row) + (size_t)(c0 + c) * a.lda);
 #pragma unroll
             for (int r = 0; r < NR; ++r)
-                if (r < a.nrhs - r_off) cfms(acc[r], u, Y[(c0 + c) + (size_t)r * a.ldb]);
+                if (r < nr) cfms(acc[r], lcur[c], Y[(c0 + c) + (size_t)r * a.ldb]);
         }
+#pragma unroll
+        for (int c = 0; c < 16; ++c) lcur[c] = lnext[c];
     }
+
     __shared__ double2 s_acc[3][TB][NR];
     if (part > 0)
 #pragma unroll
@@ -168,24 +188,33 @@ __global__ __launch_bounds__(256) void k_trsv_upper(TrsvArgs a, int r_off)
         for (int r = 0; r < NR; ++r)
 #pragma unroll
             for (int p = 0; p < 3; ++p) { acc[r].x += s_acc[p][row][r].x; acc[r].y += s_acc[p][row][r].y; }
-        for (int j = TB - 1; j >= 0; --j) {
-            const double2 u = ld2<T>(A + (r0 + row) + (size_t)(r0 + j) * a.lda);
-            // lane j divides by its diagonal; everyone else eliminates
-            double2 q[NR];
-            const double2 rinv = crecip(u.x, u.y);  // used by lane j only
+        if (!UPPER) {
+            for (int j = 0; j < TB - 1; ++j) {
+                const double2 l = row > j ? s_d[dl_idx(row, j)] : make_double2(0.0, 0.0);
 #pragma unroll
-            for (int r = 0; r < NR; ++r) {
-                if (row == j) {
-                    const double2 x = acc[r];
-                    acc[r] = make_double2(x.x * rinv.x - x.y * rinv.y, x.x * rinv.y + x.y * rinv.x);
+                for (int r = 0; r < NR; ++r) {
+                    const double2 yj = make_double2(bcast(acc[r].x, j), bcast(acc[r].y, j));
+                    if (row > j) cfms(acc[r], l, yj);
                 }
-                q[r] = make_double2(__shfl(acc[r].x, j), __shfl(acc[r].y, j));
-                if (row < j) cfms(acc[r], u, q[r]);
+            }
+        } else {
+            for (int j = TB - 1; j >= 0; --j) {
+                const double2 u = row < j ? s_d[du_idx(row, j)] : make_double2(0.0, 0.0);
+                const double2 ri = s_rd[j];
+#pragma unroll
+                for (int r = 0; r < NR; ++r) {
+                    if (row == j) {
+                        const double2 x = acc[r];
+                        acc[r] = make_double2(x.x * ri.x - x.y * ri.y, x.x * ri.y + x.y * ri.x);
+                    }
+                    const double2 xj = make_double2(bcast(acc[r].x, j), bcast(acc[r].y, j));
+                    if (row < j) cfms(acc[r], u, xj);
+                }
             }
         }
 #pragma unroll
         for (int r = 0; r < NR; ++r)
-            if (r < a.nrhs - r_off) Y[(r0 + row) + (size_t)r * a.ldb] = acc[r];
+            if (r < nr) Y[(r0 + row) + (size_t)r * a.ldb] = acc[r];
     }
     set_flag(flags + bi, a.epoch);
 }
@@ -194,9 +223,10 @@ __global__ __launch_bounds__(256) void k_trsv_upper(TrsvArgs a, int r_off)
 template <int NR>
 __global__ __launch_bounds__(256) void k_resid_part(const double2 *Z, int ldz, int64_t strideZ, int n, int nrhs,
                                                     const double2 *X, int ldb, int64_t strideB, int cols_per_split,
-                                                    int nsplit, double2 *P, int r_off)
+                                                    int nsplit, double2 *P, int r_off, const int *bidx)
 {
-    const int b = blockIdx.z, split = blockIdx.y, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int bl = blockIdx.z, split = blockIdx.y, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int b = bidx ? bidx[bl] : bl;
     const int row = blockIdx.x * 64 + lane;
     const double2 *Zb = Z + (size_t)b * strideZ;
     const double2 *Xb = X + (size_t)b * strideB + (size_t)r_off * ldb;
@@ -228,7 +258,7 @@ __global__ __launch_bounds__(256) void k_resid_part(const double2 *Z, int ldz, i
         for (int r = 0; r < NR; ++r) {
 #pragma unroll
             for (int p = 0; p < 3; ++p) { acc[r].x += s[p][lane][r].x; acc[r].y += s[p][lane][r].y; }
-            P[(((size_t)b * nsplit + split) * NR + r) * n + row] = acc[r];
+            P[(((size_t)bl * nsplit + split) * NR + r) * n + row] = acc[r];
         }
     }
 }
@@ -236,17 +266,18 @@ __global__ __launch_bounds__(256) void k_resid_part(const double2 *Z, int ldz, i
 // R = B - sum_split P, in split order.
 template <int NR>
 __global__ void k_resid_final(const double2 *P, int nsplit, int n, int nrhs, const double2 *B, int ldb,
-                              int64_t strideB, double2 *R, int r_off, int batch)
+                              int64_t strideB, double2 *R, int r_off, const int *bidx)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int b = blockIdx.y;
+    const int bl = blockIdx.y;
+    const int b = bidx ? bidx[bl] : bl;
     if (i >= n) return;
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
         if (r >= nrhs - r_off) break;
         double2 s = make_double2(0.0, 0.0);
         for (int sp = 0; sp < nsplit; ++sp) {
-            const double2 p = P[(((size_t)b * nsplit + sp) * NR + r) * n + i];
+            const double2 p = P[(((size_t)bl * nsplit + sp) * NR + r) * n + i];
             s.x += p.x;
             s.y += p.y;
         }
@@ -259,8 +290,8 @@ template <typename T, int NR>
 void launch_trsv(const TrsvArgs &a, int batch, int r_off, bool upper, hipStream_t s)
 {
     const dim3 grid(a.n / TB, batch);
-    if (upper) hipLaunchKernelGGL((k_trsv_upper<T, NR>), grid, dim3(256), 0, s, a, r_off);
-    else hipLaunchKernelGGL((k_trsv_lower<T, NR>), grid, dim3(256), 0, s, a, r_off);
+    if (upper) hipLaunchKernelGGL((k_trsv<T, NR, true>), grid, dim3(256), 0, s, a, r_off);
+    else hipLaunchKernelGGL((k_trsv<T, NR, false>), grid, dim3(256), 0, s, a, r_off);
 }
 
 // Right-hand sides handled by one pass starting at r_off: 8, 4, 2 or 1.
@@ -277,6 +308,15 @@ void trsv_dispatch(const TrsvArgs &a, int batch, int r_off, bool upper, hipStrea
     }
 }
 
+// Bstage[member] = B[matrix] (n x nrhs columns with leading dimension ldb)
+__global__ void k_stage(const double2 *B, int ldb, int64_t strideB, int n, int nrhs, double2 *S, const int *bidx)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, bl = blockIdx.y;
+    const int b = bidx ? bidx[bl] : bl;
+    if (i >= n) return;
+    for (int r = 0; r < nrhs; ++r) S[(size_t)bl * strideB + (size_t)r * ldb + i] = B[(size_t)b * strideB + (size_t)r * ldb + i];
+}
+
 __global__ void k_zero_flags(int *p, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -285,21 +325,31 @@ __global__ void k_zero_flags(int *p, int n)
 
 }  // namespace
 
-// Workspace layout in ctx->ws_solve: perm [batch][n] ints, flags [2][batch][n/64] ints.
-int getrs_batched(wm_ctx *ctx, int prec, int n, int nrhs, const void *A, int lda, int64_t strideA,
-                  const int *ipiv, int64_t strideP, void *B, int ldb, int64_t strideB, int batch, hipStream_t s)
+int perm_batched(wm_ctx *ctx, int n, const int *ipiv, int64_t strideP, int *perm, int64_t strideQ, int batch,
+                 hipStream_t s, const int *d_bidx)
+{
+    if ((size_t)n * sizeof(int) > ctx->max_lds) {
+        set_error("permutation: n=%d exceeds the LDS capacity", n);
+        return WM_ERR_ARG;
+    }
+    hipLaunchKernelGGL(k_perm, dim3(batch), dim3(256), n * sizeof(int), s, ipiv, strideP, n, perm, strideQ, d_bidx);
+    WM_HIP_TRY(hipGetLastError());
+    return WM_OK;
+}
+
+// Staging of B per launch member in ctx->ws_solve; completion flags in
+// ctx->ws_flags, 2 x batch x n/64 epoch tags.
+int getrs_perm_batched(wm_ctx *ctx, int prec, int n, int nrhs, const void *A, int lda, int64_t strideA,
+                       const int *perm, int64_t strideQ, void *B, int ldb, int64_t strideB, int batch, hipStream_t s,
+                       const int *d_bidx)
 {
     if (n <= 0 || n % TB != 0 || nrhs <= 0 || batch <= 0 || ldb < n) {
         set_error("getrs: bad arguments (n=%d multiple of %d, nrhs=%d, batch=%d)", n, TB, nrhs, batch);
         return WM_ERR_ARG;
     }
     const int nblk = n / TB;
-    const size_t perm_bytes = ((size_t)batch * n * sizeof(int) + 255) & ~(size_t)255;
-    const size_t y_bytes = (size_t)batch * strideB * sizeof(double2);
-    WM_TRY(ctx->ws_solve.ensure(perm_bytes + y_bytes));
-    char *base = (char *)ctx->ws_solve.ptr;
-    int *perm = (int *)base;
-    double2 *Ytmp = (double2 *)(base + perm_bytes);
+    WM_TRY(ctx->ws_solve.ensure((size_t)batch * strideB * sizeof(double2)));
+    double2 *Ystage = (double2 *)ctx->ws_solve.ptr;
     // Completion flags live in their own buffer and only ever hold epoch tags
     // of this context, so a wait can never be satisfied by stale data.  The
     // buffer is zeroed whenever it is (re)allocated.
@@ -316,20 +366,15 @@ int getrs_batched(wm_ctx *ctx, int prec, int n, int nrhs, const void *A, int lda
         hipLaunchKernelGGL(k_zero_flags, dim3((nf + 255) / 256), dim3(256), 0, s, flags, nf);
         ctx->solve_epoch = 0;
     }
-    if ((size_t)n * sizeof(int) > ctx->max_lds) {
-        set_error("getrs: n=%d exceeds the LDS permutation capacity", n);
-        return WM_ERR_ARG;
-    }
-    hipLaunchKernelGGL(k_perm, dim3(batch), dim3(256), n * sizeof(int), s, ipiv, strideP, n, perm, (int64_t)n);
-    // copy B to the temporary (lower solve reads B, writes Y; upper solves Y in place)
-    WM_HIP_TRY(hipMemcpy2DAsync(Ytmp, strideB * sizeof(double2), B, strideB * sizeof(double2),
-                                strideB * sizeof(double2), batch, hipMemcpyDeviceToDevice, s));
+    // the lower solve reads B (staged per member) and writes y into B; the upper solves in place
+    hipLaunchKernelGGL(k_stage, dim3((n + 255) / 256, batch), dim3(256), 0, s, (const double2 *)B, ldb, strideB, n, nrhs,
+                       Ystage, d_bidx);
     for (int r0 = 0; r0 < nrhs; r0 += rhs_chunk(nrhs - r0)) {
         TrsvArgs a;
         a.A = A; a.lda = lda; a.strideA = strideA; a.n = n; a.nrhs = nrhs;
-        a.perm = perm; a.strideQ = n; a.ldb = ldb; a.strideB = strideB; a.err = ctx->d_err;
-        // lower: Bin = Ytmp (copy of B), output into B
-        a.Bin = Ytmp; a.Y = (double2 *)B; a.flags = flags;
+        a.perm = perm; a.strideQ = strideQ; a.ldb = ldb; a.strideB = strideB; a.err = ctx->d_err;
+        a.bidx = d_bidx;
+        a.Bin = Ystage; a.Y = (double2 *)B; a.flags = flags;
         a.epoch = ++ctx->solve_epoch;
         const int nr = rhs_chunk(nrhs - r0);
         const double esz = prec == WM_PREC_F64 ? 16.0 : 8.0;
@@ -348,8 +393,17 @@ int getrs_batched(wm_ctx *ctx, int prec, int n, int nrhs, const void *A, int lda
     return WM_OK;
 }
 
+int getrs_batched(wm_ctx *ctx, int prec, int n, int nrhs, const void *A, int lda, int64_t strideA, const int *ipiv,
+                  int64_t strideP, void *B, int ldb, int64_t strideB, int batch, hipStream_t s)
+{
+    WM_TRY(ctx->ws_perm.ensure((size_t)batch * n * sizeof(int)));
+    int *perm = (int *)ctx->ws_perm.ptr;
+    WM_TRY(perm_batched(ctx, n, ipiv, strideP, perm, n, batch, s));
+    return getrs_perm_batched(ctx, prec, n, nrhs, A, lda, strideA, perm, n, B, ldb, strideB, batch, s);
+}
+
 int residual_batched(wm_ctx *ctx, int n, int nrhs, const void *Z, int ldz, int64_t strideZ, const void *X,
-                     const void *B, int ldb, int64_t strideB, void *R, int batch, hipStream_t s)
+                     const void *B, int ldb, int64_t strideB, void *R, int batch, hipStream_t s, const int *d_bidx)
 {
     if (n % 64 != 0) {
         set_error("residual: n=%d not a multiple of 64", n);
@@ -368,9 +422,9 @@ int residual_batched(wm_ctx *ctx, int n, int nrhs, const void *Z, int ldz, int64
         const dim3 g1(n / 64, nsplit, batch), g2((n + 255) / 256, batch);
 #define WM_RESID(NR)                                                                                        \
     hipLaunchKernelGGL(k_resid_part<NR>, g1, dim3(256), 0, s, (const double2 *)Z, ldz, strideZ, n, nrhs,    \
-                       (const double2 *)X, ldb, strideB, cps, nsplit, (double2 *)partial.ptr, r0);          \
+                       (const double2 *)X, ldb, strideB, cps, nsplit, (double2 *)partial.ptr, r0, d_bidx);  \
     hipLaunchKernelGGL(k_resid_final<NR>, g2, dim3(256), 0, s, (const double2 *)partial.ptr, nsplit, n, nrhs, \
-                       (const double2 *)B, ldb, strideB, (double2 *)R, r0, batch)
+                       (const double2 *)B, ldb, strideB, (double2 *)R, r0, d_bidx)
         prof_begin(ctx, K_RESID, s);
         if (left == 8) { WM_RESID(8); }
         else if (left == 4) { WM_RESID(4); }

@@ -219,3 +219,21 @@ def test_getrs_alternating_layouts(wm, ctx):
         for i in range(b):
             r = np.abs(A[i] @ X[i] - B[i]).max() / (np.abs(A[i]).max() * np.abs(X[i]).max() * n)
             assert r <= 1e-15, (n, b, nr, r)
+
+
+def test_mixed_precision_fallback_band(wm, geometries, ctx, geo_dir):
+    """Across a 10 kHz - 10 MHz band the low frequencies are too ill-conditioned
+    for an FP32 factorization; those items are refactored in FP64 inside the same
+    sweep, and every Zin still matches the FP64 path within 1e-10."""
+    p = os.path.join(geo_dir, "light1024.geo")
+    geometries.lightning(p, 1024)
+    g = wm.Geometry.load(ctx, p)
+    t = wm.configwire(p, cwd=geo_dir)
+    gap = geometries.nearest_basis(t, (0, 0, 19.9))
+    f = np.logspace(-2, 1, 12).astype(np.float32)
+    z64, _, _ = wm.solve_sweep(g, f, [gap], "f64")
+    z32, _, st = wm.solve_sweep(g, f, [gap], "f32")
+    rel = np.abs(z32[:, 0] - z64[:, 0]) / np.abs(z64[:, 0])
+    print([(float(fr), s["iters"], s["fell_back"], float(r)) for fr, s, r in zip(f, st, rel)])
+    assert (rel <= 1e-10).all(), rel
+    assert any(s["fell_back"] for s in st)

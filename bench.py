@@ -171,7 +171,8 @@ def main():
         "speedup_vs_fp64": value / fp64,
         "zin_rel_err_vs_fp64_max": float(rel.max()),
         "ir_iters_mean": iters,
-        "fell_back": int(sum(s["fell_back"] for s in st)),
+        "fell_back": int(sum(s["fell_back"] == 1 for s in st)),
+        "routed_f64": int(sum(s["fell_back"] == 2 for s in st)),
         "solve_roofline_frac": solve_frac,
         "kernel_ms": {k: round(v[0] / a.steps, 3) for k, v in prof.items()},
         "roofline": roof,

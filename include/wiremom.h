@@ -135,7 +135,8 @@ void wm_solve_opts_default(wm_solve_opts *o);
 typedef struct {
     int iters;             /* refinement iterations (0 for FP64) */
     int info;              /* LU info */
-    int fell_back;         /* 1 if the FP64 fallback produced the result */
+    int fell_back;         /* FP64 factors produced the result: 1 after refinement failed,
+                              2 routed there directly (predicted not to refine) */
     double backward_err;   /* max_r ||b - Z x||_inf / (||Z||_inf ||x||_inf + ||b||_inf) */
     double dx_rel;         /* last ||dx||_inf / ||x||_inf */
 } wm_freq_stats;
@@ -146,6 +147,10 @@ typedef struct {
  *   I   : complex128 [nfreq][nrhs][npls]   current coefficients
  *   zin : complex128 [nfreq][nrhs]          V/I at each feed
  *   st  : wm_freq_stats [nfreq]
+ * In mixed precision the frequencies are solved highest first; the refinement
+ * contraction rates of the first batch (one factorization launch wide) predict,
+ * scaled by (f_obs/f)^2, which lower frequencies cannot reach tol within
+ * max_iter, and those are factored in FP64 directly (fell_back = 2).
  * Returns the first nonzero status. */
 int wm_solve_sweep(wm_ctx *ctx, const wm_geom *g, const wm_solve_opts *opts, int nfreq,
                    const float *freq_mhz, int nrhs, const int *gaps, float ampl, float phas,

@@ -574,10 +574,26 @@ int wm_solve_sweep(wm_ctx *c, const wm_geom *g, const wm_solve_opts *opts, int n
     }
     int batch = o.batch > 0 ? o.batch : auto_batch(c, g->ld, nrhs, o.prec, o.hbm_fraction);
     int status = WM_OK;
-    for (int f0 = 0; f0 < nfreq; f0 += batch) {
-        const int rc = solve_items(c, o, items.data() + f0, std::min(batch, nfreq - f0), c->stream);
+    const bool mixed = o.prec != WM_PREC_F64;
+    if (mixed) {
+        // Highest frequencies first: their refinement rates, recorded on the
+        // geometry (cleared here), route the lower ones, whose conditioning is
+        // worse, to FP64 factors when refinement cannot reach tol in max_iter.
+        std::stable_sort(items.begin(), items.end(), [](const WorkItem &a, const WorkItem &b) { return a.freq > b.freq; });
+        g->rate_obs.clear();
+    }
+    int f0 = 0;
+    while (f0 < nfreq) {
+        int cnt = std::min(batch, nfreq - f0);
+        // the first batch of a mixed sweep is one factorization launch wide, so
+        // that the rest is routed with its rates
+        if (mixed && f0 == 0) cnt = std::min(cnt, getrf_width(c, WM_PREC_F32, g->ld));
+        if (mixed && f0 > 0) route_items(o, items.data() + f0, cnt);
+        const int rc = solve_items(c, o, items.data() + f0, cnt, c->stream);
         if (rc < 0) return rc;
         if (rc > 0 && status == WM_OK) status = rc;
+        if (mixed) record_rates(items.data() + f0, cnt);
+        f0 += cnt;
     }
     return status;
 }

@@ -131,6 +131,45 @@ int auto_batch(wm_ctx *ctx, int ld, int nrhs, int prec, double frac)
     return std::min(b, 64);
 }
 
+double predict_rate(const wm_geom *g, float f)
+{
+    const std::pair<float, double> *above = nullptr, *below = nullptr;
+    for (const auto &ob : g->rate_obs) {
+        if (ob.first >= f) {
+            if (!above || ob.first < above->first) above = &ob;
+        } else if (!below || ob.first > below->first) {
+            below = &ob;
+        }
+    }
+    const auto *ob = above ? above : below;
+    if (!ob) return -1.0;
+    const double sc = (double)ob->first / (double)f;
+    return ob->second * sc * sc;
+}
+
+void record_rates(const WorkItem *items, int count)
+{
+    for (int b = 0; b < count; ++b) {
+        const WorkItem &w = items[b];
+        if (w.direct64 || w.rate < 0) continue;
+        auto &obs = w.g->rate_obs;
+        bool found = false;
+        for (auto &ob : obs)
+            if (ob.first == w.freq) { ob.second = w.rate; found = true; }
+        if (!found) obs.emplace_back(w.freq, w.rate);
+    }
+}
+
+void route_items(const wm_solve_opts &o, WorkItem *items, int count)
+{
+    // rate that reaches tol from a first correction of relative size 1 within max_iter
+    const double rmax = o.max_iter > 1 ? std::pow(o.tol, 1.0 / (o.max_iter - 1)) : 0.0;
+    for (int b = 0; b < count; ++b) {
+        const double r = predict_rate(items[b].g, items[b].freq);
+        items[b].direct64 = r > rmax;
+    }
+}
+
 int solve_items(wm_ctx *c, const wm_solve_opts &o, WorkItem *items, int count, hipStream_t s)
 {
     if (count <= 0) return WM_OK;
@@ -217,15 +256,30 @@ int solve_items(wm_ctx *c, const wm_solve_opts &o, WorkItem *items, int count, h
         WM_TRY(check_device_error(c, s));
         WM_TRY(copy_d2h(hinfo.data(), dinfo, sizeof(int) * count, s));
     } else {
-        WM_TRY(getrf_batched(c, WM_PREC_F32, n, Z32, n, sZ, ipiv, n, count, dinfo, s));
-        WM_TRY(perm_batched(c, n, ipiv, n, perm, n, count, s));
+        // FP32 factors for the items routed to refinement
+        std::vector<int> f32;
+        for (int b = 0; b < count; ++b) {
+            items[b].rate = -1.0;
+            if (!items[b].direct64) f32.push_back(b);
+        }
+        const int n32 = (int)f32.size();
+        if (n32 == count) {
+            WM_TRY(getrf_batched(c, WM_PREC_F32, n, Z32, n, sZ, ipiv, n, count, dinfo, s));
+            WM_TRY(perm_batched(c, n, ipiv, n, perm, n, count, s));
+        } else if (n32 > 0) {
+            WM_TRY(upload_list(f32));
+            WM_TRY(getrf_batched(c, WM_PREC_F32, n, Z32, n, sZ, ipiv, n, n32, dinfo, s, dact));
+            WM_TRY(perm_batched(c, n, ipiv, n, perm, n, n32, s, dact));
+        }
         WM_TRY(check_device_error(c, s));
         WM_TRY(copy_d2h(hinfo.data(), dinfo, sizeof(int) * count, s));
         hipLaunchKernelGGL(k_zero2, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, s, X, nv);
         WM_HIP_TRY(hipMemcpyAsync(R, B, szV, hipMemcpyDeviceToDevice, s));
         std::vector<int> active;
-        for (int b = 0; b < count; ++b)
-            if (hinfo[b] == 0) active.push_back(b);  // singular in FP32: straight to the FP64 fallback
+        for (int b : f32) {
+            if (hinfo[b] == 0) active.push_back(b);
+            else items[b].rate = HUGE_VAL;  // singular in FP32: straight to the FP64 fallback
+        }
         std::vector<double> prev((size_t)count * nr, 1e300);
         std::vector<unsigned long long> hm((size_t)2 * count * nr);
         std::vector<int> conv(count, 0);
@@ -257,6 +311,8 @@ int solve_items(wm_ctx *c, const wm_solve_opts &o, WorkItem *items, int count, h
                 }
                 dxr[b] = worst;
                 iters[b] = it + 1;
+                if (it >= 1 && !stalled) items[b].rate = rate;
+                else if (it == 0 && worst <= o.tol) items[b].rate = worst;  // converged at once: rate at most this
                 if (worst <= o.tol || stalled) {
                     conv[b] = 1;
                     continue;
@@ -287,7 +343,7 @@ int solve_items(wm_ctx *c, const wm_solve_opts &o, WorkItem *items, int count, h
             WM_TRY(getrs_perm_batched(c, WM_PREC_F64, n, nr, Z, n, sZ, perm, n, X, n, sV, nb, s, dact));
             WM_TRY(check_device_error(c, s));
             WM_TRY(copy_d2h(hinfo.data(), dinfo, sizeof(int) * count, s));
-            for (int b : bad) fell[b] = 1;
+            for (int b : bad) fell[b] = items[b].direct64 ? 2 : 1;
         }
     }
 

@@ -79,6 +79,7 @@ struct wm_ctx {
     wm::DevBuf ws_resid;    // residual partial sums
     wm::DevBuf ws_sweep;    // matrices and vectors of a sweep batch
     int solve_epoch = 0;    // monotonic tag of triangular-solve flags
+    uint32_t panel_tag = 0; // base tag of the next panel launch (packets in ws_lu)
     wm::Prof prof;          // per-kernel-class event timing (off by default)
 };
 
@@ -93,6 +94,9 @@ struct wm_geom {
     double *d_src = nullptr;   // ld records of 16 doubles: rs1[3] dc1[3] dl1 an1 rs2[3] dc2[3] dl2 an2
     int *d_src_key = nullptr;  // ld records of 2 ints
     std::vector<double> h_src; // host copy of the source records (self-term precompute)
+    // observed refinement contraction rates (frequency MHz, rate) of mixed-precision
+    // solves on this structure; see wm::predict_rate
+    mutable std::vector<std::pair<float, double>> rate_obs;
 };
 
 namespace wm {
@@ -148,7 +152,23 @@ struct WorkItem {
     wm_c128 *I = nullptr;      // [nrhs][npls]
     wm_c128 *zin = nullptr;    // [nrhs]
     wm_freq_stats *st = nullptr;
+    bool direct64 = false;     // mixed precision: factor in FP64 at once (predicted not to refine)
+    double rate = -1.0;        // out: last refinement contraction rate (inf: FP32 factor singular)
 };
+
+// Refinement contraction rate predicted for frequency f on geometry g from
+// its recorded observations, or -1 if there are none.  Below the structure's
+// resonances the EFIE condition number grows as 1/(k dl)^2, and the
+// contraction rate of FP32-factor refinement with it: the nearest observation
+// at or above f is scaled by (f_obs / f)^2 (the nearest below, if none).
+double predict_rate(const wm_geom *g, float f);
+// Records the outcome of the mixed-precision items of a batch on their geometries.
+void record_rates(const WorkItem *items, int count);
+// Routes the items of a batch: direct64 when the predicted rate cannot reach
+// tol within max_iter iterations.
+void route_items(const wm_solve_opts &o, WorkItem *items, int count);
+// Matrices per launch of the batched factorization (panel cluster residency).
+int getrf_width(wm_ctx *ctx, int prec, int n);
 
 // Fill, factor and solve a batch of items on stream s (synchronous).
 // Returns <0 on error, >0 (first zero pivot) if a matrix was singular.

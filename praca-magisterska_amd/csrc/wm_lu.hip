@@ -24,6 +24,8 @@
 //                     (C^T = U12^T L21^T) so each lane's accumulator column is
 //                     a run of consecutive rows of C: the C read-modify-write is
 //                     coalesced in column-major storage.
+#include <algorithm>
+
 #include "wm_cplx.h"
 #include "wm_internal.h"
 
@@ -32,40 +34,70 @@ namespace wm {
 namespace {
 
 constexpr int NB = 64;
+constexpr int kSub = 4;  // panels per outer block of the factorization
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v16f __attribute__((ext_vector_type(16)));
 
 // ---------------------------------------------------------------------------
-// exchange buffers for the panel: per matrix, two parities of
-//   slots [Pmax] {double v; int row; int wg}, cand [Pmax][NB], diag [NB]
+// Panel exchange buffers.  The workgroups of one panel cluster talk through
+// 8-byte packets {32-bit payload, 32-bit tag}; the tag names the column (a
+// per-launch base plus the column index, never reused while the buffer lives),
+// so a reader polls each packet until its tag matches and no fence or counter
+// is needed: 8-byte loads and stores are single-copy atomic, and agent-scope
+// relaxed atomics go to the device coherence point without flushing caches.
+// Packets alternate between two buffers by column parity: a workgroup
+// publishes column j+2 only after reading all packets of column j+1, which
+// every workgroup published after its reads of column j.
+//
+// Per launch member and parity:
+//   slot [pmax][4] : |candidate| as two 32-bit halves of a double, its row, unused
+//   cand [pmax][NW]: each workgroup's candidate row (NW 32-bit words)
+//   diag [NW]      : row k+j before the interchange (from workgroup 0)
 // followed by the panel permutation: cnt, rows[2NB], src[2NB].
-struct Slot {
-    double v;
-    int row;
-    int pad;
-};
-
 template <typename V>
 struct XLayout {
+    static constexpr int NW = NB * (int)sizeof(V) / 4;  // 32-bit words per panel row
     size_t par_bytes, perm_off, mat_bytes;
     int pmax;
     __host__ __device__ XLayout(int pmax_) : pmax(pmax_)
     {
-        par_bytes = (size_t)pmax * sizeof(Slot) + (size_t)pmax * NB * sizeof(V) + NB * sizeof(V);
+        par_bytes = ((size_t)pmax * 4 + (size_t)pmax * NW + NW) * sizeof(uint64_t);
         par_bytes = (par_bytes + 255) & ~(size_t)255;
         perm_off = 2 * par_bytes;
         mat_bytes = perm_off + (1 + 4 * NB) * sizeof(int);
         mat_bytes = (mat_bytes + 255) & ~(size_t)255;
     }
-    __device__ Slot *slots(char *m, int par) const { return (Slot *)(m + par * par_bytes); }
-    __device__ V *cand(char *m, int par) const { return (V *)(m + par * par_bytes + (size_t)pmax * sizeof(Slot)); }
-    __device__ V *diag(char *m, int par) const
-    {
-        return (V *)(m + par * par_bytes + (size_t)pmax * sizeof(Slot) + (size_t)pmax * NB * sizeof(V));
-    }
+    __device__ uint64_t *slot(char *m, int par) const { return (uint64_t *)(m + par * par_bytes); }
+    __device__ uint64_t *cand(char *m, int par) const { return slot(m, par) + (size_t)pmax * 4; }
+    __device__ uint64_t *diag(char *m, int par) const { return cand(m, par) + (size_t)pmax * NW; }
     __device__ int *perm(char *m) const { return (int *)(m + perm_off); }
 };
+
+__device__ inline void put_pkt(uint64_t *p, uint32_t tag, uint32_t v)
+{
+    __hip_atomic_store(p, ((uint64_t)tag << 32) | v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ inline uint64_t ld_pkt(const uint64_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Bounded-wait bookkeeping for a polling lane: true if the lane must stop
+// (the error word is set, or this lane ran out of spins and set it).
+__device__ inline bool spin_give_up(long &it, int *err)
+{
+    if ((++it & 255) == 0) {
+        if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return true;
+        if (it > kSpinLimit) {
+            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return true;
+        }
+    }
+    __builtin_amdgcn_s_sleep(1);
+    return false;
+}
 
 template <typename T>
 struct PanelArgs {
@@ -76,10 +108,10 @@ struct PanelArgs {
     int *ipiv;
     int64_t strideP;
     int *info;
-    int *cnt;
     int *err;
     char *xbuf;
     const int *bidx;  // launch member -> matrix index (null: identity)
+    uint32_t tag0;    // packet tags of this launch are tag0 + 1 .. tag0 + NB
 };
 
 // Argmax combine: larger value wins, equal values go to the lower row.
@@ -92,6 +124,8 @@ __global__ __launch_bounds__(256) void k_panel(PanelArgs<T> a)
     constexpr int W = NB / TPR;    // columns per thread
     constexpr int RPW = 64 / TPR;  // rows per wave
     constexpr int RW = 4 * RPW;    // rows per workgroup
+    constexpr int NW = XLayout<V>::NW;
+    static_assert(NW <= 256 && RW >= NB, "panel layout");
     const int bl = blockIdx.y, w = blockIdx.x;          // bl: member of this launch
     const int b = a.bidx ? a.bidx[bl] : bl;              // b: matrix index
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -118,16 +152,18 @@ __global__ __launch_bounds__(256) void k_panel(PanelArgs<T> a)
 
     __shared__ double s_wv[4];
     __shared__ int s_wr[4];
-    __shared__ V s_u[NB];
-    __shared__ V s_d[NB];
+    __shared__ V s_c4[4][NB];  // best row of each wave
+    __shared__ V s_dg[NB];     // row k+j (workgroup 0)
+    __shared__ V s_u[NB];      // pivot row
+    __shared__ V s_d[NB];      // row k+j, for the workgroup holding the pivot row
     __shared__ double s_gv;
     __shared__ int s_gr, s_gw;
-    __shared__ double2 s_rinv;
 
     for (int j = 0; j < NB; ++j) {
         const int par = j & 1;
         const int pr = k + j;
         const int cj = j / W;
+        const uint32_t tag = a.tag0 + (uint32_t)j + 1u;
         // current value of column j in this thread's row (owner chunk only)
         T xr = 0, xi = 0;
 #pragma unroll
@@ -145,46 +181,56 @@ __global__ __launch_bounds__(256) void k_panel(PanelArgs<T> a)
             const int r2 = __shfl_xor(r, off);
             if (better(v2, r2, v, r)) { v = v2; r = r2; }
         }
-        if (lane == 0) { s_wv[wave] = v; s_wr[wave] = r; }
-        __syncthreads();
-        double wv = s_wv[0];
-        int wr = s_wr[0];
+        // stage each wave's best row and row k+j in LDS
+        if (valid && row == r) {
 #pragma unroll
-        for (int q = 1; q < 4; ++q)
-            if (better(s_wv[q], s_wr[q], wv, wr)) { wv = s_wv[q]; wr = s_wr[q]; }
-
-        // publish this workgroup's candidate, its row, and (workgroup 0) row pr
-        Slot *slots = X.slots(xm, par);
-        V *cand = X.cand(xm, par);
-        V *diag = X.diag(xm, par);
-        bool wrote = false;
-        if (tid == 0) {
-            Slot s;
-            s.v = wv;
-            s.row = wr;
-            s.pad = 0;
-            slots[w] = s;
-            wrote = true;
-        }
-        if (valid && row == wr) {
-#pragma unroll
-            for (int i = 0; i < W; ++i) cand[(size_t)w * NB + chunk * W + i] = V{re[i], im[i]};
-            wrote = true;
+            for (int i = 0; i < W; ++i) s_c4[wave][chunk * W + i] = V{re[i], im[i]};
         }
         if (valid && row == pr) {
 #pragma unroll
-            for (int i = 0; i < W; ++i) diag[chunk * W + i] = V{re[i], im[i]};
-            wrote = true;
+            for (int i = 0; i < W; ++i) s_dg[chunk * W + i] = V{re[i], im[i]};
         }
-        if (wrote) __threadfence();
-        cluster_barrier(a.cnt + bl, (j + 1) * a.P, a.err);
+        if (lane == 0) { s_wv[wave] = v; s_wr[wave] = r; }
+        __syncthreads();
 
+        // publish this workgroup's candidate and (workgroup 0) row k+j
+        int q = 0;
+        double wv = s_wv[0];
+        int wr = s_wr[0];
+#pragma unroll
+        for (int q2 = 1; q2 < 4; ++q2)
+            if (better(s_wv[q2], s_wr[q2], wv, wr)) { wv = s_wv[q2]; wr = s_wr[q2]; q = q2; }
+        uint64_t *slot = X.slot(xm, par);
+        uint64_t *cand = X.cand(xm, par);
+        uint64_t *diag = X.diag(xm, par);
+        if (tid < NW) {
+            put_pkt(cand + (size_t)w * NW + tid, tag, ((const uint32_t *)s_c4[q])[tid]);
+            if (w == 0) put_pkt(diag + tid, tag, ((const uint32_t *)s_dg)[tid]);
+        }
+        if (tid < 3) {
+            const uint64_t bits = (uint64_t)__double_as_longlong(wv);
+            const uint32_t word = tid == 0 ? (uint32_t)bits : tid == 1 ? (uint32_t)(bits >> 32) : (uint32_t)wr;
+            put_pkt(slot + (size_t)w * 4 + tid, tag, word);
+        }
+
+        // every workgroup reduces all candidates to the same pivot
         if (wave == 0) {
             double gv = -1.0;
             int gr = 0x7fffffff, gw = 0;
             for (int s = lane; s < a.P; s += 64) {
-                const double v2 = __hip_atomic_load(&slots[s].v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                const int r2 = __hip_atomic_load(&slots[s].row, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const uint64_t *ps = slot + (size_t)s * 4;
+                uint64_t w0, w1, w2;
+                long it = 0;
+                for (;;) {
+                    w0 = ld_pkt(ps);
+                    w1 = ld_pkt(ps + 1);
+                    w2 = ld_pkt(ps + 2);
+                    if ((uint32_t)(w0 >> 32) == tag && (uint32_t)(w1 >> 32) == tag && (uint32_t)(w2 >> 32) == tag)
+                        break;
+                    if (spin_give_up(it, a.err)) break;
+                }
+                const double v2 = __longlong_as_double((long long)(((w1 & 0xffffffffull) << 32) | (w0 & 0xffffffffull)));
+                const int r2 = (int)(uint32_t)w2;
                 if (better(v2, r2, gv, gr)) { gv = v2; gr = r2; gw = s; }
             }
 #pragma unroll
@@ -199,21 +245,29 @@ __global__ __launch_bounds__(256) void k_panel(PanelArgs<T> a)
         __syncthreads();
         const double gv = s_gv;
         const int gr = s_gr, gw = s_gw;
-        const bool own_gr = gr >= k + w * RW && gr < k + (w + 1) * RW;
-        if (tid < NB) {
-            s_u[tid] = cand[(size_t)gw * NB + tid];
-            if (own_gr) s_d[tid] = diag[tid];
-        }
-        __syncthreads();
-        if (tid == 0) {
-            const V p = s_u[j];
-            s_rinv = (gv != 0.0) ? crecip((double)p.x, (double)p.y) : make_double2(0.0, 0.0);
-            if (w == 0) {
-                a.ipiv[(size_t)b * a.strideP + pr] = gr + 1;
-                if (gv == 0.0 && a.info[b] == 0) a.info[b] = pr + 1;
+        const bool own_gr = gr != pr && gr >= k + w * RW && gr < k + (w + 1) * RW;
+        if (tid < NW) {
+            const uint64_t *pu = cand + (size_t)gw * NW + tid;
+            const uint64_t *pd = diag + tid;
+            uint64_t wu, wd = 0;
+            long it = 0;
+            for (;;) {
+                wu = ld_pkt(pu);
+                if (own_gr) wd = ld_pkt(pd);
+                if ((uint32_t)(wu >> 32) == tag && (!own_gr || (uint32_t)(wd >> 32) == tag)) break;
+                if (spin_give_up(it, a.err)) break;
             }
+            ((uint32_t *)s_u)[tid] = (uint32_t)wu;
+            ((uint32_t *)s_d)[tid] = (uint32_t)wd;
         }
         __syncthreads();
+
+        const V p = s_u[j];
+        const double2 rinv = (gv != 0.0) ? crecip((double)p.x, (double)p.y) : make_double2(0.0, 0.0);
+        if (w == 0 && tid == 0) {
+            a.ipiv[(size_t)b * a.strideP + pr] = gr + 1;
+            if (gv == 0.0 && a.info[b] == 0) a.info[b] = pr + 1;
+        }
         if (gr != pr && valid) {
             if (row == gr) {
 #pragma unroll
@@ -234,9 +288,8 @@ __global__ __launch_bounds__(256) void k_panel(PanelArgs<T> a)
             ci = __shfl(ci, src);
         }
         if (gv != 0.0 && valid && row > pr) {
-            const double2 ri = s_rinv;
-            const T lr = (T)((double)cr * ri.x - (double)ci * ri.y);
-            const T li = (T)((double)cr * ri.y + (double)ci * ri.x);
+            const T lr = (T)((double)cr * rinv.x - (double)ci * rinv.y);
+            const T li = (T)((double)cr * rinv.y + (double)ci * rinv.x);
 #pragma unroll
             for (int i = 0; i < W; ++i) {
                 const int col = chunk * W + i;
@@ -248,7 +301,6 @@ __global__ __launch_bounds__(256) void k_panel(PanelArgs<T> a)
                 }
             }
         }
-        __syncthreads();
     }
 
 #pragma unroll
@@ -281,10 +333,20 @@ __global__ __launch_bounds__(256) void k_panel(PanelArgs<T> a)
 }
 
 // ---------------------------------------------------------------------------
-// Row interchanges outside the panel and U12 = L11^{-1} A12.
+// Row interchanges outside the panel and U12 = L11^{-1} A12.  Tiles of CB
+// columns, three kinds by tile index:
+//   [0, ntl)           columns [tile*CB, k): interchanges only (L to the left)
+//   [ntl, ntl+ntr)     columns [c_tr0, c_tr1): interchanges (if do_swap) and
+//                      the triangular solve of rows k..k+NB-1
+//   [ntl+ntr, ...)     columns [c_sw0, n): interchanges only (columns whose
+//                      rows k..k+NB-1 still await earlier panels' updates)
+struct SwapArgs {
+    int n, k, pmax, ntl, ntr, c_tr0, c_tr1, c_sw0, do_swap;
+};
+
 template <typename T, int CB>
-__global__ __launch_bounds__(256) void k_swap_trsm(typename CT<T>::V *A, int lda, int64_t strideA, int n, int k,
-                                                   int pmax, char *xbuf, int ntile_left, int *cnt, const int *bidx)
+__global__ __launch_bounds__(256) void k_swap_trsm(typename CT<T>::V *A, int lda, int64_t strideA, SwapArgs a,
+                                                   char *xbuf, const int *bidx)
 {
     using V = typename CT<T>::V;
     constexpr int RG = 256 / CB;        // row groups for the triangular solve
@@ -292,44 +354,51 @@ __global__ __launch_bounds__(256) void k_swap_trsm(typename CT<T>::V *A, int lda
     constexpr int EPT = 2 * NB * CB / 256;  // permuted elements per thread (max)
     const int bl = blockIdx.y, tile = blockIdx.x, tid = threadIdx.x;
     const int b = bidx ? bidx[bl] : bl;
+    const int n = a.n, k = a.k;
     V *Ab = A + (size_t)b * strideA;
-    const XLayout<V> X(pmax);
+    const XLayout<V> X(a.pmax);
     char *xm = xbuf + (size_t)bl * X.mat_bytes;
-    if (tile == 0 && tid == 0) cnt[bl] = 0;  // panel barrier counter, for the next panel
 
-    const bool left = tile < ntile_left;
-    const int c0 = left ? tile * CB : k + NB + (tile - ntile_left) * CB;
-    const int cend = left ? k : n;
-
-    __shared__ int s_rows[2 * NB], s_src[2 * NB];
-    __shared__ int s_cnt;
-    const int *pm = X.perm(xm);
-    if (tid == 0) s_cnt = pm[0];
-    if (tid < 2 * NB) { s_rows[tid] = pm[1 + tid]; s_src[tid] = pm[1 + 2 * NB + tid]; }
-    __syncthreads();
-    const int tcnt = s_cnt;
+    const bool trsm = tile >= a.ntl && tile < a.ntl + a.ntr;
+    const int c0 = tile < a.ntl ? tile * CB : trsm ? a.c_tr0 + (tile - a.ntl) * CB
+                                                   : a.c_sw0 + (tile - a.ntl - a.ntr) * CB;
+    const int cend = tile < a.ntl ? k : trsm ? a.c_tr1 : n;
 
     // gather the old contents of the touched rows, keep the new top block
-    // (rows k..k+NB-1) of right-hand columns in LDS, then scatter
+    // (rows k..k+NB-1) of solved columns in LDS, then scatter
     __shared__ V s_top[NB][CB];
-    V val[EPT];
+    if (a.do_swap) {
+        __shared__ int s_rows[2 * NB], s_src[2 * NB];
+        __shared__ int s_cnt;
+        const int *pm = X.perm(xm);
+        if (tid == 0) s_cnt = pm[0];
+        if (tid < 2 * NB) { s_rows[tid] = pm[1 + tid]; s_src[tid] = pm[1 + 2 * NB + tid]; }
+        __syncthreads();
+        const int tcnt = s_cnt;
+        V val[EPT];
 #pragma unroll
-    for (int e = 0; e < EPT; ++e) {
-        const int idx = tid + 256 * e;
-        const int t = idx / CB, cc = idx % CB, col = c0 + cc;
-        if (t < tcnt && col < cend) {
-            val[e] = Ab[s_src[t] + (size_t)col * lda];
-            if (!left && t < NB) s_top[t][cc] = val[e];
+        for (int e = 0; e < EPT; ++e) {
+            const int idx = tid + 256 * e;
+            const int t = idx / CB, cc = idx % CB, col = c0 + cc;
+            if (t < tcnt && col < cend) {
+                val[e] = Ab[s_src[t] + (size_t)col * lda];
+                if (trsm && t < NB) s_top[t][cc] = val[e];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < EPT; ++e) {
+            const int idx = tid + 256 * e;
+            const int t = idx / CB, col = c0 + idx % CB;
+            if (t < tcnt && col < cend) Ab[s_rows[t] + (size_t)col * lda] = val[e];
+        }
+    } else if (trsm) {
+        for (int idx = tid; idx < NB * CB; idx += 256) {
+            const int t = idx / CB, cc = idx % CB, col = c0 + cc;
+            if (col < cend) s_top[t][cc] = Ab[(k + t) + (size_t)col * lda];
         }
     }
-    __syncthreads();
-#pragma unroll
-    for (int e = 0; e < EPT; ++e) {
-        const int idx = tid + 256 * e;
-        const int t = idx / CB, col = c0 + idx % CB;
-        if (t < tcnt && col < cend) Ab[s_rows[t] + (size_t)col * lda] = val[e];
-    }
-    if (left) return;
+    if (!trsm) return;
 
     // forward substitution with the unit lower L11 (strictly lower part in LDS)
     __shared__ V s_l[NB * (NB - 1) / 2];
@@ -340,7 +409,7 @@ __global__ __launch_bounds__(256) void k_swap_trsm(typename CT<T>::V *A, int lda
     }
     __syncthreads();
     const int c = tid % CB, rg = tid / CB;
-    const bool cok = (c0 + c) < n;
+    const bool cok = (c0 + c) < cend;
     T xr[RPT], xi[RPT];
 #pragma unroll
     for (int q = 0; q < RPT; ++q) {
@@ -374,19 +443,24 @@ __global__ __launch_bounds__(256) void k_swap_trsm(typename CT<T>::V *A, int lda
 }
 
 // ---------------------------------------------------------------------------
-// Trailing update  C (M x M) -= A (M x NB) * B (NB x M)  on the matrix cores.
+// Block update  C (M x N) -= A (M x K) * B (K x N)  on the matrix cores.
 //
 // Both kernels issue the MFMA on the transposed problem, C^T = B^T A^T:
 // MFMA rows are columns of C and MFMA columns are rows of C, so each
 // accumulator register of a lane maps to a run of consecutive rows of one
 // column of C and the read-modify-write of C is coalesced.  Complex products
-// take four real MFMAs (re*re - im*im, re*im + im*re).  K = NB = 64 is walked
-// in chunks of 16 staged through LDS; the next chunk is fetched from global
+// take four real MFMAs (re*re - im*im, re*im + im*re).  K is walked in
+// chunks of 16 staged through LDS; the next chunk is fetched from global
 // memory into registers while the current one is multiplied.
 //
 // Tiles are ordered column by column and dealt to the XCDs in contiguous runs
 // (workgroup ids go to the 8 XCDs round-robin), so tiles that share a block
 // column of U12 share an L2.
+
+// C = A[r0.., c0..] (mrows x ncols) -= A[r0.., kk..kk+K) * A[kk..kk+K, c0..); K a multiple of 16.
+struct GemmArgs {
+    int r0, c0, mrows, ncols, kk, K;
+};
 
 // Tile of workgroup g out of T, grouped per XCD; >= T means "no tile".
 __device__ inline int xcd_tile(int g, int T)
@@ -402,18 +476,18 @@ inline int xcd_grid(int T) { return 8 * ((T + 7) / 8); }
 // D[l/16 + 4*v][l%16] (measured on gfx950).
 constexpr int GD_BM = 128, GD_BN = 64, GD_KC = 16;
 
-__global__ __launch_bounds__(256, 2) void k_gemm_f64(double2 *Abase, int lda, int64_t strideA, int n, int k,
+__global__ __launch_bounds__(256, 2) void k_gemm_f64(double2 *Abase, int lda, int64_t strideA, GemmArgs g,
                                                      const int *bidx, int TI, int TJ)
 {
     const int t = xcd_tile(blockIdx.x, TI * TJ);
     if (t >= TI * TJ) return;
     const int b = bidx ? bidx[blockIdx.z] : (int)blockIdx.z;
     double2 *M = Abase + (size_t)b * strideA;
-    const int r0 = k + NB, mrows = n - r0;
+    const int mrows = g.mrows, ncols = g.ncols, K = g.K;
     const int i0 = (t % TI) * GD_BM, j0 = (t / TI) * GD_BN;
-    const double2 *A = M + r0 + (size_t)k * lda;  // L21
-    const double2 *B = M + k + (size_t)r0 * lda;  // U12
-    double2 *C = M + r0 + (size_t)r0 * lda;       // A22
+    const double2 *A = M + g.r0 + (size_t)g.kk * lda;  // L block
+    const double2 *B = M + g.kk + (size_t)g.c0 * lda;  // U block
+    double2 *C = M + g.r0 + (size_t)g.c0 * lda;        // updated block
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wi = (wave & 1) * 64, wj = (wave >> 1) * 32;
 
@@ -429,7 +503,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_f64(double2 *Abase, int lda, in
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             const int idx = tid + 256 * e, kk = idx % GD_KC, j = idx / GD_KC, gj = j0 + j;
-            pb[e] = gj < mrows ? B[(kc + kk) + (size_t)gj * lda] : double2{0.0, 0.0};
+            pb[e] = gj < ncols ? B[(kc + kk) + (size_t)gj * lda] : double2{0.0, 0.0};
         }
     };
     auto stash = [&]() {
@@ -453,8 +527,8 @@ __global__ __launch_bounds__(256, 2) void k_gemm_f64(double2 *Abase, int lda, in
     fetch(0);
     stash();
     __syncthreads();
-    for (int kc = 0; kc < NB; kc += GD_KC) {
-        if (kc + GD_KC < NB) fetch(kc + GD_KC);
+    for (int kc = 0; kc < K; kc += GD_KC) {
+        if (kc + GD_KC < K) fetch(kc + GD_KC);
 #pragma unroll
         for (int ks = 0; ks < GD_KC; ks += 4) {
             const int kk = ks + lane / 16;
@@ -485,7 +559,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_f64(double2 *Abase, int lda, in
                 for (int ti = 0; ti < 4; ++ti)
                     accI[tj][ti] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[tj].y, av[ti].x, accI[tj][ti], 0, 0, 0);
         }
-        if (kc + GD_KC < NB) {
+        if (kc + GD_KC < K) {
             __syncthreads();
             stash();
             __syncthreads();
@@ -499,7 +573,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_f64(double2 *Abase, int lda, in
             for (int v = 0; v < 4; ++v) {
                 const int gi = i0 + wi + ti * 16 + lane % 16;
                 const int gj = j0 + wj + tj * 16 + (lane / 16) + 4 * v;
-                if (gi < mrows && gj < mrows) {
+                if (gi < mrows && gj < ncols) {
                     double2 c = C[gi + (size_t)gj * lda];
                     c.x -= accR[tj][ti][v];
                     c.y -= accI[tj][ti][v];
@@ -513,18 +587,18 @@ __global__ __launch_bounds__(256, 2) void k_gemm_f64(double2 *Abase, int lda, in
 // srcB[l/32][l%32] and D[8*(v/4)+4*(l/32)+v%4][l%32].
 constexpr int GS_BM = 128, GS_BN = 128, GS_KC = 16;
 
-__global__ __launch_bounds__(256, 2) void k_gemm_f32(float2 *Abase, int lda, int64_t strideA, int n, int k,
+__global__ __launch_bounds__(256, 2) void k_gemm_f32(float2 *Abase, int lda, int64_t strideA, GemmArgs g,
                                                      const int *bidx, int TI, int TJ)
 {
     const int t = xcd_tile(blockIdx.x, TI * TJ);
     if (t >= TI * TJ) return;
     const int b = bidx ? bidx[blockIdx.z] : (int)blockIdx.z;
     float2 *M = Abase + (size_t)b * strideA;
-    const int r0 = k + NB, mrows = n - r0;
+    const int mrows = g.mrows, ncols = g.ncols, K = g.K;
     const int i0 = (t % TI) * GS_BM, j0 = (t / TI) * GS_BN;
-    const float2 *A = M + r0 + (size_t)k * lda;
-    const float2 *B = M + k + (size_t)r0 * lda;
-    float2 *C = M + r0 + (size_t)r0 * lda;
+    const float2 *A = M + g.r0 + (size_t)g.kk * lda;
+    const float2 *B = M + g.kk + (size_t)g.c0 * lda;
+    float2 *C = M + g.r0 + (size_t)g.c0 * lda;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wi = (wave & 1) * 64, wj = (wave >> 1) * 64;
 
@@ -541,7 +615,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_f32(float2 *Abase, int lda, int
         for (int e = 0; e < 8; ++e) pa[e] = arow ? pA[(size_t)(2 * e) * lda] : float2{0.f, 0.f};
         const float2 *pB = B + (kc + bk) + (size_t)(j0 + bj) * lda;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) pb[e] = (j0 + bj + 16 * e < mrows) ? pB[(size_t)(16 * e) * lda] : float2{0.f, 0.f};
+        for (int e = 0; e < 8; ++e) pb[e] = (j0 + bj + 16 * e < ncols) ? pB[(size_t)(16 * e) * lda] : float2{0.f, 0.f};
     };
     auto stash = [&]() {
 #pragma unroll
@@ -560,8 +634,8 @@ __global__ __launch_bounds__(256, 2) void k_gemm_f32(float2 *Abase, int lda, int
     fetch(0);
     stash();
     __syncthreads();
-    for (int kc = 0; kc < NB; kc += GS_KC) {
-        if (kc + GS_KC < NB) fetch(kc + GS_KC);
+    for (int kc = 0; kc < K; kc += GS_KC) {
+        if (kc + GS_KC < K) fetch(kc + GS_KC);
 #pragma unroll
         for (int ks = 0; ks < GS_KC; ks += 2) {
             const int kk = ks + lane / 32;
@@ -592,7 +666,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_f32(float2 *Abase, int lda, int
                 for (int ti = 0; ti < 2; ++ti)
                     accI[tj][ti] = __builtin_amdgcn_mfma_f32_32x32x2f32(bv[tj].y, av[ti].x, accI[tj][ti], 0, 0, 0);
         }
-        if (kc + GS_KC < NB) {
+        if (kc + GS_KC < K) {
             __syncthreads();
             stash();
             __syncthreads();
@@ -606,7 +680,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_f32(float2 *Abase, int lda, int
             for (int v = 0; v < 16; ++v) {
                 const int gi = i0 + wi + ti * 32 + lane % 32;
                 const int gj = j0 + wj + tj * 32 + 8 * (v / 4) + 4 * (lane / 32) + (v % 4);
-                if (gi < mrows && gj < mrows) {
+                if (gi < mrows && gj < ncols) {
                     float2 c = C[gi + (size_t)gj * lda];
                     c.x -= accR[tj][ti][v];
                     c.y -= accI[tj][ti][v];
@@ -615,19 +689,20 @@ __global__ __launch_bounds__(256, 2) void k_gemm_f32(float2 *Abase, int lda, int
             }
 }
 
-// Launches the trailing update of step k for nb_ matrices.
-void launch_gemm(int esz, void *A, int lda, int64_t strideA, int n, int k, int nb_, const int *bidx, hipStream_t s)
+// Launches C -= L U for nb_ matrices, C = A[r0.., c0..] (mrows x ncols),
+// L = A[r0.., kk..kk+K), U = A[kk..kk+K, c0..).
+void launch_gemm(int esz, void *A, int lda, int64_t strideA, const GemmArgs &g, int nb_, const int *bidx,
+                 hipStream_t s)
 {
-    const int m2 = n - k - NB;
-    if (m2 <= 0) return;
+    if (g.mrows <= 0 || g.ncols <= 0 || g.K <= 0) return;
     if (esz == 8) {
-        const int TI = (m2 + GD_BM - 1) / GD_BM, TJ = (m2 + GD_BN - 1) / GD_BN;
+        const int TI = (g.mrows + GD_BM - 1) / GD_BM, TJ = (g.ncols + GD_BN - 1) / GD_BN;
         hipLaunchKernelGGL(k_gemm_f64, dim3(xcd_grid(TI * TJ), 1, nb_), dim3(256), 0, s, (double2 *)A, lda, strideA,
-                           n, k, bidx, TI, TJ);
+                           g, bidx, TI, TJ);
     } else {
-        const int TI = (m2 + GS_BM - 1) / GS_BM, TJ = (m2 + GS_BN - 1) / GS_BN;
+        const int TI = (g.mrows + GS_BM - 1) / GS_BM, TJ = (g.ncols + GS_BN - 1) / GS_BN;
         hipLaunchKernelGGL(k_gemm_f32, dim3(xcd_grid(TI * TJ), 1, nb_), dim3(256), 0, s, (float2 *)A, lda, strideA,
-                           n, k, bidx, TI, TJ);
+                           g, bidx, TI, TJ);
     }
 }
 
@@ -641,6 +716,42 @@ __global__ void k_zero_idx(int *p, const int *idx, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[idx[i]] = 0;
+}
+
+__global__ void k_zero_u64(uint64_t *p, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// Exchange buffer of at least `bytes` for the panel launches of one call that
+// together use `ntags` tags.  It is cleared when (re)allocated and when the
+// tag counter could wrap during the call, so no stale packet can carry a
+// current tag.  Clearing is queued on s; work of earlier calls on other
+// streams has been joined into s.
+int prepare_xbuf(wm_ctx *ctx, size_t bytes, uint64_t ntags, hipStream_t s)
+{
+    if (ntags > 0x40000000u) {
+        set_error("getrf: %llu panel tags in one call", (unsigned long long)ntags);
+        return WM_ERR_ARG;
+    }
+    const size_t had = ctx->ws_lu.bytes;
+    WM_TRY(ctx->ws_lu.ensure(bytes));
+    if (ctx->ws_lu.bytes != had || (uint64_t)ctx->panel_tag + ntags + NB > 0xFFFFFFFFull) {
+        const size_t nw = ctx->ws_lu.bytes / sizeof(uint64_t);
+        hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, s,
+                           (uint64_t *)ctx->ws_lu.ptr, nw);
+        ctx->panel_tag = 0;
+    }
+    return WM_OK;
+}
+
+// Tag base of the next panel launch (tags base+1 .. base+NB).
+uint32_t next_tag(wm_ctx *ctx)
+{
+    const uint32_t t = ctx->panel_tag;
+    ctx->panel_tag += NB;
+    return t;
 }
 
 template <typename T, int TPR>
@@ -674,43 +785,89 @@ int getrf_impl(wm_ctx *ctx, int n, typename CT<T>::V *A, int lda, int64_t stride
     int per_launch = resident / pmax;
     if (per_launch > batch) per_launch = batch;
     if (per_launch > wm_ctx::kSyncCap) per_launch = wm_ctx::kSyncCap;
-    WM_TRY(ctx->ws_lu.ensure((size_t)per_launch * X.mat_bytes));
-    char *xbuf = (char *)ctx->ws_lu.ptr;
+    const int nchunks = (batch + per_launch - 1) / per_launch;
+    WM_TRY(prepare_xbuf(ctx, (size_t)per_launch * X.mat_bytes, (uint64_t)nchunks * n, s));
+    char *xbase = (char *)ctx->ws_lu.ptr;
+    const double esz = 2.0 * sizeof(T);
+
+    // One launch group: m matrices factored together on one stream.
+    struct Grp {
+        V *Ab;
+        int *ipb, *infob;
+        const int *bidx;
+        int m;
+        char *xbuf;
+        hipStream_t st;
+    };
+    auto panel = [&](const Grp &G, int c) {
+        PanelArgs<T> pa;
+        pa.A = G.Ab; pa.lda = lda; pa.strideA = strideA; pa.n = n; pa.k = c;
+        pa.P = (n - c + RW - 1) / RW; pa.pmax = pmax;
+        pa.ipiv = G.ipb; pa.strideP = strideP; pa.info = G.infob;
+        pa.err = ctx->d_err; pa.xbuf = G.xbuf; pa.bidx = G.bidx; pa.tag0 = next_tag(ctx);
+        const double mr = n - c;
+        // per column j, (mr-j) scalings and (mr-j)(NB-j-1) complex multiply-adds
+        prof_begin(ctx, K_PANEL, G.st);
+        hipLaunchKernelGGL((k_panel<T, TPR>), dim3(pa.P, G.m), dim3(256), 0, G.st, pa);
+        prof_end(ctx, K_PANEL, G.st, G.m * 8.0 * (mr * NB * NB / 2.0 - (double)NB * NB * NB / 6.0),
+                 G.m * 2.0 * mr * NB * esz);
+    };
+    // interchanges of the panel at c left of c; solve for [tr0, tr1); interchanges only for [tr1, n)
+    auto swap = [&](const Grp &G, int c, int tr1, int do_swap, int tr0) {
+        SwapArgs sa;
+        sa.n = n; sa.k = c; sa.pmax = pmax; sa.do_swap = do_swap;
+        sa.ntl = do_swap ? (c + CB - 1) / CB : 0;
+        sa.c_tr0 = tr0; sa.c_tr1 = tr1;
+        sa.ntr = tr1 > tr0 ? (tr1 - tr0 + CB - 1) / CB : 0;
+        sa.c_sw0 = tr1;
+        const int nsw = do_swap && n > tr1 ? (n - tr1 + CB - 1) / CB : 0;
+        if (sa.ntl + sa.ntr + nsw == 0) return;
+        prof_begin(ctx, K_SWAP, G.st);
+        hipLaunchKernelGGL((k_swap_trsm<T, CB>), dim3(sa.ntl + sa.ntr + nsw, G.m), dim3(256), 0, G.st, G.Ab, lda,
+                           strideA, sa, G.xbuf, G.bidx);
+        prof_end(ctx, K_SWAP, G.st, G.m * 4.0 * NB * NB * (tr1 - tr0), G.m * 4.0 * NB * n * esz);
+    };
+    auto gemm = [&](const Grp &G, int r0, int c0, int mrows, int ncols, int kk, int K) {
+        if (mrows <= 0 || ncols <= 0) return;
+        const GemmArgs g{r0, c0, mrows, ncols, kk, K};
+        // C read and written, L and U read once
+        prof_begin(ctx, K_GEMM, G.st);
+        launch_gemm((int)sizeof(T), G.Ab, lda, strideA, g, G.m, G.bidx, G.st);
+        prof_end(ctx, K_GEMM, G.st, G.m * 8.0 * mrows * ncols * K,
+                 G.m * (2.0 * mrows * ncols + (double)(mrows + ncols) * K) * esz);
+    };
 
     for (int b0 = 0; b0 < batch; b0 += per_launch) {
-        const int nb_ = (batch - b0 < per_launch) ? batch - b0 : per_launch;
+        Grp G;
+        G.m = (batch - b0 < per_launch) ? batch - b0 : per_launch;
         // with an index table the bases stay put and members are looked up
-        V *Ab = d_bidx ? A : A + (size_t)b0 * strideA;
-        int *ipb = d_bidx ? ipiv : ipiv + (size_t)b0 * strideP;
-        int *infob = d_bidx ? d_info : d_info + b0;
-        const int *bidx = d_bidx ? d_bidx + b0 : nullptr;
-        hipLaunchKernelGGL(k_zero_int, dim3((nb_ + 255) / 256), dim3(256), 0, s, ctx->d_sync, nb_);
-        for (int k = 0; k < n; k += NB) {
-            PanelArgs<T> pa;
-            pa.A = Ab; pa.lda = lda; pa.strideA = strideA; pa.n = n; pa.k = k;
-            pa.P = (n - k + RW - 1) / RW; pa.pmax = pmax;
-            pa.ipiv = ipb; pa.strideP = strideP; pa.info = infob;
-            pa.cnt = ctx->d_sync; pa.err = ctx->d_err; pa.xbuf = xbuf; pa.bidx = bidx;
-            const double mr = n - k, m2 = n - k - NB, esz = 2.0 * sizeof(T);
-            // panel: per column j, (mr-j) scalings and (mr-j)(NB-j-1) complex multiply-adds
-            prof_begin(ctx, K_PANEL, s);
-            hipLaunchKernelGGL((k_panel<T, TPR>), dim3(pa.P, nb_), dim3(256), 0, s, pa);
-            prof_end(ctx, K_PANEL, s, nb_ * 8.0 * (mr * NB * NB / 2.0 - (double)NB * NB * NB / 6.0),
-                     nb_ * 2.0 * mr * NB * esz);
-            const int ntl = (k + CB - 1) / CB;
-            const int ntr = (n - k - NB + CB - 1) / CB;
-            if (ntl + ntr > 0) {
-                prof_begin(ctx, K_SWAP, s);
-                hipLaunchKernelGGL((k_swap_trsm<T, CB>), dim3(ntl + ntr, nb_), dim3(256), 0, s, Ab, lda, strideA, n,
-                                   k, pmax, xbuf, ntl, ctx->d_sync, bidx);
-                prof_end(ctx, K_SWAP, s, nb_ * 4.0 * NB * NB * m2, nb_ * 4.0 * NB * n * esz);
+        G.Ab = d_bidx ? A : A + (size_t)b0 * strideA;
+        G.ipb = d_bidx ? ipiv : ipiv + (size_t)b0 * strideP;
+        G.infob = d_bidx ? d_info : d_info + b0;
+        G.bidx = d_bidx ? d_bidx + b0 : nullptr;
+        G.xbuf = xbase;
+        G.st = s;
+        // Outer blocks of kSub panels: inside a block the panels are applied to
+        // the block's own columns at once, while the columns to the right get
+        // their interchanges at once and their update from the whole block in
+        // one product with K = kSub * NB (rows of the later panels' solves are
+        // updated first by a short product).
+        for (int k = 0; k < n; k += kSub * NB) {
+            const int kend = (n - k < kSub * NB) ? n : k + kSub * NB;
+            for (int c = k; c < kend; c += NB) {
+                if (c > k) gemm(G, c, c, n - c, kend - c, c - NB, NB);
+                panel(G, c);
+                if (c == k) {
+                    swap(G, c, n, 1, c + NB);
+                } else {
+                    swap(G, c, kend, 1, c + NB);
+                    if (kend < n) {
+                        gemm(G, c, kend, NB, n - kend, k, c - k);
+                        swap(G, c, n, 0, kend);
+                    }
+                }
             }
-            if (m2 > 0) {
-                // trailing update: 8 M^2 NB real flops; C read and written, L21 and U12 read once
-                prof_begin(ctx, K_GEMM, s);
-                launch_gemm((int)sizeof(T), Ab, lda, strideA, n, k, nb_, bidx, s);
-                prof_end(ctx, K_GEMM, s, nb_ * 8.0 * m2 * m2 * NB, nb_ * (2.0 * m2 * m2 + 2.0 * m2 * NB) * esz);
-            }
+            gemm(G, kend, kend, n - kend, n - kend, k, kend - k);
         }
         WM_HIP_TRY(hipGetLastError());
     }
@@ -718,6 +875,16 @@ int getrf_impl(wm_ctx *ctx, int n, typename CT<T>::V *A, int lda, int64_t stride
 }
 
 }  // namespace
+
+int getrf_width(wm_ctx *ctx, int prec, int n)
+{
+    int bpc = 0;
+    const bool f64 = prec == WM_PREC_F64;
+    if ((f64 ? occupancy_panel<double, 2>(&bpc) : occupancy_panel<float, 1>(&bpc)) != WM_OK || bpc <= 0) return 1;
+    const int RW = f64 ? 128 : 256;
+    const int pmax = (n + RW - 1) / RW;
+    return std::max(1, bpc * ctx->num_cu / pmax);
+}
 
 int getrf_batched(wm_ctx *ctx, int prec, int n, void *A, int lda, int64_t strideA, int *ipiv, int64_t strideP,
                   int batch, int *d_info, hipStream_t s, const int *d_bidx)
@@ -748,7 +915,8 @@ extern "C" int wmx_gemm_step(wm_ctx *ctx, int prec, int n, void *A, int k)
     using namespace wm;
     const int m2 = n - k - NB;
     if (m2 <= 0 || n % NB || k % NB) return WM_ERR_ARG;
-    launch_gemm(prec == WM_PREC_F64 ? 8 : 4, A, n, 0, n, k, 1, nullptr, ctx->stream);
+    const GemmArgs g{k + NB, k + NB, m2, m2, k, NB};
+    launch_gemm(prec == WM_PREC_F64 ? 8 : 4, A, n, 0, g, 1, nullptr, ctx->stream);
     WM_HIP_TRY(hipStreamSynchronize(ctx->stream));
     return WM_OK;
 }
@@ -761,30 +929,34 @@ extern "C" int wmx_panel_step(wm_ctx *ctx, int prec, int n, void *A, int *ipiv, 
     int *info = nullptr;
     WM_HIP_TRY(hipMalloc(&info, sizeof(int)));
     WM_HIP_TRY(hipMemset(info, 0, sizeof(int)));
-    WM_HIP_TRY(hipMemset(ctx->d_sync, 0, sizeof(int)));
     auto run = [&](auto tag, auto *Ap, int TPRv) -> int {
         using T = decltype(tag);
         using V = typename CT<T>::V;
         const int RW = 256 / TPRv;
         const int pmax = (n + RW - 1) / RW;
         const XLayout<V> X(pmax);
-        WM_TRY(ctx->ws_lu.ensure(X.mat_bytes));
+        WM_TRY(prepare_xbuf(ctx, X.mat_bytes, NB, ctx->stream));
+        char *xbuf = (char *)ctx->ws_lu.ptr;
+        const uint32_t tag0 = next_tag(ctx);
         PanelArgs<T> pa;
         pa.A = (V *)Ap; pa.lda = n; pa.strideA = 0; pa.n = n; pa.k = k;
         pa.P = (n - k + RW - 1) / RW; pa.pmax = pmax;
         pa.ipiv = ipiv; pa.strideP = 0; pa.info = info;
-        pa.cnt = ctx->d_sync; pa.err = ctx->d_err; pa.xbuf = (char *)ctx->ws_lu.ptr; pa.bidx = nullptr;
+        pa.err = ctx->d_err; pa.xbuf = xbuf; pa.bidx = nullptr; pa.tag0 = tag0;
         constexpr int CB = (sizeof(T) == 8) ? 16 : 32;
         if (sizeof(T) == 8)
             hipLaunchKernelGGL((k_panel<double, 2>), dim3(pa.P, 1), dim3(256), 0, ctx->stream, *(PanelArgs<double> *)&pa);
         else
             hipLaunchKernelGGL((k_panel<float, 1>), dim3(pa.P, 1), dim3(256), 0, ctx->stream, *(PanelArgs<float> *)&pa);
         if (do_swap) {
-            const int ntl = (k + CB - 1) / CB, ntr = (n - k - NB + CB - 1) / CB;
-            if (ntl + ntr > 0)
-                hipLaunchKernelGGL((k_swap_trsm<T, CB>), dim3(ntl + ntr, 1), dim3(256), 0, ctx->stream, (V *)Ap, n,
-                                   (int64_t)0, n, k, pmax, (char *)ctx->ws_lu.ptr, ntl, ctx->d_sync,
-                                   (const int *)nullptr);
+            SwapArgs sa;
+            sa.n = n; sa.k = k; sa.pmax = pmax; sa.do_swap = 1;
+            sa.ntl = (k + CB - 1) / CB;
+            sa.ntr = (n - k - NB + CB - 1) / CB;
+            sa.c_tr0 = k + NB; sa.c_tr1 = n; sa.c_sw0 = n;
+            if (sa.ntl + sa.ntr > 0)
+                hipLaunchKernelGGL((k_swap_trsm<T, CB>), dim3(sa.ntl + sa.ntr, 1), dim3(256), 0, ctx->stream, (V *)Ap,
+                                   n, (int64_t)0, sa, xbuf, (const int *)nullptr);
         }
         return WM_OK;
     };

@@ -57,7 +57,7 @@ def test_fill_f32_copy_is_rounded_f64(wm, geometries, ctx, geo_dir):
     assert np.array_equal(Z32, Z64.astype(np.complex64))
 
 
-@pytest.mark.parametrize("n", [64, 128, 320])
+@pytest.mark.parametrize("n", [64, 128, 320, 576])
 def test_getrf_f64_matches_lapack(wm, ctx, oracle, n):
     rng = np.random.RandomState(n)
     A = rng.randn(2, n, n) + 1j * rng.randn(2, n, n)
@@ -237,3 +237,14 @@ def test_mixed_precision_fallback_band(wm, geometries, ctx, geo_dir):
     print([(float(fr), s["iters"], s["fell_back"], float(r)) for fr, s, r in zip(f, st, rel)])
     assert (rel <= 1e-10).all(), rel
     assert any(s["fell_back"] for s in st)
+    # batches of 4: the 4 highest frequencies are refined first and their
+    # contraction rates route the lowest ones to FP64 factors directly
+    z32b, _, stb = wm.solve_sweep(g, f, [gap], "f32", batch=4)
+    relb = np.abs(z32b[:, 0] - z64[:, 0]) / np.abs(z64[:, 0])
+    print([(float(fr), s["iters"], s["fell_back"], float(r)) for fr, s, r in zip(f, stb, relb)])
+    assert (relb <= 1e-10).all(), relb
+    routed = [s["fell_back"] == 2 for s in stb]
+    assert routed[0] and not any(routed[-4:])  # lowest routed, highest refined
+    for a, b in zip(st, stb):  # routing only skips refinements that fail
+        if b["fell_back"] == 2:
+            assert a["fell_back"] == 1

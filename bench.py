@@ -12,10 +12,13 @@ precision given by --prec); the plain FP64 GPU path is timed beside it and
 every Zin is compared with it.
 
 Also reported: the roofline of the dominant kernel class (HIP events on the
-launch stream, algorithmic flops per launch, MI355X datasheet peaks) and the
-CPU oracle timed on a bounded sample of the same workload on the host cores.
+launch stream, algorithmic flops per launch, MI355X datasheet peaks), a
+serving leg (a Poisson stream of requests on the continuous-batching server,
+p50/p99 latency and solves/s) and the CPU oracle timed on a bounded sample of
+the same workload on the host cores.
 
     python bench.py [--steps K] [--warmup W] [--prec f32|f64] [--nfreq F] [--no-cpu]
+                    [--no-serving] [--requests R] [--rate PER_S]
 """
 from __future__ import annotations
 
@@ -35,6 +38,13 @@ sys.path.insert(0, ROOT)
 PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3, "bf16x3": 2516.6 / 3.0}
 PEAK_FP64_VALU = 78.6
 PEAK_HBM_GBS = 8000.0
+# kernel classes timed by the library profiler: (bound, peak)
+CLASSES = {
+    "fill": ("fp64", PEAK_FP64_VALU),
+    "panel": ("fp64", 78.6), "swap_trsm": ("fp64", 78.6), "gemm": ("mfma", 78.6),
+    "panel_f32": ("fp32", 157.3), "swap_trsm_f32": ("fp32", 157.3), "gemm_f32": ("mfma", 157.3),
+    "trsv": ("hbm", PEAK_HBM_GBS), "resid": ("hbm", PEAK_HBM_GBS),
+}
 REF_BEST_PUBLISHED = 0.148  # freq-solves/s at N=8192, reference's best variant (GPU fill + MKL LU, 6 procs)
 
 
@@ -69,6 +79,36 @@ def cpu_baseline(geo: str, freq: float, threads: int):
                       f"fill {t_fill:.1f} s + LU {t_lu:.1f} s per freq-solve (extrapolated)"}
 
 
+def serving_leg(wm, ctx, tmp, count, rate):
+    """Poisson request stream against the continuous-batching server: planar
+    grids with N uniform in {512, ..., 8192}, 1-32 frequencies uniform in
+    10 kHz - 50 MHz, 1-8 feeds, FP32 factors + FP64 refinement."""
+    import wiremom.geometries as G
+    import wiremom.serving as S
+    reqs = S.poisson_requests(count, rate, seed=5)
+    geoms, feeds = [], []
+    for i, r in enumerate(reqs):
+        p = os.path.join(tmp, f"req{i}.geo")
+        G.planar_grid(p, r["n"], seed=r["seed"])
+        g = wm.Geometry.load(ctx, p)
+        geoms.append(g)
+        feeds.append(np.random.RandomState(r["seed"]).choice(g.npls, r["nrhs"], replace=False) + 1)
+    srv = wm.Server(ctx, "f32")
+    out = S.replay(srv, reqs, geoms, feeds)
+    srv.close()
+    lat = np.array(out["latency"])
+    nsolve = sum(len(r["freqs"]) for r in reqs)
+    st = [s for res in out["results"] for s in res[2]]
+    return {"requests": count, "arrival_rate_per_s": rate, "freq_solves": nsolve,
+            "rhs_solves": int(sum(len(r["freqs"]) * r["nrhs"] for r in reqs)),
+            "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
+            "max_latency_s": float(lat.max()), "solves_per_s": nsolve / out["makespan"],
+            "makespan_s": out["makespan"], "fell_back": int(sum(s["fell_back"] == 1 for s in st)),
+            "routed_f64": int(sum(s["fell_back"] == 2 for s in st)), "prec": "f32",
+            "workload": "Poisson stream; planar grids N in {512,1024,2048,4096,8192}, 1-32 freqs "
+                        "U(10 kHz, 50 MHz), 1-8 feeds"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,6 +118,9 @@ def main():
     ap.add_argument("--nfreq", type=int, default=64)
     ap.add_argument("--batch", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--requests", type=int, default=48)
+    ap.add_argument("--rate", type=float, default=4.0)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported "
@@ -109,7 +152,7 @@ def main():
         zin, I, st = sweep(a.prec)
     ctx.sync()
     dt = time.perf_counter() - t0
-    prof = {k: ctx.prof_get(k) for k in ("fill", "panel", "swap_trsm", "gemm", "trsv", "resid")}
+    prof = {k: ctx.prof_get(k) for k in CLASSES}
     ctx.prof_enable(False)
     value = a.steps * len(freqs) / dt
 
@@ -127,16 +170,13 @@ def main():
     peak_mfma = PEAK_TFLOPS.get(a.prec, 157.3)
     dom = max(prof, key=lambda k: prof[k][0])
     ms, nl, fl, by = prof[dom]
-    if dom in ("gemm", "panel", "swap_trsm"):
-        ach = fl / (ms * 1e-3) / 1e12
-        roof = {"bound": "mfma", "achieved": ach, "peak": peak_mfma, "unit": "TFLOP/s", "frac": ach / peak_mfma}
-    elif dom == "fill":
-        ach = fl / (ms * 1e-3) / 1e12
-        roof = {"bound": "mfma", "achieved": ach, "peak": PEAK_FP64_VALU, "unit": "TFLOP/s",
-                "frac": ach / PEAK_FP64_VALU}
-    else:
+    bound, peak = CLASSES[dom]
+    if bound == "hbm":
         ach = by / (ms * 1e-3) / 1e9
-        roof = {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": ach / PEAK_HBM_GBS}
+        roof = {"bound": "hbm", "achieved": ach, "peak": peak, "unit": "GB/s", "frac": ach / peak}
+    else:
+        ach = fl / (ms * 1e-3) / 1e12
+        roof = {"bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak}
     roof.update({"kernel": dom, "launches": nl, "avg_launch_us": 1e3 * ms / max(nl, 1), "traffic": None})
 
     # whole-solve roofline fraction: t_ideal / t_measured per freq-solve
@@ -146,13 +186,15 @@ def main():
                + (16 * N ** 2 + iters * 24 * N ** 2) / (PEAK_HBM_GBS * 1e9))
     solve_frac = t_ideal / (dt / (a.steps * len(freqs)))
 
+    serving = None if a.no_serving else serving_leg(wm, ctx, tmp, a.requests, a.rate)
+
     cpu = None
     if not a.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         cpu = cpu_baseline(geo, float(freqs[-1]), threads)
 
     out = {
-        "metric": "freq-solves/s (fill+LU) at N=8192",
+        "metric": "freq-solves/s (fill+LU) at N=8192, FP32-MFMA+IR vs FP64; p99 request latency",
         "value": value,
         "unit": "freq-solves/s",
         "n_gpus": 1,
@@ -176,6 +218,7 @@ def main():
         "solve_roofline_frac": solve_frac,
         "kernel_ms": {k: round(v[0] / a.steps, 3) for k, v in prof.items()},
         "roofline": roof,
+        "serving": serving,
         "cpu_baseline": cpu,
         "reference_best_published_solves_per_s": REF_BEST_PUBLISHED,
         "device": f"{name} x{ncu} CU",

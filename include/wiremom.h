@@ -183,6 +183,9 @@ int wm_server_step(wm_server *s, int wait, int *done);
  * latency = completion - submission in seconds.  Releases the request. */
 int wm_server_result(wm_server *s, int64_t id, wm_c128 *I, wm_c128 *zin, wm_freq_stats *st,
                      double *latency_s);
+/* Ids of completed requests whose results have not been read (ascending, at
+ * most cap of them). */
+int wm_server_done(wm_server *s, int64_t *ids, int cap, int *count);
 /* Queued, admitted and completed-but-unread request counts, HBM in use. */
 int wm_server_status(wm_server *s, int *queued, int *running, int *completed, size_t *hbm_used);
 int wm_server_destroy(wm_server *s);

@@ -33,7 +33,7 @@ EXPORTS = [
     "wm_dev_free", "wm_memcpy_h2d", "wm_memcpy_d2h", "wm_geom_create", "wm_geom_load", "wm_geom_destroy",
     "wm_geom_npls", "wm_geom_ld", "wm_zfill", "wm_getrf_batched", "wm_getrs_batched", "wm_solve_opts_default",
     "wm_solve_sweep", "wm_server_opts_default", "wm_server_create", "wm_server_submit", "wm_server_step",
-    "wm_server_result", "wm_server_status", "wm_server_destroy", "geoinp", "config", "configwire", "zmatrix",
+    "wm_server_result", "wm_server_status", "wm_server_done", "wm_server_destroy", "geoinp", "config", "configwire", "zmatrix",
     "zmatrix_gpu", "zmatrix_gpu_rank", "LU_cuBLAS", "CalculateByFreq",
 ]
 
@@ -105,6 +105,7 @@ def lib() -> C.CDLL:
         L.wm_server_step.argtypes = [vp, C.c_int, ip]
         L.wm_server_result.argtypes = [vp, C.c_int64, vp, vp, _p(FreqStats), dp]
         L.wm_server_status.argtypes = [vp, ip, ip, ip, _p(C.c_size_t)]
+        L.wm_server_done.argtypes = [vp, _p(C.c_int64), C.c_int, ip]
         L.wm_server_destroy.argtypes = [vp]
         L.geoinp.argtypes = [C.c_int, C.c_int, C.c_int, ip, fp, fp, fp, ip, ip, fp, ip, ip, ip]
         L.geoinp.restype = None
@@ -456,6 +457,13 @@ class Server:
         d = C.c_int()
         _check(lib().wm_server_step(self.h, 0, C.byref(d)))
         return d.value
+
+    def done(self, cap: int = 4096):
+        """Ids of completed requests whose results have not been read."""
+        ids = (C.c_int64 * cap)()
+        k = C.c_int()
+        _check(lib().wm_server_done(self.h, ids, cap, C.byref(k)))
+        return [ids[i] for i in range(k.value)]
 
     def status(self):
         q, r, c = C.c_int(), C.c_int(), C.c_int()

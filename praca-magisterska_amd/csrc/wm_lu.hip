@@ -789,6 +789,9 @@ int getrf_impl(wm_ctx *ctx, int n, typename CT<T>::V *A, int lda, int64_t stride
     WM_TRY(prepare_xbuf(ctx, (size_t)per_launch * X.mat_bytes, (uint64_t)nchunks * n, s));
     char *xbase = (char *)ctx->ws_lu.ptr;
     const double esz = 2.0 * sizeof(T);
+    const int kPanelClass = sizeof(T) == 8 ? K_PANEL : K_PANEL32;
+    const int kSwapClass = sizeof(T) == 8 ? K_SWAP : K_SWAP32;
+    const int kGemmClass = sizeof(T) == 8 ? K_GEMM : K_GEMM32;
 
     // One launch group: m matrices factored together on one stream.
     struct Grp {
@@ -807,9 +810,9 @@ int getrf_impl(wm_ctx *ctx, int n, typename CT<T>::V *A, int lda, int64_t stride
         pa.err = ctx->d_err; pa.xbuf = G.xbuf; pa.bidx = G.bidx; pa.tag0 = next_tag(ctx);
         const double mr = n - c;
         // per column j, (mr-j) scalings and (mr-j)(NB-j-1) complex multiply-adds
-        prof_begin(ctx, K_PANEL, G.st);
+        prof_begin(ctx, kPanelClass, G.st);
         hipLaunchKernelGGL((k_panel<T, TPR>), dim3(pa.P, G.m), dim3(256), 0, G.st, pa);
-        prof_end(ctx, K_PANEL, G.st, G.m * 8.0 * (mr * NB * NB / 2.0 - (double)NB * NB * NB / 6.0),
+        prof_end(ctx, kPanelClass, G.st, G.m * 8.0 * (mr * NB * NB / 2.0 - (double)NB * NB * NB / 6.0),
                  G.m * 2.0 * mr * NB * esz);
     };
     // interchanges of the panel at c left of c; solve for [tr0, tr1); interchanges only for [tr1, n)
@@ -822,18 +825,18 @@ int getrf_impl(wm_ctx *ctx, int n, typename CT<T>::V *A, int lda, int64_t stride
         sa.c_sw0 = tr1;
         const int nsw = do_swap && n > tr1 ? (n - tr1 + CB - 1) / CB : 0;
         if (sa.ntl + sa.ntr + nsw == 0) return;
-        prof_begin(ctx, K_SWAP, G.st);
+        prof_begin(ctx, kSwapClass, G.st);
         hipLaunchKernelGGL((k_swap_trsm<T, CB>), dim3(sa.ntl + sa.ntr + nsw, G.m), dim3(256), 0, G.st, G.Ab, lda,
                            strideA, sa, G.xbuf, G.bidx);
-        prof_end(ctx, K_SWAP, G.st, G.m * 4.0 * NB * NB * (tr1 - tr0), G.m * 4.0 * NB * n * esz);
+        prof_end(ctx, kSwapClass, G.st, G.m * 4.0 * NB * NB * (tr1 - tr0), G.m * 4.0 * NB * n * esz);
     };
     auto gemm = [&](const Grp &G, int r0, int c0, int mrows, int ncols, int kk, int K) {
         if (mrows <= 0 || ncols <= 0) return;
         const GemmArgs g{r0, c0, mrows, ncols, kk, K};
         // C read and written, L and U read once
-        prof_begin(ctx, K_GEMM, G.st);
+        prof_begin(ctx, kGemmClass, G.st);
         launch_gemm((int)sizeof(T), G.Ab, lda, strideA, g, G.m, G.bidx, G.st);
-        prof_end(ctx, K_GEMM, G.st, G.m * 8.0 * mrows * ncols * K,
+        prof_end(ctx, kGemmClass, G.st, G.m * 8.0 * mrows * ncols * K,
                  G.m * (2.0 * mrows * ncols + (double)(mrows + ncols) * K) * esz);
     };
 

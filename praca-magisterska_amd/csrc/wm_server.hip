@@ -7,8 +7,11 @@
 //   1. admits queued requests in arrival order while their footprint fits
 //      the HBM budget (a request that can never fit is failed at submit);
 //   2. packs pending frequencies of admitted requests that share the padded
-//      size of the oldest one into one batch, oldest request first, as many
-//      as the budget and the in-flight cap allow;
+//      size of the oldest one into one batch, oldest request first (and each
+//      request's frequencies highest first), as many as the budget and the
+//      in-flight cap allow; in mixed precision, frequencies that the refinement
+//      rates already seen on their structure predict not to refine are
+//      factored in FP64 directly;
 //   3. fills, factors and solves that batch (solve_items) and retires the
 //      requests whose last frequency finished.
 // A request that arrives during a step joins at the next one, next to the
@@ -41,7 +44,8 @@ struct Req {
     int nrhs = 0;
     std::vector<wm_c128> I, zin;
     std::vector<wm_freq_stats> st;
-    int next = 0;  // first frequency not yet scheduled
+    std::vector<int> order;  // frequencies highest first (refinement rates then route lower ones)
+    int next = 0;            // first entry of order not yet scheduled
     int done = 0;
     int status = WM_OK;
     double t_submit = 0, t_done = 0;
@@ -128,6 +132,9 @@ int wm_server_submit(wm_server *s, const wm_geom *g, int nfreq, const float *fre
     r->I.assign((size_t)nfreq * nrhs * g->npls, wm_c128{0, 0});
     r->zin.assign((size_t)nfreq * nrhs, wm_c128{0, 0});
     r->st.assign(nfreq, wm_freq_stats{});
+    r->order.resize(nfreq);
+    for (int f = 0; f < nfreq; ++f) r->order[f] = f;
+    std::stable_sort(r->order.begin(), r->order.end(), [&](int a, int b) { return r->freqs[a] > r->freqs[b]; });
     r->t_submit = now_s();
     std::lock_guard<std::mutex> lk(s->qmu);
     r->id = s->next_id++;
@@ -172,7 +179,7 @@ int wm_server_step(wm_server *s, int wait, int *done)
     for (auto &r : s->running) {
         if (r->g->ld != ld) continue;
         while (r->next < (int)r->freqs.size() && (int)items.size() < cap) {
-            const int f = r->next++;
+            const int f = r->order[r->next++];
             WorkItem w;
             w.g = r->g;
             w.freq = r->freqs[f];
@@ -202,7 +209,11 @@ int wm_server_step(wm_server *s, int wait, int *done)
     {
         std::lock_guard<std::recursive_mutex> lk(c->mu);
         (void)hipSetDevice(c->device);
+        // mixed precision: structures solved before carry refinement rates that
+        // route frequencies which cannot refine to FP64 factors at once
+        if (o.prec != WM_PREC_F64) route_items(o, sorted.data(), (int)sorted.size());
         rc = solve_items(c, o, sorted.data(), (int)sorted.size(), c->stream);
+        if (rc >= 0 && o.prec != WM_PREC_F64) record_rates(sorted.data(), (int)sorted.size());
     }
     if (rc < 0) return rc;
     // 3. retire
@@ -250,6 +261,16 @@ int wm_server_result(wm_server *s, int64_t id, wm_c128 *I, wm_c128 *zin, wm_freq
     if (st) std::copy(r->st.begin(), r->st.end(), st);
     if (latency_s) *latency_s = r->t_done - r->t_submit;
     return r->status;
+}
+
+int wm_server_done(wm_server *s, int64_t *ids, int cap, int *count)
+{
+    if (!s || !count || cap < 0 || (cap > 0 && !ids)) return WM_ERR_ARG;
+    std::lock_guard<std::mutex> lk(s->qmu);
+    int k = 0;
+    for (auto it = s->completed.begin(); it != s->completed.end() && k < cap; ++it) ids[k++] = it->first;
+    *count = k;
+    return WM_OK;
 }
 
 int wm_server_status(wm_server *s, int *queued, int *running, int *completed, size_t *hbm_used)

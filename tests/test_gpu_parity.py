@@ -248,3 +248,24 @@ def test_mixed_precision_fallback_band(wm, geometries, ctx, geo_dir):
     for a, b in zip(st, stb):  # routing only skips refinements that fail
         if b["fell_back"] == 2:
             assert a["fell_back"] == 1
+
+
+def test_serving_replay_matches_sweeps(wm, geometries, ctx, geo_dir):
+    """A Poisson stream on the server: every request's Zin matches the FP64
+    sweep of its structure within 1e-10, and each latency is positive."""
+    import wiremom.serving as S
+    reqs = S.poisson_requests(6, 40.0, seed=3, sizes=(512, 1024), fmin=1.0, fmax=50.0, max_freqs=4, max_rhs=2)
+    geoms, feeds = [], []
+    for i, r in enumerate(reqs):
+        p = os.path.join(geo_dir, f"srv{i}.geo")
+        geometries.planar_grid(p, r["n"], seed=r["seed"])
+        geoms.append(wm.Geometry.load(ctx, p))
+        feeds.append(np.random.RandomState(r["seed"]).choice(geoms[-1].npls, r["nrhs"], replace=False) + 1)
+    srv = wm.Server(ctx, "f32")
+    out = S.replay(srv, reqs, geoms, feeds)
+    srv.close()
+    assert all(t > 0 for t in out["latency"])
+    for r, g, fd, (zin, I, st) in zip(reqs, geoms, feeds, out["results"]):
+        z64, _, _ = wm.solve_sweep(g, r["freqs"], fd, "f64")
+        rel = np.abs(zin - z64) / np.abs(z64)
+        assert rel.max() <= 1e-10, (r["n"], rel.max())

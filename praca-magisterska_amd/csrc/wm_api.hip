@@ -268,7 +268,7 @@ int wm_shutdown(wm_ctx *c)
 
 // FP64 factorization kernels are "panel", "swap_trsm", "gemm"; complex64 ones carry "_f32".
 static const char *kClassNames[K_NCLASS] = {"fill", "panel", "swap_trsm", "gemm", "trsv", "resid",
-                                            "panel_f32", "swap_trsm_f32", "gemm_f32"};
+                                            "panel_f32", "swap_trsm_f32", "gemm_f32", "gemm_bf16x3"};
 
 int wm_prof_enable(wm_ctx *c, int on)
 {

@@ -14,6 +14,8 @@
 //                    ||dx||/||x|| <= tol or the correction stalls at the
 //                    FP64 floor; items that do not converge are refactored in
 //                    FP64 (fallback).
+//   WM_PREC_BF16X3 : as WM_PREC_F32, with the factorization's trailing
+//                    updates taken as split-BF16x3 products (panels in FP32).
 // Each item's arithmetic depends only on that item (per-item convergence
 // mask, fixed reduction splits), so an item's result is the same alone or
 // in any batch.
@@ -188,8 +190,8 @@ int solve_items(wm_ctx *c, const wm_solve_opts &o, WorkItem *items, int count, h
         nr = std::max(nr, items[b].nrhs);
     }
     const bool mixed = o.prec != WM_PREC_F64;
-    if (o.prec == WM_PREC_BF16X3) {
-        set_error("precision BF16X3 is not available in this build");
+    if (o.prec != WM_PREC_F64 && o.prec != WM_PREC_F32 && o.prec != WM_PREC_BF16X3) {
+        set_error("unknown precision %d", o.prec);
         return WM_ERR_ARG;
     }
     const size_t nn = (size_t)n * n;
@@ -264,11 +266,11 @@ int solve_items(wm_ctx *c, const wm_solve_opts &o, WorkItem *items, int count, h
         }
         const int n32 = (int)f32.size();
         if (n32 == count) {
-            WM_TRY(getrf_batched(c, WM_PREC_F32, n, Z32, n, sZ, ipiv, n, count, dinfo, s));
+            WM_TRY(getrf_batched(c, o.prec, n, Z32, n, sZ, ipiv, n, count, dinfo, s));
             WM_TRY(perm_batched(c, n, ipiv, n, perm, n, count, s));
         } else if (n32 > 0) {
             WM_TRY(upload_list(f32));
-            WM_TRY(getrf_batched(c, WM_PREC_F32, n, Z32, n, sZ, ipiv, n, n32, dinfo, s, dact));
+            WM_TRY(getrf_batched(c, o.prec, n, Z32, n, sZ, ipiv, n, n32, dinfo, s, dact));
             WM_TRY(perm_batched(c, n, ipiv, n, perm, n, n32, s, dact));
         }
         WM_TRY(check_device_error(c, s));

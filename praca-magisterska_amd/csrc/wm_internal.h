@@ -47,7 +47,7 @@ enum DevErr : int { DEV_OK = 0, DEV_SPIN_TIMEOUT = 1 };
 
 namespace wm {
 // Kernel classes timed by the optional profiler (events on the launch stream).
-enum KClass { K_FILL = 0, K_PANEL, K_SWAP, K_GEMM, K_TRSV, K_RESID, K_PANEL32, K_SWAP32, K_GEMM32, K_NCLASS };
+enum KClass { K_FILL = 0, K_PANEL, K_SWAP, K_GEMM, K_TRSV, K_RESID, K_PANEL32, K_SWAP32, K_GEMM32, K_GEMMB3, K_NCLASS };
 struct ProfRec {
     hipEvent_t a, b;
     double flops, bytes;

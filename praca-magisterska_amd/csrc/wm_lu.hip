@@ -689,19 +689,178 @@ __global__ __launch_bounds__(256, 2) void k_gemm_f32(float2 *Abase, int lda, int
             }
 }
 
+// complex64 with split-BF16x3 products: every FP32 operand x is split into
+// bf16 hi = rn(x) and lo = rn(x - hi), and each real product X*Y is taken as
+// Xhi*Yhi + Xhi*Ylo + Xlo*Yhi on v_mfma_f32_32x32x16_bf16 with FP32
+// accumulation (relative error about 2^-16 per product instead of 2^-24).
+// Workgroup tile 128 x 128, four waves of 64 x 64 (2 x 2 MFMA tiles), K in
+// chunks of 16.  Lane l holds srcA[l%32][8*(l/32)+i], srcB[8*(l/32)+i][l%32]
+// (i = 0..7) and D[8*(v/4)+4*(l/32)+v%4][l%32].  The chunk is staged in LDS as
+// four bf16 planes (re hi, re lo, im hi, im lo) with k contiguous per row of L
+// and per column of U, so each operand is one 16-byte LDS read.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+constexpr int GB_BM = 128, GB_BN = 128, GB_KC = 16, GB_KP = GB_KC + 8;  // padded k stride (bf16)
+
+__device__ inline void split_bf16(float x, __bf16 &hi, __bf16 &lo)
+{
+    hi = (__bf16)x;
+    lo = (__bf16)(x - (float)hi);
+}
+
+__device__ inline bf16x8 neg_bf16x8(bf16x8 v)
+{
+    u32x4 u = __builtin_bit_cast(u32x4, v);
+    u ^= 0x80008000u;
+    return __builtin_bit_cast(bf16x8, u);
+}
+
+__global__ __launch_bounds__(256, 2) void k_gemm_b3(float2 *Abase, int lda, int64_t strideA, GemmArgs g,
+                                                    const int *bidx, int TI, int TJ)
+{
+    const int t = xcd_tile(blockIdx.x, TI * TJ);
+    if (t >= TI * TJ) return;
+    const int b = bidx ? bidx[blockIdx.z] : (int)blockIdx.z;
+    float2 *M = Abase + (size_t)b * strideA;
+    const int mrows = g.mrows, ncols = g.ncols, K = g.K;
+    const int i0 = (t % TI) * GB_BM, j0 = (t / TI) * GB_BN;
+    const float2 *A = M + g.r0 + (size_t)g.kk * lda;  // L block
+    const float2 *B = M + g.kk + (size_t)g.c0 * lda;  // U block
+    float2 *C = M + g.r0 + (size_t)g.c0 * lda;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wi = (wave & 1) * 64, wj = (wave >> 1) * 64;
+
+    // planes: 0 re hi, 1 re lo, 2 im hi, 3 im lo
+    __shared__ __attribute__((aligned(16))) __bf16 sL[4][GB_BM][GB_KP];
+    __shared__ __attribute__((aligned(16))) __bf16 sU[4][GB_BN][GB_KP];
+    // chunk elements moved by this thread: 8 consecutive k of one row of L and of one column of U
+    const int lr = tid % GB_BM, lh = tid / GB_BM;  // L: row lr, k = 8*lh + e
+    const int uc = tid / 2, uh = tid % 2;           // U: column uc, k = 8*uh + e
+    const bool lok = i0 + lr < mrows, uok = j0 + uc < ncols;
+    float2 pa[8], pb[8];
+    auto fetch = [&](int kc) {
+        const float2 *pA = A + (i0 + lr) + (size_t)(kc + 8 * lh) * lda;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) pa[e] = lok ? pA[(size_t)e * lda] : float2{0.f, 0.f};
+        const float2 *pB = B + (kc + 8 * uh) + (size_t)(j0 + uc) * lda;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) pb[e] = uok ? pB[e] : float2{0.f, 0.f};
+    };
+    auto stash = [&]() {
+        bf16x8 v[4];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            __bf16 h, l;
+            split_bf16(pa[e].x, h, l);
+            v[0][e] = h;
+            v[1][e] = l;
+            split_bf16(pa[e].y, h, l);
+            v[2][e] = h;
+            v[3][e] = l;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) *(bf16x8 *)&sL[q][lr][8 * lh] = v[q];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            __bf16 h, l;
+            split_bf16(pb[e].x, h, l);
+            v[0][e] = h;
+            v[1][e] = l;
+            split_bf16(pb[e].y, h, l);
+            v[2][e] = h;
+            v[3][e] = l;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) *(bf16x8 *)&sU[q][uc][8 * uh] = v[q];
+    };
+    v16f accR[2][2], accI[2][2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) { accR[x][y][v] = 0.f; accI[x][y][v] = 0.f; }
+
+    fetch(0);
+    stash();
+    __syncthreads();
+    const int kofs = 8 * (lane / 32);
+    for (int kc = 0; kc < K; kc += GB_KC) {
+        if (kc + GB_KC < K) fetch(kc + GB_KC);
+        // U operands (MFMA rows = columns of C), L operands (MFMA columns = rows of C)
+        bf16x8 ur[2][2], ui[2][2], un[2][2], lre[2][2], lim[2][2];
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int cj = wj + q * 32 + lane % 32, ri = wi + q * 32 + lane % 32;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                ur[q][h] = *(const bf16x8 *)&sU[0 + h][cj][kofs];
+                ui[q][h] = *(const bf16x8 *)&sU[2 + h][cj][kofs];
+                un[q][h] = neg_bf16x8(ui[q][h]);
+                lre[q][h] = *(const bf16x8 *)&sL[0 + h][ri][kofs];
+                lim[q][h] = *(const bf16x8 *)&sL[2 + h][ri][kofs];
+            }
+        }
+        // split terms (hi,hi), (hi,lo), (lo,hi); real and imaginary accumulators alternate
+#pragma unroll
+        for (int term = 0; term < 3; ++term) {
+            const int hu = term == 2 ? 1 : 0, hl = term == 1 ? 1 : 0;
+#pragma unroll
+            for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+                for (int ti = 0; ti < 2; ++ti) {
+                    accR[tj][ti] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ur[tj][hu], lre[ti][hl], accR[tj][ti], 0, 0, 0);
+                    accI[tj][ti] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ur[tj][hu], lim[ti][hl], accI[tj][ti], 0, 0, 0);
+                }
+#pragma unroll
+            for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+                for (int ti = 0; ti < 2; ++ti) {
+                    accR[tj][ti] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(un[tj][hu], lim[ti][hl], accR[tj][ti], 0, 0, 0);
+                    accI[tj][ti] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ui[tj][hu], lre[ti][hl], accI[tj][ti], 0, 0, 0);
+                }
+        }
+        if (kc + GB_KC < K) {
+            __syncthreads();
+            stash();
+            __syncthreads();
+        }
+    }
+#pragma unroll
+    for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+        for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const int gi = i0 + wi + ti * 32 + lane % 32;
+                const int gj = j0 + wj + tj * 32 + 8 * (v / 4) + 4 * (lane / 32) + (v % 4);
+                if (gi < mrows && gj < ncols) {
+                    float2 c = C[gi + (size_t)gj * lda];
+                    c.x -= accR[tj][ti][v];
+                    c.y -= accI[tj][ti][v];
+                    C[gi + (size_t)gj * lda] = c;
+                }
+            }
+}
+
 // Launches C -= L U for nb_ matrices, C = A[r0.., c0..] (mrows x ncols),
-// L = A[r0.., kk..kk+K), U = A[kk..kk+K, c0..).
-void launch_gemm(int esz, void *A, int lda, int64_t strideA, const GemmArgs &g, int nb_, const int *bidx,
+// L = A[r0.., kk..kk+K), U = A[kk..kk+K, c0..).  gk: 8 complex128, 4 complex64,
+// 2 complex64 with split-BF16x3 products.
+void launch_gemm(int gk, void *A, int lda, int64_t strideA, const GemmArgs &g, int nb_, const int *bidx,
                  hipStream_t s)
 {
     if (g.mrows <= 0 || g.ncols <= 0 || g.K <= 0) return;
-    if (esz == 8) {
+    if (gk == 8) {
         const int TI = (g.mrows + GD_BM - 1) / GD_BM, TJ = (g.ncols + GD_BN - 1) / GD_BN;
         hipLaunchKernelGGL(k_gemm_f64, dim3(xcd_grid(TI * TJ), 1, nb_), dim3(256), 0, s, (double2 *)A, lda, strideA,
                            g, bidx, TI, TJ);
-    } else {
+    } else if (gk == 4) {
         const int TI = (g.mrows + GS_BM - 1) / GS_BM, TJ = (g.ncols + GS_BN - 1) / GS_BN;
         hipLaunchKernelGGL(k_gemm_f32, dim3(xcd_grid(TI * TJ), 1, nb_), dim3(256), 0, s, (float2 *)A, lda, strideA,
+                           g, bidx, TI, TJ);
+    } else {
+        const int TI = (g.mrows + GB_BM - 1) / GB_BM, TJ = (g.ncols + GB_BN - 1) / GB_BN;
+        hipLaunchKernelGGL(k_gemm_b3, dim3(xcd_grid(TI * TJ), 1, nb_), dim3(256), 0, s, (float2 *)A, lda, strideA,
                            g, bidx, TI, TJ);
     }
 }
@@ -767,7 +926,7 @@ int occupancy_panel(int *blocks_per_cu)
 
 template <typename T, int TPR>
 int getrf_impl(wm_ctx *ctx, int n, typename CT<T>::V *A, int lda, int64_t strideA, int *ipiv, int64_t strideP,
-               int batch, int *d_info, const int *d_bidx, hipStream_t s)
+               int batch, int *d_info, const int *d_bidx, hipStream_t s, int gk)
 {
     using V = typename CT<T>::V;
     constexpr int RW = 256 / TPR;
@@ -791,7 +950,7 @@ int getrf_impl(wm_ctx *ctx, int n, typename CT<T>::V *A, int lda, int64_t stride
     const double esz = 2.0 * sizeof(T);
     const int kPanelClass = sizeof(T) == 8 ? K_PANEL : K_PANEL32;
     const int kSwapClass = sizeof(T) == 8 ? K_SWAP : K_SWAP32;
-    const int kGemmClass = sizeof(T) == 8 ? K_GEMM : K_GEMM32;
+    const int kGemmClass = sizeof(T) == 8 ? K_GEMM : gk == 2 ? K_GEMMB3 : K_GEMM32;
 
     // One launch group: m matrices factored together on one stream.
     struct Grp {
@@ -835,7 +994,7 @@ int getrf_impl(wm_ctx *ctx, int n, typename CT<T>::V *A, int lda, int64_t stride
         const GemmArgs g{r0, c0, mrows, ncols, kk, K};
         // C read and written, L and U read once
         prof_begin(ctx, kGemmClass, G.st);
-        launch_gemm((int)sizeof(T), G.Ab, lda, strideA, g, G.m, G.bidx, G.st);
+        launch_gemm(gk, G.Ab, lda, strideA, g, G.m, G.bidx, G.st);
         prof_end(ctx, kGemmClass, G.st, G.m * 8.0 * mrows * ncols * K,
                  G.m * (2.0 * mrows * ncols + (double)(mrows + ncols) * K) * esz);
     };
@@ -882,7 +1041,7 @@ int getrf_impl(wm_ctx *ctx, int n, typename CT<T>::V *A, int lda, int64_t stride
 int getrf_width(wm_ctx *ctx, int prec, int n)
 {
     int bpc = 0;
-    const bool f64 = prec == WM_PREC_F64;
+    const bool f64 = prec == WM_PREC_F64;  // complex64 factors (FP32, BF16X3) share the FP32 panel
     if ((f64 ? occupancy_panel<double, 2>(&bpc) : occupancy_panel<float, 1>(&bpc)) != WM_OK || bpc <= 0) return 1;
     const int RW = f64 ? 128 : 256;
     const int pmax = (n + RW - 1) / RW;
@@ -900,9 +1059,11 @@ int getrf_batched(wm_ctx *ctx, int prec, int n, void *A, int lda, int64_t stride
     if (!d_bidx) hipLaunchKernelGGL(k_zero_int, dim3((batch + 255) / 256), dim3(256), 0, s, d_info, batch);
     else hipLaunchKernelGGL(k_zero_idx, dim3((batch + 255) / 256), dim3(256), 0, s, d_info, d_bidx, batch);
     if (prec == WM_PREC_F64)
-        return getrf_impl<double, 2>(ctx, n, (double2 *)A, lda, strideA, ipiv, strideP, batch, d_info, d_bidx, s);
+        return getrf_impl<double, 2>(ctx, n, (double2 *)A, lda, strideA, ipiv, strideP, batch, d_info, d_bidx, s, 8);
     if (prec == WM_PREC_F32)
-        return getrf_impl<float, 1>(ctx, n, (float2 *)A, lda, strideA, ipiv, strideP, batch, d_info, d_bidx, s);
+        return getrf_impl<float, 1>(ctx, n, (float2 *)A, lda, strideA, ipiv, strideP, batch, d_info, d_bidx, s, 4);
+    if (prec == WM_PREC_BF16X3)
+        return getrf_impl<float, 1>(ctx, n, (float2 *)A, lda, strideA, ipiv, strideP, batch, d_info, d_bidx, s, 2);
     set_error("getrf: precision %d not available", prec);
     return WM_ERR_ARG;
 }
@@ -919,7 +1080,7 @@ extern "C" int wmx_gemm_step(wm_ctx *ctx, int prec, int n, void *A, int k)
     const int m2 = n - k - NB;
     if (m2 <= 0 || n % NB || k % NB) return WM_ERR_ARG;
     const GemmArgs g{k + NB, k + NB, m2, m2, k, NB};
-    launch_gemm(prec == WM_PREC_F64 ? 8 : 4, A, n, 0, g, 1, nullptr, ctx->stream);
+    launch_gemm(prec == WM_PREC_F64 ? 8 : prec == WM_PREC_BF16X3 ? 2 : 4, A, n, 0, g, 1, nullptr, ctx->stream);
     WM_HIP_TRY(hipStreamSynchronize(ctx->stream));
     return WM_OK;
 }

@@ -88,7 +88,7 @@ int wm_server_create(wm_ctx *c, const wm_server_opts *o, wm_server **out)
     s->ctx = c;
     wm_server_opts_default(&s->o);
     if (o) s->o = *o;
-    if (s->o.prec != WM_PREC_F64 && s->o.prec != WM_PREC_F32) {
+    if (s->o.prec != WM_PREC_F64 && s->o.prec != WM_PREC_F32 && s->o.prec != WM_PREC_BF16X3) {
         delete s;
         set_error("wm_server_create: precision %d not available", o ? o->prec : -1);
         return WM_ERR_ARG;

@@ -119,6 +119,7 @@ def main():
     ap.add_argument("--batch", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-bf16x3", action="store_true")
     ap.add_argument("--requests", type=int, default=48)
     ap.add_argument("--rate", type=float, default=4.0)
     a = ap.parse_args()
@@ -164,7 +165,23 @@ def main():
         fp64 = len(freqs) / dt64
         rel = np.abs(zin[:, 0] - z64[:, 0]) / np.abs(z64[:, 0])
     else:
-        fp64, rel = value, np.zeros(len(freqs))
+        z64, fp64, rel = zin, value, np.zeros(len(freqs))
+
+    # the same sweep with split-BF16x3 updates (precision ladder: BF16x3, FP32, FP64 per frequency)
+    other = None
+    if not a.no_bf16x3 and a.prec != "bf16x3":
+        sweep("bf16x3")
+        ctx.sync()
+        t0 = time.perf_counter()
+        zb, _, stb = sweep("bf16x3")
+        dtb = time.perf_counter() - t0
+        relb = np.abs(zb[:, 0] - z64[:, 0]) / np.abs(z64[:, 0])
+        names = {0: "f64", 1: "f32", 2: "bf16x3"}
+        other = {"prec": "bf16x3", "solves_per_s": len(freqs) / dtb, "speedup_vs_fp64": len(freqs) / dtb / fp64,
+                 "zin_rel_err_vs_fp64_max": float(relb.max()),
+                 "factor_prec_counts": {v: int(sum(s["prec"] == k for s in stb)) for k, v in names.items()},
+                 "fell_back": int(sum(s["fell_back"] == 1 for s in stb)),
+                 "ir_iters_mean": float(np.mean([s["iters"] for s in stb]))}
 
     # roofline of the dominant kernel class (largest event time)
     peak_mfma = PEAK_TFLOPS.get(a.prec, 157.3)
@@ -215,9 +232,12 @@ def main():
         "ir_iters_mean": iters,
         "fell_back": int(sum(s["fell_back"] == 1 for s in st)),
         "routed_f64": int(sum(s["fell_back"] == 2 for s in st)),
+        "rel_err_estimate_max": float(max(s["dx_rel"] * s["rate"] / (1 - s["rate"]) if 0 <= s["rate"] < 1 else 0.0
+                                          for s in st)),
         "solve_roofline_frac": solve_frac,
         "kernel_ms": {k: round(v[0] / a.steps, 3) for k, v in prof.items()},
         "roofline": roof,
+        "bf16x3_sweep": other,
         "serving": serving,
         "cpu_baseline": cpu,
         "reference_best_published_solves_per_s": REF_BEST_PUBLISHED,

@@ -137,8 +137,10 @@ typedef struct {
     int info;              /* LU info */
     int fell_back;         /* FP64 factors produced the result: 1 after refinement failed,
                               2 routed there directly (predicted not to refine) */
-    double backward_err;   /* max_r ||b - Z x||_inf / (||Z||_inf ||x||_inf + ||b||_inf) */
-    double dx_rel;         /* last ||dx||_inf / ||x||_inf */
+    int prec;              /* WM_PREC_* of the factors that produced the result */
+    double rate;           /* last refinement contraction ||dx_k|| / ||dx_k-1|| (-1: none) */
+    double dx_rel;         /* last ||dx||_inf / ||x||_inf: the refined result's relative
+                              distance from the FP64 solution is about dx_rel * rate / (1 - rate) */
 } wm_freq_stats;
 
 /* Sweep over nfreq frequencies (MHz, float32 like the reference's freq.txt)
@@ -149,8 +151,9 @@ typedef struct {
  *   st  : wm_freq_stats [nfreq]
  * In mixed precision the frequencies are solved highest first; the refinement
  * contraction rates of the first batch (one factorization launch wide) predict,
- * scaled by (f_obs/f)^2, which lower frequencies cannot reach tol within
- * max_iter, and those are factored in FP64 directly (fell_back = 2).
+ * scaled by (f_obs/f)^2, the rate at each lower frequency, and each is factored
+ * in the cheapest precision predicted to reach tol within max_iter: the
+ * requested one, else FP32 (for BF16X3), else FP64 directly (fell_back = 2).
  * Returns the first nonzero status. */
 int wm_solve_sweep(wm_ctx *ctx, const wm_geom *g, const wm_solve_opts *opts, int nfreq,
                    const float *freq_mhz, int nrhs, const int *gaps, float ampl, float phas,

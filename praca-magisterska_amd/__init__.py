@@ -49,9 +49,13 @@ class SolveOpts(C.Structure):
                 ("hbm_fraction", C.c_double), ("fallback_f64", C.c_int)]
 
 
+def _stats(s) -> dict:
+    return dict(iters=s.iters, info=s.info, fell_back=s.fell_back, prec=s.prec, rate=s.rate, dx_rel=s.dx_rel)
+
+
 class FreqStats(C.Structure):
-    _fields_ = [("iters", C.c_int), ("info", C.c_int), ("fell_back", C.c_int), ("backward_err", C.c_double),
-                ("dx_rel", C.c_double)]
+    _fields_ = [("iters", C.c_int), ("info", C.c_int), ("fell_back", C.c_int), ("prec", C.c_int),
+                ("rate", C.c_double), ("dx_rel", C.c_double)]
 
 
 class ServerOpts(C.Structure):
@@ -357,7 +361,7 @@ def solve_sweep(geom: Geometry, freqs_mhz, gaps, prec="f64", ampl: float = 1.0, 
     rc = lib().wm_solve_sweep(geom.ctx.h, geom.h, C.byref(o), nf, _ptr(f, C.c_float), nr, _ptr(g, C.c_int),
                               float(ampl), float(phas), I.ctypes.data, zin.ctypes.data, st)
     _check(rc)
-    stats = [dict(iters=s.iters, info=s.info, fell_back=s.fell_back, dx_rel=s.dx_rel) for s in st]
+    stats = [_stats(s) for s in st]
     return zin, I, stats
 
 
@@ -479,7 +483,7 @@ class Server:
         lat = C.c_double()
         _check(lib().wm_server_result(self.h, rid, I.ctypes.data, zin.ctypes.data, st, C.byref(lat)))
         del self.meta[rid]
-        return zin, I, [dict(iters=s.iters, info=s.info, fell_back=s.fell_back, dx_rel=s.dx_rel) for s in st], lat.value
+        return zin, I, [_stats(s) for s in st], lat.value
 
     def close(self):
         if self.h:

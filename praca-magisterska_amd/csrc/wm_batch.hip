@@ -153,12 +153,13 @@ void record_rates(const WorkItem *items, int count)
 {
     for (int b = 0; b < count; ++b) {
         const WorkItem &w = items[b];
-        if (w.direct64 || w.rate < 0) continue;
+        if (w.fprec == WM_PREC_F64 || w.rate < 0) continue;
+        const double r32 = w.fprec == WM_PREC_BF16X3 ? w.rate / kBf16x3RateFactor : w.rate;
         auto &obs = w.g->rate_obs;
         bool found = false;
         for (auto &ob : obs)
-            if (ob.first == w.freq) { ob.second = w.rate; found = true; }
-        if (!found) obs.emplace_back(w.freq, w.rate);
+            if (ob.first == w.freq) { ob.second = r32; found = true; }
+        if (!found) obs.emplace_back(w.freq, r32);
     }
 }
 
@@ -168,7 +169,12 @@ void route_items(const wm_solve_opts &o, WorkItem *items, int count)
     const double rmax = o.max_iter > 1 ? std::pow(o.tol, 1.0 / (o.max_iter - 1)) : 0.0;
     for (int b = 0; b < count; ++b) {
         const double r = predict_rate(items[b].g, items[b].freq);
-        items[b].direct64 = r > rmax;
+        int p = o.prec;
+        if (r >= 0) {
+            if (p == WM_PREC_BF16X3 && r * kBf16x3RateFactor > rmax) p = WM_PREC_F32;
+            if (r > rmax) p = WM_PREC_F64;
+        }
+        items[b].fprec = p;
     }
 }
 
@@ -258,21 +264,28 @@ int solve_items(wm_ctx *c, const wm_solve_opts &o, WorkItem *items, int count, h
         WM_TRY(check_device_error(c, s));
         WM_TRY(copy_d2h(hinfo.data(), dinfo, sizeof(int) * count, s));
     } else {
-        // FP32 factors for the items routed to refinement
-        std::vector<int> f32;
+        // complex64 factors for the items routed to refinement, one batched
+        // factorization per precision (split-BF16x3 updates, FP32 updates)
+        std::vector<int> lp[2], f32;
         for (int b = 0; b < count; ++b) {
             items[b].rate = -1.0;
-            if (!items[b].direct64) f32.push_back(b);
+            if (items[b].fprec < 0) items[b].fprec = o.prec;
+            if (items[b].fprec == WM_PREC_BF16X3) lp[0].push_back(b);
+            else if (items[b].fprec == WM_PREC_F32) lp[1].push_back(b);
         }
-        const int n32 = (int)f32.size();
-        if (n32 == count) {
-            WM_TRY(getrf_batched(c, o.prec, n, Z32, n, sZ, ipiv, n, count, dinfo, s));
-            WM_TRY(perm_batched(c, n, ipiv, n, perm, n, count, s));
-        } else if (n32 > 0) {
-            WM_TRY(upload_list(f32));
-            WM_TRY(getrf_batched(c, o.prec, n, Z32, n, sZ, ipiv, n, n32, dinfo, s, dact));
-            WM_TRY(perm_batched(c, n, ipiv, n, perm, n, n32, s, dact));
+        for (int q = 0; q < 2; ++q) {
+            const int np = (int)lp[q].size(), pq = q == 0 ? WM_PREC_BF16X3 : WM_PREC_F32;
+            if (np == count) {
+                WM_TRY(getrf_batched(c, pq, n, Z32, n, sZ, ipiv, n, count, dinfo, s));
+                WM_TRY(perm_batched(c, n, ipiv, n, perm, n, count, s));
+            } else if (np > 0) {
+                WM_TRY(upload_list(lp[q]));
+                WM_TRY(getrf_batched(c, pq, n, Z32, n, sZ, ipiv, n, np, dinfo, s, dact));
+                WM_TRY(perm_batched(c, n, ipiv, n, perm, n, np, s, dact));
+            }
+            f32.insert(f32.end(), lp[q].begin(), lp[q].end());
         }
+        std::sort(f32.begin(), f32.end());
         WM_TRY(check_device_error(c, s));
         WM_TRY(copy_d2h(hinfo.data(), dinfo, sizeof(int) * count, s));
         hipLaunchKernelGGL(k_zero2, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, s, X, nv);
@@ -280,7 +293,7 @@ int solve_items(wm_ctx *c, const wm_solve_opts &o, WorkItem *items, int count, h
         std::vector<int> active;
         for (int b : f32) {
             if (hinfo[b] == 0) active.push_back(b);
-            else items[b].rate = HUGE_VAL;  // singular in FP32: straight to the FP64 fallback
+            else items[b].rate = HUGE_VAL;  // singular complex64 factor: straight to the FP64 fallback
         }
         std::vector<double> prev((size_t)count * nr, 1e300);
         std::vector<unsigned long long> hm((size_t)2 * count * nr);
@@ -345,7 +358,7 @@ int solve_items(wm_ctx *c, const wm_solve_opts &o, WorkItem *items, int count, h
             WM_TRY(getrs_perm_batched(c, WM_PREC_F64, n, nr, Z, n, sZ, perm, n, X, n, sV, nb, s, dact));
             WM_TRY(check_device_error(c, s));
             WM_TRY(copy_d2h(hinfo.data(), dinfo, sizeof(int) * count, s));
-            for (int b : bad) fell[b] = items[b].direct64 ? 2 : 1;
+            for (int b : bad) fell[b] = items[b].fprec == WM_PREC_F64 ? 2 : 1;
         }
     }
 
@@ -373,7 +386,8 @@ int solve_items(wm_ctx *c, const wm_solve_opts &o, WorkItem *items, int count, h
             w.st->iters = iters[b];
             w.st->info = hinfo[b];
             w.st->fell_back = fell[b];
-            w.st->backward_err = -1.0;
+            w.st->prec = (!mixed || fell[b]) ? WM_PREC_F64 : w.fprec;
+            w.st->rate = mixed ? w.rate : -1.0;
             w.st->dx_rel = dxr[b];
         }
     }

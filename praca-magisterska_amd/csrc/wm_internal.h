@@ -152,20 +152,26 @@ struct WorkItem {
     wm_c128 *I = nullptr;      // [nrhs][npls]
     wm_c128 *zin = nullptr;    // [nrhs]
     wm_freq_stats *st = nullptr;
-    bool direct64 = false;     // mixed precision: factor in FP64 at once (predicted not to refine)
-    double rate = -1.0;        // out: last refinement contraction rate (inf: FP32 factor singular)
+    int fprec = -1;            // mixed precision: factor precision chosen by routing (-1: the batch's)
+    double rate = -1.0;        // out: last refinement contraction rate (inf: complex64 factor singular)
 };
 
-// Refinement contraction rate predicted for frequency f on geometry g from
-// its recorded observations, or -1 if there are none.  Below the structure's
-// resonances the EFIE condition number grows as 1/(k dl)^2, and the
-// contraction rate of FP32-factor refinement with it: the nearest observation
-// at or above f is scaled by (f_obs / f)^2 (the nearest below, if none).
+// Refinement contraction rates scale with the factorization's effective unit
+// roundoff; split-BF16x3 factors (about 16-bit products) contract this many
+// times slower than FP32 factors on the same matrix.
+constexpr double kBf16x3RateFactor = 32.0;
+
+// FP32-factor refinement contraction rate predicted for frequency f on
+// geometry g from its recorded observations, or -1 if there are none.  Below
+// the structure's resonances the EFIE condition number grows as 1/(k dl)^2,
+// and the contraction rate with it: the nearest observation at or above f is
+// scaled by (f_obs / f)^2 (the nearest below, if none).
 double predict_rate(const wm_geom *g, float f);
-// Records the outcome of the mixed-precision items of a batch on their geometries.
+// Records the outcome of the refined items of a batch on their geometries
+// (as FP32-equivalent rates).
 void record_rates(const WorkItem *items, int count);
-// Routes the items of a batch: direct64 when the predicted rate cannot reach
-// tol within max_iter iterations.
+// Routes the items of a batch to the cheapest factor precision (o.prec, then
+// FP32, then FP64) whose predicted rate reaches tol within max_iter.
 void route_items(const wm_solve_opts &o, WorkItem *items, int count);
 // Matrices per launch of the batched factorization (panel cluster residency).
 int getrf_width(wm_ctx *ctx, int prec, int n);

@@ -284,6 +284,12 @@ def test_mixed_precision_fallback_band(wm, geometries, ctx, geo_dir):
     for a, b in zip(st, stb):  # routing only skips refinements that fail
         if b["fell_back"] == 2:
             assert a["fell_back"] == 1
+    # precision ladder: split-BF16x3 factors where they refine, then FP32, then FP64
+    zb, _, stl = wm.solve_sweep(g, f, [gap], "bf16x3", batch=4)
+    rell = np.abs(zb[:, 0] - z64[:, 0]) / np.abs(z64[:, 0])
+    print([(float(fr), s["iters"], s["prec"], s["fell_back"], float(r)) for fr, s, r in zip(f, stl, rell)])
+    assert (rell <= 1e-10).all(), rell
+    assert stl[-1]["prec"] == wm.PREC_BF16X3 and stl[0]["prec"] == wm.PREC_F64
 
 
 def test_serving_replay_matches_sweeps(wm, geometries, ctx, geo_dir):

@@ -196,10 +196,13 @@ def main():
         roof = {"bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak}
     roof.update({"kernel": dom, "launches": nl, "avg_launch_us": 1e3 * ms / max(nl, 1), "traffic": None})
 
-    # whole-solve roofline fraction: t_ideal / t_measured per freq-solve
+    # whole-solve roofline fraction: t_ideal / t_measured per freq-solve, each
+    # frequency's LU priced at the peak of the precision its factors used
     N = g.ld
     iters = float(np.mean([s["iters"] for s in st]))
-    t_ideal = (8 / 3 * N ** 3 / (peak_mfma * 1e12) + 220 * N ** 2 / (PEAK_FP64_VALU * 1e12)
+    peak_of = {0: PEAK_TFLOPS["f64"], 1: PEAK_TFLOPS["f32"], 2: PEAK_TFLOPS["bf16x3"]}
+    t_lu = float(np.mean([8 / 3 * N ** 3 / (peak_of[s["prec"]] * 1e12) for s in st]))
+    t_ideal = (t_lu + 220 * N ** 2 / (PEAK_FP64_VALU * 1e12)
                + (16 * N ** 2 + iters * 24 * N ** 2) / (PEAK_HBM_GBS * 1e9))
     solve_frac = t_ideal / (dt / (a.steps * len(freqs)))
 

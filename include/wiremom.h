@@ -149,11 +149,13 @@ typedef struct {
  *   I   : complex128 [nfreq][nrhs][npls]   current coefficients
  *   zin : complex128 [nfreq][nrhs]          V/I at each feed
  *   st  : wm_freq_stats [nfreq]
- * In mixed precision the frequencies are solved highest first; the refinement
- * contraction rates of the first batch (one factorization launch wide) predict,
- * scaled by (f_obs/f)^2, the rate at each lower frequency, and each is factored
- * in the cheapest precision predicted to reach tol within max_iter: the
- * requested one, else FP32 (for BF16X3), else FP64 directly (fell_back = 2).
+ * In mixed precision the frequencies are solved highest first, in batches one
+ * factorization launch wide; the refinement contraction rates already seen,
+ * scaled by (f_obs/f)^3, predict the rate at each lower frequency, and each is
+ * factored in the cheapest precision predicted to reach tol within 3/4 of
+ * max_iter: the requested one, else FP32 (for BF16X3), else FP64 directly
+ * (fell_back = 2).  A split-BF16x3 attempt that does not converge is retried
+ * with FP32 factors, an FP32 one with FP64 factors (fell_back = 1).
  * Returns the first nonzero status. */
 int wm_solve_sweep(wm_ctx *ctx, const wm_geom *g, const wm_solve_opts *opts, int nfreq,
                    const float *freq_mhz, int nrhs, const int *gaps, float ampl, float phas,

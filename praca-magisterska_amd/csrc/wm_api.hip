@@ -587,9 +587,9 @@ int wm_solve_sweep(wm_ctx *c, const wm_geom *g, const wm_solve_opts *opts, int n
     int f0 = 0;
     while (f0 < nfreq) {
         int cnt = std::min(batch, nfreq - f0);
-        // the first batch of a mixed sweep is one factorization launch wide, so
-        // that the rest is routed with its rates
-        if (mixed && f0 == 0) cnt = std::min(cnt, getrf_width(c, WM_PREC_F32, g->ld));
+        // mixed sweeps go in batches one factorization launch wide, so that each
+        // batch is routed with the rates of the frequencies just above it
+        if (mixed) cnt = std::min(cnt, getrf_width(c, WM_PREC_F32, g->ld));
         if (mixed && f0 > 0) route_items(o, items.data() + f0, cnt);
         const int rc = solve_items(c, o, items.data() + f0, cnt, c->stream);
         if (rc < 0) return rc;

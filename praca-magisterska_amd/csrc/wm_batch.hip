@@ -85,6 +85,23 @@ __global__ void k_copy_items(double2 *Dst, const double2 *Src, int64_t stride, i
     Dst[o] = Src[o];
 }
 
+// Dst[bidx[y]] = 0 over len entries (items at a fixed stride).
+__global__ void k_zero_items(double2 *Dst, int64_t stride, int64_t len, const int *bidx)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < len) Dst[(int64_t)bidx[blockIdx.y] * stride + i] = make_double2(0.0, 0.0);
+}
+
+// Complex64 copy of complex128 matrices bidx[y] (nn entries each).
+__global__ void k_to_c64(float2 *Z32, const double2 *Z, int64_t nn, int64_t stride, const int *bidx)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nn) return;
+    const int64_t o = (int64_t)bidx[blockIdx.y] * stride + i;
+    const double2 z = Z[o];
+    Z32[o] = make_float2((float)z.x, (float)z.y);
+}
+
 __global__ void k_zero_u64(unsigned long long *p, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -133,45 +150,59 @@ int auto_batch(wm_ctx *ctx, int ld, int nrhs, int prec, double frac)
     return std::min(b, 64);
 }
 
-double predict_rate(const wm_geom *g, float f)
+double predict_rate(const wm_geom *g, float f, int prec)
 {
-    const std::pair<float, double> *above = nullptr, *below = nullptr;
-    for (const auto &ob : g->rate_obs) {
-        if (ob.first >= f) {
-            if (!above || ob.first < above->first) above = &ob;
-        } else if (!below || ob.first > below->first) {
-            below = &ob;
+    for (int same = 1; same >= 0; --same) {
+        const wm_geom::RateObs *above = nullptr, *below = nullptr;
+        for (const auto &ob : g->rate_obs) {
+            if ((ob.prec == prec) != (same == 1)) continue;
+            if (ob.freq >= f) {
+                if (!above || ob.freq < above->freq) above = &ob;
+            } else if (!below || ob.freq > below->freq) {
+                below = &ob;
+            }
         }
+        const auto *ob = above ? above : below;
+        if (!ob) continue;
+        const double sc = (double)ob->freq / (double)f;
+        double r = ob->rate * sc * sc * sc;
+        if (!same) r = prec == WM_PREC_BF16X3 ? r * kBf16x3RateFactor : r / kBf16x3RateFactor;
+        return r;
     }
-    const auto *ob = above ? above : below;
-    if (!ob) return -1.0;
-    const double sc = (double)ob->first / (double)f;
-    return ob->second * sc * sc;
+    return -1.0;
 }
 
 void record_rates(const WorkItem *items, int count)
 {
+    auto put = [](const wm_geom *g, float f, int prec, double rate) {
+        for (auto &ob : g->rate_obs)
+            if (ob.freq == f && ob.prec == prec) {
+                ob.rate = rate;
+                return;
+            }
+        g->rate_obs.push_back(wm_geom::RateObs{f, prec, rate});
+    };
     for (int b = 0; b < count; ++b) {
         const WorkItem &w = items[b];
-        if (w.fprec == WM_PREC_F64 || w.rate < 0) continue;
-        const double r32 = w.fprec == WM_PREC_BF16X3 ? w.rate / kBf16x3RateFactor : w.rate;
-        auto &obs = w.g->rate_obs;
-        bool found = false;
-        for (auto &ob : obs)
-            if (ob.first == w.freq) { ob.second = r32; found = true; }
-        if (!found) obs.emplace_back(w.freq, r32);
+        if (w.rate_b3 >= 0) put(w.g, w.freq, WM_PREC_BF16X3, w.rate_b3);
+        if (w.rate >= 0 && (w.fprec == WM_PREC_BF16X3 || w.fprec == WM_PREC_F32)) put(w.g, w.freq, w.fprec, w.rate);
     }
 }
 
 void route_items(const wm_solve_opts &o, WorkItem *items, int count)
 {
-    // rate that reaches tol from a first correction of relative size 1 within max_iter
-    const double rmax = o.max_iter > 1 ? std::pow(o.tol, 1.0 / (o.max_iter - 1)) : 0.0;
+    // rate that reaches tol from a first correction of relative size 1 within
+    // three quarters of max_iter (beyond that a complex64 factorization plus the
+    // iterations costs about as much as FP64 factors)
+    const double rmax = o.max_iter > 1 ? std::pow(o.tol, 1.0 / (0.75 * o.max_iter)) : 0.0;
     for (int b = 0; b < count; ++b) {
-        const double r = predict_rate(items[b].g, items[b].freq);
         int p = o.prec;
-        if (r >= 0) {
-            if (p == WM_PREC_BF16X3 && r * kBf16x3RateFactor > rmax) p = WM_PREC_F32;
+        if (p == WM_PREC_BF16X3) {
+            const double r = predict_rate(items[b].g, items[b].freq, WM_PREC_BF16X3);
+            if (r > rmax) p = WM_PREC_F32;
+        }
+        if (p == WM_PREC_F32) {
+            const double r = predict_rate(items[b].g, items[b].freq, WM_PREC_F32);
             if (r > rmax) p = WM_PREC_F64;
         }
         items[b].fprec = p;
@@ -264,84 +295,117 @@ int solve_items(wm_ctx *c, const wm_solve_opts &o, WorkItem *items, int count, h
         WM_TRY(check_device_error(c, s));
         WM_TRY(copy_d2h(hinfo.data(), dinfo, sizeof(int) * count, s));
     } else {
-        // complex64 factors for the items routed to refinement, one batched
-        // factorization per precision (split-BF16x3 updates, FP32 updates)
-        std::vector<int> lp[2], f32;
+        // Complex64 factors and FP64 refinement, level by level: split-BF16x3
+        // updates, then FP32 updates (items routed there, and BF16x3 attempts
+        // that did not converge, refactored from the complex128 matrix), then
+        // the FP64 fallback.
+        std::vector<int> lvl[2], conv(count, 0);
         for (int b = 0; b < count; ++b) {
-            items[b].rate = -1.0;
+            items[b].rate = items[b].rate_b3 = -1.0;
             if (items[b].fprec < 0) items[b].fprec = o.prec;
-            if (items[b].fprec == WM_PREC_BF16X3) lp[0].push_back(b);
-            else if (items[b].fprec == WM_PREC_F32) lp[1].push_back(b);
+            if (items[b].fprec == WM_PREC_BF16X3) lvl[0].push_back(b);
+            else if (items[b].fprec == WM_PREC_F32) lvl[1].push_back(b);
         }
+        std::vector<double> prev((size_t)count * nr), first(count);
+        std::vector<unsigned long long> hm((size_t)2 * count * nr);
+        std::vector<int> retry;  // level-0 items refactored at level 1
         for (int q = 0; q < 2; ++q) {
-            const int np = (int)lp[q].size(), pq = q == 0 ? WM_PREC_BF16X3 : WM_PREC_F32;
+            std::vector<int> &L = lvl[q];
+            if (L.empty()) continue;
+            std::sort(L.begin(), L.end());
+            const int pq = q == 0 ? WM_PREC_BF16X3 : WM_PREC_F32, np = (int)L.size();
+            if (!retry.empty()) {
+                // the failed attempts' factors overwrote their complex64 copies
+                WM_TRY(upload_list(retry));
+                hipLaunchKernelGGL(k_to_c64, dim3((unsigned)((nn + 255) / 256), (unsigned)retry.size()), dim3(256), 0, s,
+                                   Z32, Z, (int64_t)nn, sZ, dact);
+            }
+            WM_TRY(upload_list(L));
             if (np == count) {
                 WM_TRY(getrf_batched(c, pq, n, Z32, n, sZ, ipiv, n, count, dinfo, s));
                 WM_TRY(perm_batched(c, n, ipiv, n, perm, n, count, s));
-            } else if (np > 0) {
-                WM_TRY(upload_list(lp[q]));
+            } else {
                 WM_TRY(getrf_batched(c, pq, n, Z32, n, sZ, ipiv, n, np, dinfo, s, dact));
                 WM_TRY(perm_batched(c, n, ipiv, n, perm, n, np, s, dact));
             }
-            f32.insert(f32.end(), lp[q].begin(), lp[q].end());
-        }
-        std::sort(f32.begin(), f32.end());
-        WM_TRY(check_device_error(c, s));
-        WM_TRY(copy_d2h(hinfo.data(), dinfo, sizeof(int) * count, s));
-        hipLaunchKernelGGL(k_zero2, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, s, X, nv);
-        WM_HIP_TRY(hipMemcpyAsync(R, B, szV, hipMemcpyDeviceToDevice, s));
-        std::vector<int> active;
-        for (int b : f32) {
-            if (hinfo[b] == 0) active.push_back(b);
-            else items[b].rate = HUGE_VAL;  // singular complex64 factor: straight to the FP64 fallback
-        }
-        std::vector<double> prev((size_t)count * nr, 1e300);
-        std::vector<unsigned long long> hm((size_t)2 * count * nr);
-        std::vector<int> conv(count, 0);
-        for (int it = 0; it < o.max_iter && !active.empty(); ++it) {
-            const int na = (int)active.size();
-            WM_TRY(upload_list(active));
-            // D = R for the active items, then D = LU32^{-1} D (FP64 arithmetic on widened factors)
-            hipLaunchKernelGGL(k_copy_items, dim3((unsigned)((sV + 255) / 256), na), dim3(256), 0, s, D, R, sV, sV, dact);
-            WM_TRY(getrs_perm_batched(c, WM_PREC_F32, n, nr, Z32, n, sZ, perm, n, D, n, sV, na, s, dact));
-            hipLaunchKernelGGL(k_zero_u64, dim3((2 * count * nr + 255) / 256), dim3(256), 0, s, mx, 2 * count * nr);
-            hipLaunchKernelGGL(k_update, dim3((n + 255) / 256, na * nr), dim3(256), 0, s, X, D, n, n, sV, nr, dact, mx);
-            WM_TRY(residual_batched(c, n, nr, Z, n, sZ, X, B, n, sV, R, na, s, dact));
+            // X = 0, R = B for the items of this level
+            hipLaunchKernelGGL(k_zero_items, dim3((unsigned)((sV + 255) / 256), np), dim3(256), 0, s, X, sV, sV, dact);
+            hipLaunchKernelGGL(k_copy_items, dim3((unsigned)((sV + 255) / 256), np), dim3(256), 0, s, R, B, sV, sV, dact);
             WM_TRY(check_device_error(c, s));
-            WM_TRY(copy_d2h(hm.data(), mx, hm.size() * sizeof(unsigned long long), s));
-            std::vector<int> still;
-            for (int b : active) {
-                double worst = 0.0, rate = 0.0;
-                bool stalled = true;
-                for (int r = 0; r < items[b].nrhs; ++r) {
-                    double dm, xm;
-                    std::memcpy(&dm, &hm[2 * ((size_t)b * nr + r)], 8);
-                    std::memcpy(&xm, &hm[2 * ((size_t)b * nr + r) + 1], 8);
-                    const double rel = xm > 0 ? dm / xm : 0.0;
-                    worst = std::max(worst, rel);
-                    double &p = prev[(size_t)b * nr + r];
-                    rate = std::max(rate, p < 1e299 && p > 0 ? rel / p : 0.0);
-                    if (!(rel > 0.5 * p && p < 1e-9)) stalled = false;
-                    p = rel;
+            WM_TRY(copy_d2h(hinfo.data(), dinfo, sizeof(int) * count, s));
+            std::vector<int> active, failed;
+            for (int b : L) {
+                for (int r = 0; r < nr; ++r) prev[(size_t)b * nr + r] = 1e300;
+                items[b].fprec = pq;
+                if (hinfo[b] == 0) active.push_back(b);
+                else {
+                    items[b].rate = HUGE_VAL;  // singular complex64 factor
+                    failed.push_back(b);
                 }
-                dxr[b] = worst;
-                iters[b] = it + 1;
-                if (it >= 1 && !stalled) items[b].rate = rate;
-                else if (it == 0 && worst <= o.tol) items[b].rate = worst;  // converged at once: rate at most this
-                if (worst <= o.tol || stalled) {
-                    conv[b] = 1;
-                    continue;
-                }
-                // contraction too slow to reach tol within max_iter: stop now (FP64 fallback)
-                if (it >= 1) {
-                    const double need = rate > 0 && rate < 1 ? std::log(o.tol / worst) / std::log(rate) : 1e9;
-                    if (it + 1 + need > o.max_iter) continue;
-                }
-                still.push_back(b);
             }
-            active.swap(still);
+            for (int it = 0; it < o.max_iter && !active.empty(); ++it) {
+                const int na = (int)active.size();
+                WM_TRY(upload_list(active));
+                // D = R for the active items, then D = LU32^{-1} D (FP64 arithmetic on widened factors)
+                hipLaunchKernelGGL(k_copy_items, dim3((unsigned)((sV + 255) / 256), na), dim3(256), 0, s, D, R, sV, sV,
+                                   dact);
+                WM_TRY(getrs_perm_batched(c, WM_PREC_F32, n, nr, Z32, n, sZ, perm, n, D, n, sV, na, s, dact));
+                hipLaunchKernelGGL(k_zero_u64, dim3((2 * count * nr + 255) / 256), dim3(256), 0, s, mx, 2 * count * nr);
+                hipLaunchKernelGGL(k_update, dim3((n + 255) / 256, na * nr), dim3(256), 0, s, X, D, n, n, sV, nr, dact, mx);
+                WM_TRY(residual_batched(c, n, nr, Z, n, sZ, X, B, n, sV, R, na, s, dact));
+                WM_TRY(check_device_error(c, s));
+                WM_TRY(copy_d2h(hm.data(), mx, hm.size() * sizeof(unsigned long long), s));
+                std::vector<int> still;
+                for (int b : active) {
+                    double worst = 0.0, rate = 0.0;
+                    bool stalled = true;
+                    for (int r = 0; r < items[b].nrhs; ++r) {
+                        double dm, xm;
+                        std::memcpy(&dm, &hm[2 * ((size_t)b * nr + r)], 8);
+                        std::memcpy(&xm, &hm[2 * ((size_t)b * nr + r) + 1], 8);
+                        const double rel = xm > 0 ? dm / xm : 0.0;
+                        worst = std::max(worst, rel);
+                        double &p = prev[(size_t)b * nr + r];
+                        rate = std::max(rate, p < 1e299 && p > 0 ? rel / p : 0.0);
+                        if (!(rel > 0.5 * p && p < 1e-9)) stalled = false;
+                        p = rel;
+                    }
+                    dxr[b] = worst;
+                    iters[b] += 1;
+                    // recorded rate: geometric mean contraction since the first correction
+                    if (it == 0) first[b] = worst;
+                    if (it >= 1 && !stalled && first[b] > 0)
+                        items[b].rate = std::pow(worst / first[b], 1.0 / it);
+                    else if (it == 0 && worst <= o.tol) items[b].rate = worst;  // converged at once: rate at most this
+                    if (worst <= o.tol || stalled) {
+                        conv[b] = 1;
+                        continue;
+                    }
+                    // contraction too slow to reach tol within max_iter: stop now
+                    if (it >= 1) {
+                        const double gm = items[b].rate;
+                        const double need = gm > 0 && gm < 1 ? std::log(o.tol / worst) / std::log(gm) : 1e9;
+                        if (it + 1 + need > o.max_iter) {
+                            failed.push_back(b);
+                            continue;
+                        }
+                    }
+                    still.push_back(b);
+                }
+                active.swap(still);
+            }
+            for (int b : active) failed.push_back(b);  // ran out of iterations
+            retry.clear();
+            if (q == 0) {
+                for (int b : failed) {
+                    items[b].rate_b3 = items[b].rate;
+                    items[b].rate = -1.0;
+                    lvl[1].push_back(b);
+                    retry.push_back(b);
+                }
+            }
         }
-        // items that did not converge (or were singular in FP32): one batched FP64 refactorization
+        // items that did not converge (or had singular complex64 factors): one batched FP64 refactorization
         std::vector<int> bad;
         for (int b = 0; b < count; ++b)
             if (!conv[b]) bad.push_back(b);
@@ -358,7 +422,10 @@ int solve_items(wm_ctx *c, const wm_solve_opts &o, WorkItem *items, int count, h
             WM_TRY(getrs_perm_batched(c, WM_PREC_F64, n, nr, Z, n, sZ, perm, n, X, n, sV, nb, s, dact));
             WM_TRY(check_device_error(c, s));
             WM_TRY(copy_d2h(hinfo.data(), dinfo, sizeof(int) * count, s));
-            for (int b : bad) fell[b] = items[b].fprec == WM_PREC_F64 ? 2 : 1;
+            for (int b : bad) {
+                fell[b] = items[b].fprec == WM_PREC_F64 ? 2 : 1;
+                items[b].fprec = WM_PREC_F64;
+            }
         }
     }
 
@@ -386,7 +453,7 @@ int solve_items(wm_ctx *c, const wm_solve_opts &o, WorkItem *items, int count, h
             w.st->iters = iters[b];
             w.st->info = hinfo[b];
             w.st->fell_back = fell[b];
-            w.st->prec = (!mixed || fell[b]) ? WM_PREC_F64 : w.fprec;
+            w.st->prec = mixed ? w.fprec : WM_PREC_F64;
             w.st->rate = mixed ? w.rate : -1.0;
             w.st->dx_rel = dxr[b];
         }

@@ -94,9 +94,14 @@ struct wm_geom {
     double *d_src = nullptr;   // ld records of 16 doubles: rs1[3] dc1[3] dl1 an1 rs2[3] dc2[3] dl2 an2
     int *d_src_key = nullptr;  // ld records of 2 ints
     std::vector<double> h_src; // host copy of the source records (self-term precompute)
-    // observed refinement contraction rates (frequency MHz, rate) of mixed-precision
-    // solves on this structure; see wm::predict_rate
-    mutable std::vector<std::pair<float, double>> rate_obs;
+    // observed refinement contraction rates of mixed-precision solves on this
+    // structure; see wm::predict_rate
+    struct RateObs {
+        float freq;   // MHz
+        int prec;     // WM_PREC_BF16X3 or WM_PREC_F32 factors
+        double rate;  // last contraction ||dx_k|| / ||dx_k-1|| (inf: singular factor)
+    };
+    mutable std::vector<RateObs> rate_obs;
 };
 
 namespace wm {
@@ -152,23 +157,30 @@ struct WorkItem {
     wm_c128 *I = nullptr;      // [nrhs][npls]
     wm_c128 *zin = nullptr;    // [nrhs]
     wm_freq_stats *st = nullptr;
-    int fprec = -1;            // mixed precision: factor precision chosen by routing (-1: the batch's)
+    int fprec = -1;            // mixed precision: factor precision chosen by routing (-1: the batch's);
+                               // out: the precision of the factors that produced the result
     double rate = -1.0;        // out: last refinement contraction rate (inf: complex64 factor singular)
+    double rate_b3 = -1.0;     // out: contraction rate of a failed split-BF16x3 attempt
 };
 
-// Refinement contraction rates scale with the factorization's effective unit
-// roundoff; split-BF16x3 factors (about 16-bit products) contract this many
-// times slower than FP32 factors on the same matrix.
-constexpr double kBf16x3RateFactor = 32.0;
+// Without an observation in one precision, split-BF16x3 factors are taken to
+// contract this many times slower than FP32 factors on the same matrix.  The
+// rounding of Z to complex64, common to both, dominates their backward error
+// (measured: equal iteration counts within one or two on the lightning
+// structure at N = 8192), so the factor is small.
+constexpr double kBf16x3RateFactor = 2.0;
 
-// FP32-factor refinement contraction rate predicted for frequency f on
-// geometry g from its recorded observations, or -1 if there are none.  Below
-// the structure's resonances the EFIE condition number grows as 1/(k dl)^2,
-// and the contraction rate with it: the nearest observation at or above f is
-// scaled by (f_obs / f)^2 (the nearest below, if none).
-double predict_rate(const wm_geom *g, float f);
-// Records the outcome of the refined items of a batch on their geometries
-// (as FP32-equivalent rates).
+// Refinement contraction rate of factors in precision prec (WM_PREC_BF16X3 or
+// WM_PREC_F32) predicted for frequency f on geometry g from its recorded
+// observations, or -1 if there are none.  Below the structure's resonances
+// the EFIE condition number grows as 1/(k dl)^2, and the contraction rate
+// with it, more steeply as the rate nears 1 (measured exponent about 3 near
+// the FP32 limit of the lightning structure): the nearest observation at or
+// above f (the nearest below, if none) is scaled by (f_obs / f)^3;
+// observations in the same precision are preferred, else the other
+// precision's are converted by kBf16x3RateFactor.
+double predict_rate(const wm_geom *g, float f, int prec);
+// Records the refinement rates of the items of a batch on their geometries.
 void record_rates(const WorkItem *items, int count);
 // Routes the items of a batch to the cheapest factor precision (o.prec, then
 // FP32, then FP64) whose predicted rate reaches tol within max_iter.

@@ -281,9 +281,9 @@ def test_mixed_precision_fallback_band(wm, geometries, ctx, geo_dir):
     assert (relb <= 1e-10).all(), relb
     routed = [s["fell_back"] == 2 for s in stb]
     assert routed[0] and not any(routed[-4:])  # lowest routed, highest refined
-    for a, b in zip(st, stb):  # routing only skips refinements that fail
+    for a, b in zip(st, stb):  # routing only skips refinements that fail or take most of max_iter
         if b["fell_back"] == 2:
-            assert a["fell_back"] == 1
+            assert a["fell_back"] == 1 or a["iters"] >= 12, (a, b)
     # precision ladder: split-BF16x3 factors where they refine, then FP32, then FP64
     zb, _, stl = wm.solve_sweep(g, f, [gap], "bf16x3", batch=4)
     rell = np.abs(zb[:, 0] - z64[:, 0]) / np.abs(z64[:, 0])

@@ -48,6 +48,19 @@ CLASSES = {
 REF_BEST_PUBLISHED = 0.148  # freq-solves/s at N=8192, reference's best variant (GPU fill + MKL LU, 6 procs)
 
 
+def _roctx():
+    """roctx range markers (visible to rocprofv3 --marker-trace), or None."""
+    import ctypes as C
+    for name in ("librocprofiler-sdk-roctx.so.1", "/opt/rocm/lib/librocprofiler-sdk-roctx.so.1"):
+        try:
+            L = C.CDLL(name)
+            L.roctxRangePushA.argtypes = [C.c_char_p]
+            return L
+        except OSError:
+            continue
+    return None
+
+
 def _load():
     import __graft_entry__ as ge
     return ge._load_package()
@@ -120,6 +133,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-bf16x3", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="also run a plain device copy of MIB MiB (three times), the reference for HBM counters")
     ap.add_argument("--requests", type=int, default=48)
     ap.add_argument("--rate", type=float, default=4.0)
     a = ap.parse_args()
@@ -148,11 +163,16 @@ def main():
         sweep(a.prec)
     ctx.sync()
     ctx.prof_enable(True)
+    rx = _roctx()
+    if rx:
+        rx.roctxRangePushA(b"timed")
     t0 = time.perf_counter()
     for _ in range(a.steps):
         zin, I, st = sweep(a.prec)
     ctx.sync()
     dt = time.perf_counter() - t0
+    if rx:
+        rx.roctxRangePop()
     prof = {k: ctx.prof_get(k) for k in CLASSES}
     ctx.prof_enable(False)
     value = a.steps * len(freqs) / dt
@@ -239,6 +259,7 @@ def main():
                                           for s in st)),
         "solve_roofline_frac": solve_frac,
         "kernel_ms": {k: round(v[0] / a.steps, 3) for k, v in prof.items()},
+        "kernel_algo": {k: {"launches": v[1], "flops": v[2], "bytes": v[3]} for k, v in prof.items() if v[1]},
         "roofline": roof,
         "bf16x3_sweep": other,
         "serving": serving,
@@ -246,6 +267,19 @@ def main():
         "reference_best_published_solves_per_s": REF_BEST_PUBLISHED,
         "device": f"{name} x{ncu} CU",
     }
+    if a.calibrate_copy:
+        # plain device-to-device copies of known size (the runtime's copy kernel)
+        import ctypes as C
+        L = wm.lib()
+        nbytes = a.calibrate_copy * 2 ** 20
+        src, dst = C.c_void_p(), C.c_void_p()
+        wm._check(L.wm_dev_alloc(ctx.h, nbytes, C.byref(src)))
+        wm._check(L.wm_dev_alloc(ctx.h, nbytes, C.byref(dst)))
+        for _ in range(3):
+            wm._check(L.wm_memcpy_d2d(ctx.h, dst, src, nbytes))
+        wm._check(L.wm_dev_free(ctx.h, src))
+        wm._check(L.wm_dev_free(ctx.h, dst))
+        out["calibrate_copy_bytes"] = nbytes
     print(json.dumps(out))
 
 

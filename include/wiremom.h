@@ -80,6 +80,8 @@ int wm_dev_alloc(wm_ctx *ctx, size_t bytes, void **ptr);
 int wm_dev_free(wm_ctx *ctx, void *ptr);
 int wm_memcpy_h2d(wm_ctx *ctx, void *dst, const void *src, size_t bytes);
 int wm_memcpy_d2h(wm_ctx *ctx, void *dst, const void *src, size_t bytes);
+/* Device-to-device copy on the context's stream; returns when it is done. */
+int wm_memcpy_d2d(wm_ctx *ctx, void *dst, const void *src, size_t bytes);
 
 /* ---- geometry ------------------------------------------------------------ */
 typedef struct wm_geom wm_geom;

@@ -30,7 +30,7 @@ NSMAX = NWMAX + NPWMAX
 EXPORTS = [
     "wm_last_error", "wm_version", "wm_init", "wm_shutdown", "wm_sync", "wm_device_info", "wm_prof_enable",
     "wm_prof_get", "wm_dev_alloc",
-    "wm_dev_free", "wm_memcpy_h2d", "wm_memcpy_d2h", "wm_geom_create", "wm_geom_load", "wm_geom_destroy",
+    "wm_dev_free", "wm_memcpy_h2d", "wm_memcpy_d2h", "wm_memcpy_d2d", "wm_geom_create", "wm_geom_load", "wm_geom_destroy",
     "wm_geom_npls", "wm_geom_ld", "wm_zfill", "wm_getrf_batched", "wm_getrs_batched", "wm_solve_opts_default",
     "wm_solve_sweep", "wm_server_opts_default", "wm_server_create", "wm_server_submit", "wm_server_step",
     "wm_server_result", "wm_server_status", "wm_server_done", "wm_server_destroy", "geoinp", "config", "configwire", "zmatrix",
@@ -89,6 +89,7 @@ def lib() -> C.CDLL:
         L.wm_dev_free.argtypes = [vp, vp]
         L.wm_memcpy_h2d.argtypes = [vp, vp, vp, C.c_size_t]
         L.wm_memcpy_d2h.argtypes = [vp, vp, vp, C.c_size_t]
+        L.wm_memcpy_d2d.argtypes = [vp, vp, vp, C.c_size_t]
         L.wm_geom_create.argtypes = [vp, fp, fp, fp, fp, ip, ip, ip, C.c_int, C.c_int, _p(vp)]
         L.wm_geom_load.argtypes = [vp, C.c_char_p, C.c_char_p, _p(vp)]
         L.wm_geom_destroy.argtypes = [vp]

@@ -362,6 +362,15 @@ int wm_memcpy_d2h(wm_ctx *c, void *dst, const void *src, size_t bytes)
     return copy_d2h(dst, src, bytes, c->stream);
 }
 
+int wm_memcpy_d2d(wm_ctx *c, void *dst, const void *src, size_t bytes)
+{
+    if (!c) return WM_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    WM_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream));
+    WM_HIP_TRY(hipStreamSynchronize(c->stream));
+    return WM_OK;
+}
+
 // ---- geometry -------------------------------------------------------------------
 int wm_geom_create(wm_ctx *c, const float *x, const float *y, const float *z, const float *rad, const int *ipc,
                    const int *ips, const int *ise, int nsmax, int npls, wm_geom **out)

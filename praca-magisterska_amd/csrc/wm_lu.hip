@@ -453,9 +453,9 @@ __global__ __launch_bounds__(256) void k_swap_trsm(typename CT<T>::V *A, int lda
 // chunks of 16 staged through LDS; the next chunk is fetched from global
 // memory into registers while the current one is multiplied.
 //
-// Tiles are ordered column by column and dealt to the XCDs in contiguous runs
-// (workgroup ids go to the 8 XCDs round-robin), so tiles that share a block
-// column of U12 share an L2.
+// Tiles are dealt to the XCDs in contiguous runs (workgroup ids go to the 8
+// XCDs round-robin) and ordered in groups of row tiles (tile_coords), so the
+// tiles an XCD runs at once share their L and U blocks in its L2.
 
 // C = A[r0.., c0..] (mrows x ncols) -= A[r0.., kk..kk+K) * A[kk..kk+K, c0..); K a multiple of 16.
 struct GemmArgs {
@@ -471,6 +471,22 @@ __device__ inline int xcd_tile(int g, int T)
 
 inline int xcd_grid(int T) { return 8 * ((T + 7) / 8); }
 
+// Tile t (of TI x TJ) -> (row tile, column tile), in groups of kGroupM row
+// tiles: consecutive t walk down a group's rows, then across its columns.
+// With the contiguous per-XCD runs of xcd_tile, the 64 tiles an XCD runs at
+// once cover about 8 row blocks x 8 column blocks, so their L and U blocks
+// are shared through that XCD's L2 instead of each tile streaming its own.
+constexpr int kGroupM = 8;
+__device__ inline void tile_coords(int t, int TI, int TJ, int &ti, int &tj)
+{
+    const int per = kGroupM * TJ;
+    const int g = t / per, r0 = g * kGroupM;
+    const int rows = (TI - r0 < kGroupM) ? TI - r0 : kGroupM;
+    const int u = t - g * per;
+    ti = r0 + u % rows;
+    tj = u / rows;
+}
+
 // complex128: workgroup tile 128 (rows) x 64 (columns), four waves of 64 x 32,
 // v_mfma_f64_16x16x4f64.  Lane l holds srcA[l%16][l/16], srcB[l/16][l%16] and
 // D[l/16 + 4*v][l%16] (measured on gfx950).
@@ -484,7 +500,9 @@ __global__ __launch_bounds__(256, 2) void k_gemm_f64(double2 *Abase, int lda, in
     const int b = bidx ? bidx[blockIdx.z] : (int)blockIdx.z;
     double2 *M = Abase + (size_t)b * strideA;
     const int mrows = g.mrows, ncols = g.ncols, K = g.K;
-    const int i0 = (t % TI) * GD_BM, j0 = (t / TI) * GD_BN;
+    int ti, tj;
+    tile_coords(t, TI, TJ, ti, tj);
+    const int i0 = ti * GD_BM, j0 = tj * GD_BN;
     const double2 *A = M + g.r0 + (size_t)g.kk * lda;  // L block
     const double2 *B = M + g.kk + (size_t)g.c0 * lda;  // U block
     double2 *C = M + g.r0 + (size_t)g.c0 * lda;        // updated block
@@ -595,7 +613,9 @@ __global__ __launch_bounds__(256, 2) void k_gemm_f32(float2 *Abase, int lda, int
     const int b = bidx ? bidx[blockIdx.z] : (int)blockIdx.z;
     float2 *M = Abase + (size_t)b * strideA;
     const int mrows = g.mrows, ncols = g.ncols, K = g.K;
-    const int i0 = (t % TI) * GS_BM, j0 = (t / TI) * GS_BN;
+    int ti, tj;
+    tile_coords(t, TI, TJ, ti, tj);
+    const int i0 = ti * GS_BM, j0 = tj * GS_BN;
     const float2 *A = M + g.r0 + (size_t)g.kk * lda;
     const float2 *B = M + g.kk + (size_t)g.c0 * lda;
     float2 *C = M + g.r0 + (size_t)g.c0 * lda;
@@ -723,7 +743,9 @@ __global__ __launch_bounds__(256, 2) void k_gemm_b3(float2 *Abase, int lda, int6
     const int b = bidx ? bidx[blockIdx.z] : (int)blockIdx.z;
     float2 *M = Abase + (size_t)b * strideA;
     const int mrows = g.mrows, ncols = g.ncols, K = g.K;
-    const int i0 = (t % TI) * GB_BM, j0 = (t / TI) * GB_BN;
+    int ti, tj;
+    tile_coords(t, TI, TJ, ti, tj);
+    const int i0 = ti * GB_BM, j0 = tj * GB_BN;
     const float2 *A = M + g.r0 + (size_t)g.kk * lda;  // L block
     const float2 *B = M + g.kk + (size_t)g.c0 * lda;  // U block
     float2 *C = M + g.r0 + (size_t)g.c0 * lda;

@@ -45,6 +45,9 @@ CLASSES = {
     "panel_f32": ("fp32", 157.3), "swap_trsm_f32": ("fp32", 157.3), "gemm_f32": ("mfma", 157.3),
     "trsv": ("hbm", PEAK_HBM_GBS), "resid": ("hbm", PEAK_HBM_GBS),
 }
+# HBM bytes / algorithmic bytes of k_gemm_f64, from FETCH_SIZE + WRITE_SIZE passes
+# calibrated on a plain copy (profiles/gemm_f64_traffic_c3_n8192.md)
+GEMM_F64_HBM_RATIO = 1.53
 REF_BEST_PUBLISHED = 0.148  # freq-solves/s at N=8192, reference's best variant (GPU fill + MKL LU, 6 procs)
 
 
@@ -214,7 +217,11 @@ def main():
     else:
         ach = fl / (ms * 1e-3) / 1e12
         roof = {"bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak}
-    roof.update({"kernel": dom, "launches": nl, "avg_launch_us": 1e3 * ms / max(nl, 1), "traffic": None})
+    traffic = GEMM_F64_HBM_RATIO * by / max(nl, 1) if dom == "gemm" else None
+    roof.update({"kernel": dom, "launches": nl, "avg_launch_us": 1e3 * ms / max(nl, 1), "traffic": traffic,
+                 "algorithmic_bytes_per_launch": by / max(nl, 1)})
+    if traffic is not None:
+        roof["traffic_source"] = "rocprofv3 FETCH_SIZE+WRITE_SIZE ratio, profiles/gemm_f64_traffic_c3_n8192.md"
 
     # whole-solve roofline fraction: t_ideal / t_measured per freq-solve, each
     # frequency's LU priced at the peak of the precision its factors used

@@ -114,6 +114,15 @@ int wm_zfill(wm_ctx *ctx, const wm_geom *g, int nfreq, const double *ka,
  * info (host, batch entries, may be NULL). */
 int wm_getrf_batched(wm_ctx *ctx, int prec, int n, void *A, int lda, int64_t strideA,
                      int *ipiv, int64_t strideP, int batch, int *info);
+/* Factorization tuning.  The blocked LU applies a trailing update after every
+ * `panels` panels of 64 columns (K = 64 * panels), per factor precision and
+ * size class (n / 1024).  wm_set_getrf_block sets it (0 = default, 4; else
+ * 1, 2, 4 or 8); wm_tune_getrf times one batch of `batch` (0: one launch
+ * width) synthetic n x n factorizations for each choice, keeps the fastest
+ * for this context and returns it with its time. */
+int wm_set_getrf_block(wm_ctx *ctx, int prec, int n, int panels);
+int wm_get_getrf_block(wm_ctx *ctx, int prec, int n);
+int wm_tune_getrf(wm_ctx *ctx, int prec, int n, int batch, int *best_panels, double *ms_best);
 /* Solves A X = B with the factors of wm_getrf_batched; B complex128
  * (n x nrhs, ldb) per member, overwritten by X.  The factors may be complex64
  * (prec != WM_PREC_F64); the solve then runs in FP64 on widened values. */

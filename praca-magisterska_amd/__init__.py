@@ -31,7 +31,7 @@ EXPORTS = [
     "wm_last_error", "wm_version", "wm_init", "wm_shutdown", "wm_sync", "wm_device_info", "wm_prof_enable",
     "wm_prof_get", "wm_dev_alloc",
     "wm_dev_free", "wm_memcpy_h2d", "wm_memcpy_d2h", "wm_memcpy_d2d", "wm_geom_create", "wm_geom_load", "wm_geom_destroy",
-    "wm_geom_npls", "wm_geom_ld", "wm_zfill", "wm_getrf_batched", "wm_getrs_batched", "wm_solve_opts_default",
+    "wm_geom_npls", "wm_geom_ld", "wm_zfill", "wm_getrf_batched", "wm_getrs_batched", "wm_set_getrf_block", "wm_get_getrf_block", "wm_tune_getrf", "wm_solve_opts_default",
     "wm_solve_sweep", "wm_server_opts_default", "wm_server_create", "wm_server_submit", "wm_server_step",
     "wm_server_result", "wm_server_status", "wm_server_done", "wm_server_destroy", "geoinp", "config", "configwire", "zmatrix",
     "zmatrix_gpu", "zmatrix_gpu_rank", "LU_cuBLAS", "CalculateByFreq",
@@ -47,6 +47,10 @@ class WireMoMError(RuntimeError):
 class SolveOpts(C.Structure):
     _fields_ = [("prec", C.c_int), ("max_iter", C.c_int), ("tol", C.c_double), ("batch", C.c_int),
                 ("hbm_fraction", C.c_double), ("fallback_f64", C.c_int)]
+
+
+def _prec(p) -> int:
+    return PREC_NAMES[p] if isinstance(p, str) else int(p)
 
 
 def _stats(s) -> dict:
@@ -99,6 +103,9 @@ def lib() -> C.CDLL:
         L.wm_getrf_batched.argtypes = [vp, C.c_int, C.c_int, vp, C.c_int, C.c_int64, vp, C.c_int64, C.c_int, ip]
         L.wm_getrs_batched.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, C.c_int, C.c_int64, vp, C.c_int64, vp,
                                        C.c_int, C.c_int64, C.c_int]
+        L.wm_set_getrf_block.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+        L.wm_get_getrf_block.argtypes = [vp, C.c_int, C.c_int]
+        L.wm_tune_getrf.argtypes = [vp, C.c_int, C.c_int, C.c_int, ip, dp]
         L.wm_solve_opts_default.argtypes = [_p(SolveOpts)]
         L.wm_solve_opts_default.restype = None
         L.wm_solve_sweep.argtypes = [vp, vp, _p(SolveOpts), C.c_int, fp, C.c_int, ip, C.c_float, C.c_float, vp, vp,
@@ -255,6 +262,19 @@ class Context:
         n = C.c_int64()
         _check(lib().wm_prof_get(self.h, kernel.encode(), C.byref(ms), C.byref(n), C.byref(fl), C.byref(by)))
         return ms.value, n.value, fl.value, by.value
+
+    def set_getrf_block(self, prec, n: int, panels: int):
+        """Panels per trailing update of the factorization for (prec, size class of n); 0 = default."""
+        _check(lib().wm_set_getrf_block(self.h, _prec(prec), n, panels))
+
+    def getrf_block(self, prec, n: int) -> int:
+        return lib().wm_get_getrf_block(self.h, _prec(prec), n)
+
+    def tune_getrf(self, prec, n: int, batch: int = 0):
+        """Times each outer-block depth on synthetic matrices, keeps the fastest; returns (panels, ms)."""
+        best, ms = C.c_int(), C.c_double()
+        _check(lib().wm_tune_getrf(self.h, _prec(prec), n, batch, C.byref(best), C.byref(ms)))
+        return best.value, ms.value
 
     def info(self):
         name = C.create_string_buffer(256)

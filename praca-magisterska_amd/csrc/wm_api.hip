@@ -532,6 +532,103 @@ int wm_getrf_batched(wm_ctx *c, int prec, int n, void *A, int lda, int64_t strid
     return first;
 }
 
+// ---- factorization tuning ---------------------------------------------------------------
+namespace {
+
+// Diagonally dominant test matrices for timing: entries in [-1, 1) from a hash
+// of (member, i, j), n on the diagonal; complex128 (esz 16) or complex64.
+__global__ void k_tune_fill(void *A, int n, int esz, int batch)
+{
+    const int64_t nn = (int64_t)n * n;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nn * batch) return;
+    const int64_t b = t / nn, r = t % nn;
+    const int i = (int)(r % n), j = (int)(r / n);
+    uint32_t h = (uint32_t)(b * 2654435761u) ^ (uint32_t)(i * 40503u) ^ (uint32_t)(j * 69069u + 12345u);
+    h ^= h >> 15;
+    h *= 0x2c1b3c6du;
+    h ^= h >> 12;
+    const double x = (double)(h & 0xffff) / 32768.0 - 1.0, y = (double)(h >> 16) / 32768.0 - 1.0;
+    const double d = (i == j) ? (double)n : 0.0;
+    if (esz == 16) ((double2 *)A)[t] = make_double2(x + d, y);
+    else ((float2 *)A)[t] = make_float2((float)(x + d), (float)y);
+}
+
+int valid_block(int panels) { return panels == 0 || panels == 1 || panels == 2 || panels == 4 || panels == 8; }
+
+}  // namespace
+
+int wm_set_getrf_block(wm_ctx *c, int prec, int n, int panels)
+{
+    if (!c || prec < 0 || prec > 2 || n <= 0 || !valid_block(panels)) {
+        set_error("wm_set_getrf_block: bad arguments (panels must be 0, 1, 2, 4 or 8)");
+        return WM_ERR_ARG;
+    }
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    c->getrf_block[prec][std::min(n / 1024, wm_ctx::kTuneSizes - 1)] = panels;
+    return WM_OK;
+}
+
+int wm_get_getrf_block(wm_ctx *c, int prec, int n)
+{
+    if (!c || prec < 0 || prec > 2 || n <= 0) return 0;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    return getrf_block(c, prec, n);
+}
+
+int wm_tune_getrf(wm_ctx *c, int prec, int n, int batch, int *best, double *ms_best)
+{
+    if (!c || prec < 0 || prec > 2 || n <= 0 || n % 64 != 0) {
+        set_error("wm_tune_getrf: bad arguments (n must be a positive multiple of 64)");
+        return WM_ERR_ARG;
+    }
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    (void)hipSetDevice(c->device);
+    if (batch <= 0) batch = getrf_width(c, prec, n);
+    const int esz = prec == WM_PREC_F64 ? 16 : 8;
+    const int64_t nn = (int64_t)n * n;
+    DevMem A, piv, info;
+    WM_TRY(A.alloc((size_t)nn * batch * esz));
+    WM_TRY(piv.alloc((size_t)n * batch * sizeof(int)));
+    WM_TRY(info.alloc((size_t)batch * sizeof(int)));
+    hipEvent_t e0, e1;
+    WM_HIP_TRY(hipEventCreate(&e0));
+    WM_HIP_TRY(hipEventCreate(&e1));
+    int &slot = c->getrf_block[prec][std::min(n / 1024, wm_ctx::kTuneSizes - 1)];
+    const int saved = slot;
+    int bestp = 0;
+    float bestms = 0.f;
+    int rc = WM_OK;
+    const unsigned nb = (unsigned)((nn * batch + 255) / 256);
+    for (int panels : {1, 2, 4, 8}) {
+        if (panels > 1 && panels * 64 > n) break;
+        slot = panels;
+        float ms = 0.f;
+        for (int rep = 0; rep < 2 && rc == WM_OK; ++rep) {  // the first run warms up
+            hipLaunchKernelGGL(k_tune_fill, dim3(nb), dim3(256), 0, c->stream, A.p, n, esz, batch);
+            if (hipEventRecord(e0, c->stream) != hipSuccess) rc = WM_ERR_HIP;
+            if (rc == WM_OK)
+                rc = getrf_batched(c, prec, n, A.p, n, nn, (int *)piv.p, n, batch, (int *)info.p, c->stream);
+            if (rc == WM_OK && hipEventRecord(e1, c->stream) != hipSuccess) rc = WM_ERR_HIP;
+            if (rc == WM_OK && hipEventSynchronize(e1) != hipSuccess) rc = WM_ERR_HIP;
+            if (rc == WM_OK && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = WM_ERR_HIP;
+        }
+        if (rc != WM_OK) break;
+        if (bestp == 0 || ms < bestms) {
+            bestp = panels;
+            bestms = ms;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc == WM_OK) rc = check_device_error(c, c->stream);
+    slot = rc == WM_OK ? bestp : saved;
+    if (rc != WM_OK) return rc;
+    if (best) *best = bestp;
+    if (ms_best) *ms_best = bestms;
+    return WM_OK;
+}
+
 int wm_getrs_batched(wm_ctx *c, int prec, int n, int nrhs, const void *A, int lda, int64_t strideA, const int *ipiv,
                      int64_t strideP, wm_c128 *B, int ldb, int64_t strideB, int batch)
 {

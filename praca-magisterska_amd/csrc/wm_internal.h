@@ -80,6 +80,11 @@ struct wm_ctx {
     wm::DevBuf ws_sweep;    // matrices and vectors of a sweep batch
     int solve_epoch = 0;    // monotonic tag of triangular-solve flags
     uint32_t panel_tag = 0; // base tag of the next panel launch (packets in ws_lu)
+    // factorization outer-block depth (panels per trailing update) per factor
+    // precision and size class (ld / 1024, capped), 0 = default; set by
+    // wm_tune_getrf or wm_set_getrf_block
+    static const int kTuneSizes = 32;
+    int getrf_block[3][kTuneSizes] = {};
     wm::Prof prof;          // per-kernel-class event timing (off by default)
 };
 
@@ -187,6 +192,8 @@ void record_rates(const WorkItem *items, int count);
 void route_items(const wm_solve_opts &o, WorkItem *items, int count);
 // Matrices per launch of the batched factorization (panel cluster residency).
 int getrf_width(wm_ctx *ctx, int prec, int n);
+// Panels per outer block for a factorization of size n in precision prec.
+int getrf_block(const wm_ctx *ctx, int prec, int n);
 
 // Fill, factor and solve a batch of items on stream s (synchronous).
 // Returns <0 on error, >0 (first zero pivot) if a matrix was singular.

@@ -34,7 +34,7 @@ namespace wm {
 namespace {
 
 constexpr int NB = 64;
-constexpr int kSub = 4;  // panels per outer block of the factorization
+constexpr int kSubDefault = 4;  // panels per outer block of the factorization (tunable per context)
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v16f __attribute__((ext_vector_type(16)));
@@ -117,22 +117,72 @@ struct PanelArgs {
 // Argmax combine: larger value wins, equal values go to the lower row.
 __device__ inline bool better(double v2, int r2, double v1, int r1) { return v2 > v1 || (v2 == v1 && r2 < r1); }
 
+// Cross-lane moves within rows of 16 lanes (DPP: quad permutations and row
+// rotations, every source lane valid).
+template <int CTRL>
+__device__ inline double dpp_f64(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+template <int CTRL>
+__device__ inline int dpp_i32(int v)
+{
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xf, 0xf, false);
+}
+__device__ inline double readlane_f64(double v, int l)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+// Maximum over the wave (quad_perm [1,0,3,2], [2,3,0,1], row_ror 4, row_ror 8,
+// then the four row maxima); every lane gets the result.
+__device__ inline double wave_max_f64(double v)
+{
+    v = fmax(v, dpp_f64<0xB1>(v));
+    v = fmax(v, dpp_f64<0x4E>(v));
+    v = fmax(v, dpp_f64<0x124>(v));
+    v = fmax(v, dpp_f64<0x128>(v));
+    return fmax(fmax(readlane_f64(v, 0), readlane_f64(v, 16)), fmax(readlane_f64(v, 32), readlane_f64(v, 48)));
+}
+__device__ inline int wave_min_i32(int v)
+{
+    v = min(v, dpp_i32<0xB1>(v));
+    v = min(v, dpp_i32<0x4E>(v));
+    v = min(v, dpp_i32<0x124>(v));
+    v = min(v, dpp_i32<0x128>(v));
+    return min(min(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16)),
+               min(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
+}
+
+// GEPP of the (n-k) x NB panel at column k for one matrix per blockIdx.y.
+// Workgroup w holds rows k + w*RW .. k + (w+1)*RW - 1 in registers.  A row is
+// split into TPR parts of W = NB/TPR columns held by different waves ("chunk"
+// c = wave / NWC), so that within a wave the column index is uniform:
+// finished columns are skipped by uniform branches, only the waves owning
+// column j take part in its argmax (a DPP reduction), and each thread carries
+// its row's value in the next pivot column (nxr, nxi), taken from the rank-1
+// update that produces it, so registers are only indexed statically.
 template <typename T, int TPR>
-__global__ __launch_bounds__(256) void k_panel(PanelArgs<T> a)
+__global__ __launch_bounds__(256, 2) void k_panel(PanelArgs<T> a)
 {
     using V = typename CT<T>::V;
-    constexpr int W = NB / TPR;    // columns per thread
-    constexpr int RPW = 64 / TPR;  // rows per wave
-    constexpr int RW = 4 * RPW;    // rows per workgroup
+    constexpr int W = NB / TPR;     // columns per thread
+    constexpr int NWC = 4 / TPR;    // waves per chunk
+    constexpr int RW = 64 * NWC;    // rows per workgroup
     constexpr int NW = XLayout<V>::NW;
     static_assert(NW <= 256 && RW >= NB, "panel layout");
     const int bl = blockIdx.y, w = blockIdx.x;          // bl: member of this launch
     const int b = a.bidx ? a.bidx[bl] : bl;              // b: matrix index
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int chunk = (TPR == 1) ? 0 : lane / RPW;
-    const int row = a.k + w * RW + wave * RPW + (lane % RPW);
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int c = wave / NWC, rb = wave % NWC;           // chunk (columns c*W ..), row block
+    const int k = a.k, base = k + w * RW;
+    const int row = base + rb * 64 + lane;
     const bool valid = row < a.n;
-    const int k = a.k;
     V *Ab = a.A + (size_t)b * a.strideA;
     const XLayout<V> X(a.pmax);
     char *xm = a.xbuf + (size_t)bl * X.mat_bytes;
@@ -140,71 +190,64 @@ __global__ __launch_bounds__(256) void k_panel(PanelArgs<T> a)
     T re[W], im[W];
 #pragma unroll
     for (int i = 0; i < W; ++i) {
-        if (valid) {
-            const V v = Ab[row + (size_t)(k + chunk * W + i) * a.lda];
-            re[i] = v.x;
-            im[i] = v.y;
-        } else {
-            re[i] = 0;
-            im[i] = 0;
-        }
+        V x = V{0, 0};
+        if (valid) x = Ab[row + (size_t)(k + c * W + i) * a.lda];
+        re[i] = x.x;
+        im[i] = x.y;
     }
 
-    __shared__ double s_wv[4];
-    __shared__ int s_wr[4];
-    __shared__ V s_c4[4][NB];  // best row of each wave
-    __shared__ V s_dg[NB];     // row k+j (workgroup 0)
-    __shared__ V s_u[NB];      // pivot row
-    __shared__ V s_d[NB];      // row k+j, for the workgroup holding the pivot row
+    __shared__ double s_wv[NWC];
+    __shared__ int s_wr[NWC];
+    __shared__ V s_cand[NB];  // this workgroup's best row
+    __shared__ V s_dg[NB];    // row k+j (workgroup 0)
+    __shared__ V s_u[NB];     // pivot row
+    __shared__ V s_d[NB];     // row k+j, for the workgroup holding the pivot row
+    __shared__ V s_l[TPR > 1 ? RW : 1];  // multipliers for the other chunks
     __shared__ double s_gv;
-    __shared__ int s_gr, s_gw;
+    __shared__ int s_gr;
 
+    // this row's value in the column of the next step (column 0 is in chunk 0)
+    T nxr = re[0], nxi = im[0];
     for (int j = 0; j < NB; ++j) {
         const int par = j & 1;
         const int pr = k + j;
         const int cj = j / W;
         const uint32_t tag = a.tag0 + (uint32_t)j + 1u;
-        // current value of column j in this thread's row (owner chunk only)
-        T xr = 0, xi = 0;
-#pragma unroll
-        for (int i = 0; i < W; ++i)
-            if (chunk * W + i == j) { xr = re[i]; xi = im[i]; }
-        double v = -1.0;
-        int r = 0x7fffffff;
-        if (valid && row >= pr && chunk == cj) {
-            v = fabs((double)xr) + fabs((double)xi);
-            r = row;
+
+        // 1. best row of each wave of the chunk holding column j
+        if (c == cj) {
+            const double v = (valid && row >= pr) ? fabs((double)nxr) + fabs((double)nxi) : -1.0;
+            const double m = wave_max_f64(v);
+            const unsigned long long bal = __ballot(v == m && v >= 0.0);
+            if (lane == 0) {
+                s_wv[rb] = m;
+                s_wr[rb] = bal ? base + rb * 64 + __builtin_ctzll(bal) : 0x7fffffff;
+            }
         }
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) {
-            const double v2 = __shfl_xor(v, off);
-            const int r2 = __shfl_xor(r, off);
-            if (better(v2, r2, v, r)) { v = v2; r = r2; }
-        }
-        // stage each wave's best row and row k+j in LDS
-        if (valid && row == r) {
-#pragma unroll
-            for (int i = 0; i < W; ++i) s_c4[wave][chunk * W + i] = V{re[i], im[i]};
-        }
-        if (valid && row == pr) {
-#pragma unroll
-            for (int i = 0; i < W; ++i) s_dg[chunk * W + i] = V{re[i], im[i]};
-        }
-        if (lane == 0) { s_wv[wave] = v; s_wr[wave] = r; }
         __syncthreads();
 
-        // publish this workgroup's candidate and (workgroup 0) row k+j
-        int q = 0;
+        // 2. this workgroup's candidate; stage it and (workgroup 0) row k+j in LDS
         double wv = s_wv[0];
         int wr = s_wr[0];
 #pragma unroll
-        for (int q2 = 1; q2 < 4; ++q2)
-            if (better(s_wv[q2], s_wr[q2], wv, wr)) { wv = s_wv[q2]; wr = s_wr[q2]; q = q2; }
+        for (int q = 1; q < NWC; ++q)
+            if (better(s_wv[q], s_wr[q], wv, wr)) { wv = s_wv[q]; wr = s_wr[q]; }
+        if (wr != 0x7fffffff && rb == (wr - base) / 64 && lane == (wr - base) % 64) {
+#pragma unroll
+            for (int i = 0; i < W; ++i) s_cand[c * W + i] = V{re[i], im[i]};
+        }
+        if (w == 0 && rb == 0 && lane == j) {
+#pragma unroll
+            for (int i = 0; i < W; ++i) s_dg[c * W + i] = V{re[i], im[i]};
+        }
+        __syncthreads();
+
+        // 3. publish
         uint64_t *slot = X.slot(xm, par);
         uint64_t *cand = X.cand(xm, par);
         uint64_t *diag = X.diag(xm, par);
         if (tid < NW) {
-            put_pkt(cand + (size_t)w * NW + tid, tag, ((const uint32_t *)s_c4[q])[tid]);
+            put_pkt(cand + (size_t)w * NW + tid, tag, ((const uint32_t *)s_cand)[tid]);
             if (w == 0) put_pkt(diag + tid, tag, ((const uint32_t *)s_dg)[tid]);
         }
         if (tid < 3) {
@@ -213,10 +256,10 @@ __global__ __launch_bounds__(256) void k_panel(PanelArgs<T> a)
             put_pkt(slot + (size_t)w * 4 + tid, tag, word);
         }
 
-        // every workgroup reduces all candidates to the same pivot
+        // 4. every workgroup reduces all candidates to the same pivot (first max)
         if (wave == 0) {
             double gv = -1.0;
-            int gr = 0x7fffffff, gw = 0;
+            int gr = 0x7fffffff;
             for (int s = lane; s < a.P; s += 64) {
                 const uint64_t *ps = slot + (size_t)s * 4;
                 uint64_t w0, w1, w2;
@@ -231,21 +274,17 @@ __global__ __launch_bounds__(256) void k_panel(PanelArgs<T> a)
                 }
                 const double v2 = __longlong_as_double((long long)(((w1 & 0xffffffffull) << 32) | (w0 & 0xffffffffull)));
                 const int r2 = (int)(uint32_t)w2;
-                if (better(v2, r2, gv, gr)) { gv = v2; gr = r2; gw = s; }
+                if (better(v2, r2, gv, gr)) { gv = v2; gr = r2; }
             }
-#pragma unroll
-            for (int off = 32; off >= 1; off >>= 1) {
-                const double v2 = __shfl_xor(gv, off);
-                const int r2 = __shfl_xor(gr, off);
-                const int w2 = __shfl_xor(gw, off);
-                if (better(v2, r2, gv, gr)) { gv = v2; gr = r2; gw = w2; }
-            }
-            if (lane == 0) { s_gv = gv; s_gr = gr; s_gw = gw; }
+            const double m = wave_max_f64(gv);
+            const int r = wave_min_i32(gv == m ? gr : 0x7fffffff);
+            if (lane == 0) { s_gv = m; s_gr = r; }
         }
         __syncthreads();
         const double gv = s_gv;
-        const int gr = s_gr, gw = s_gw;
-        const bool own_gr = gr != pr && gr >= k + w * RW && gr < k + (w + 1) * RW;
+        const int gr = s_gr;
+        const int gw = (gr - k) / RW;
+        const bool own_gr = gr != pr && gw == w;
         if (tid < NW) {
             const uint64_t *pu = cand + (size_t)gw * NW + tid;
             const uint64_t *pd = diag + tid;
@@ -262,50 +301,82 @@ __global__ __launch_bounds__(256) void k_panel(PanelArgs<T> a)
         }
         __syncthreads();
 
+        // 5. interchange rows k+j and gr, scale column j, rank-1 update
         const V p = s_u[j];
         const double2 rinv = (gv != 0.0) ? crecip((double)p.x, (double)p.y) : make_double2(0.0, 0.0);
         if (w == 0 && tid == 0) {
             a.ipiv[(size_t)b * a.strideP + pr] = gr + 1;
             if (gv == 0.0 && a.info[b] == 0) a.info[b] = pr + 1;
         }
-        if (gr != pr && valid) {
-            if (row == gr) {
+        if (gr != pr) {
+            if (own_gr && rb == (gr - base) / 64 && lane == (gr - base) % 64) {
 #pragma unroll
-                for (int i = 0; i < W; ++i) { const V d = s_d[chunk * W + i]; re[i] = d.x; im[i] = d.y; }
-            } else if (row == pr) {
+                for (int i = 0; i < W; ++i) {
+                    const V d = s_d[c * W + i];
+                    re[i] = d.x;
+                    im[i] = d.y;
+                }
+            }
+            if (w == 0 && rb == 0 && lane == j) {
 #pragma unroll
-                for (int i = 0; i < W; ++i) { const V u = s_u[chunk * W + i]; re[i] = u.x; im[i] = u.y; }
+                for (int i = 0; i < W; ++i) {
+                    const V u = s_u[c * W + i];
+                    re[i] = u.x;
+                    im[i] = u.y;
+                }
             }
         }
-        // multiplier: column j of this row, held by chunk cj of the row
-        T cr = 0, ci = 0;
-#pragma unroll
-        for (int i = 0; i < W; ++i)
-            if (chunk * W + i == j) { cr = re[i]; ci = im[i]; }
-        if (TPR > 1) {
-            const int src = (lane % RPW) + RPW * cj;
-            cr = __shfl(cr, src);
-            ci = __shfl(ci, src);
+        const bool act = gv != 0.0 && valid && row > pr;
+        T lr = 0, li = 0;
+        if (c == cj) {
+            // column j after the interchange: the row that moved to gr brings row k+j's value
+            T cr = nxr, ci = nxi;
+            if (own_gr && row == gr) {
+                const V d = s_d[j];
+                cr = d.x;
+                ci = d.y;
+            }
+            if (act) {
+                lr = (T)((double)cr * rinv.x - (double)ci * rinv.y);
+                li = (T)((double)cr * rinv.y + (double)ci * rinv.x);
+            }
+            if (TPR > 1) s_l[rb * 64 + lane] = V{lr, li};
         }
-        if (gv != 0.0 && valid && row > pr) {
-            const T lr = (T)((double)cr * rinv.x - (double)ci * rinv.y);
-            const T li = (T)((double)cr * rinv.y + (double)ci * rinv.x);
+        if (TPR > 1) {
+            __syncthreads();
+            if (c != cj) {
+                const V l = s_l[rb * 64 + lane];
+                lr = l.x;
+                li = l.y;
+            }
+        }
 #pragma unroll
-            for (int i = 0; i < W; ++i) {
-                const int col = chunk * W + i;
-                if (col == j) { re[i] = lr; im[i] = li; }
-                if (col > j) {
+        for (int i = 0; i < W; ++i) {
+            const int col = c * W + i;  // uniform
+            if (col == j) {             // the multiplier replaces column j
+                if (act) {
+                    re[i] = lr;
+                    im[i] = li;
+                }
+            } else if (col > j) {
+                if (act) {
                     const V u = s_u[col];
                     re[i] -= lr * u.x - li * u.y;
                     im[i] -= lr * u.y + li * u.x;
                 }
+                if (col == j + 1) {
+                    nxr = re[i];
+                    nxi = im[i];
+                }
             }
+            // keep the scheduler from hoisting every column's pivot-row load at once
+            if ((i & 3) == 3) __builtin_amdgcn_sched_barrier(0);
         }
     }
 
 #pragma unroll
     for (int i = 0; i < W; ++i)
-        if (valid) Ab[row + (size_t)(k + chunk * W + i) * a.lda] = V{re[i], im[i]};
+        if (valid) Ab[row + (size_t)(k + c * W + i) * a.lda] = V{re[i], im[i]};
 
     // composite permutation of the panel's interchanges, for the other columns:
     // after applying them in order, row rows[t] holds the old content of src[t].
@@ -314,18 +385,18 @@ __global__ __launch_bounds__(256) void k_panel(PanelArgs<T> a)
         int *rows = pm + 1, *src = pm + 1 + 2 * NB;
         int cnt = NB;
         for (int t = 0; t < NB; ++t) { rows[t] = k + t; src[t] = k + t; }
-        for (int j = 0; j < NB; ++j) {
-            const int p = a.ipiv[(size_t)b * a.strideP + k + j] - 1;
-            if (p == k + j) continue;
+        for (int jx = 0; jx < NB; ++jx) {
+            const int pp = a.ipiv[(size_t)b * a.strideP + k + jx] - 1;
+            if (pp == k + jx) continue;
             int tp = -1;
-            if (p < k + NB) tp = p - k;
+            if (pp < k + NB) tp = pp - k;
             else {
                 for (int t = NB; t < cnt; ++t)
-                    if (rows[t] == p) { tp = t; break; }
-                if (tp < 0) { tp = cnt++; rows[tp] = p; src[tp] = p; }
+                    if (rows[t] == pp) { tp = t; break; }
+                if (tp < 0) { tp = cnt++; rows[tp] = pp; src[tp] = pp; }
             }
-            const int tmp = src[j];
-            src[j] = src[tp];
+            const int tmp = src[jx];
+            src[jx] = src[tp];
             src[tp] = tmp;
         }
         pm[0] = cnt;
@@ -948,7 +1019,7 @@ int occupancy_panel(int *blocks_per_cu)
 
 template <typename T, int TPR>
 int getrf_impl(wm_ctx *ctx, int n, typename CT<T>::V *A, int lda, int64_t strideA, int *ipiv, int64_t strideP,
-               int batch, int *d_info, const int *d_bidx, hipStream_t s, int gk)
+               int batch, int *d_info, const int *d_bidx, hipStream_t s, int gk, int kSub)
 {
     using V = typename CT<T>::V;
     constexpr int RW = 256 / TPR;
@@ -1060,6 +1131,14 @@ int getrf_impl(wm_ctx *ctx, int n, typename CT<T>::V *A, int lda, int64_t stride
 
 }  // namespace
 
+int getrf_block(const wm_ctx *ctx, int prec, int n)
+{
+    if (prec < 0 || prec > 2) return kSubDefault;
+    const int sc = std::min(n / 1024, wm_ctx::kTuneSizes - 1);
+    const int v = ctx->getrf_block[prec][sc];
+    return v > 0 ? v : kSubDefault;
+}
+
 int getrf_width(wm_ctx *ctx, int prec, int n)
 {
     int bpc = 0;
@@ -1080,12 +1159,13 @@ int getrf_batched(wm_ctx *ctx, int prec, int n, void *A, int lda, int64_t stride
     }
     if (!d_bidx) hipLaunchKernelGGL(k_zero_int, dim3((batch + 255) / 256), dim3(256), 0, s, d_info, batch);
     else hipLaunchKernelGGL(k_zero_idx, dim3((batch + 255) / 256), dim3(256), 0, s, d_info, d_bidx, batch);
+    const int ks = getrf_block(ctx, prec, n);
     if (prec == WM_PREC_F64)
-        return getrf_impl<double, 2>(ctx, n, (double2 *)A, lda, strideA, ipiv, strideP, batch, d_info, d_bidx, s, 8);
+        return getrf_impl<double, 2>(ctx, n, (double2 *)A, lda, strideA, ipiv, strideP, batch, d_info, d_bidx, s, 8, ks);
     if (prec == WM_PREC_F32)
-        return getrf_impl<float, 1>(ctx, n, (float2 *)A, lda, strideA, ipiv, strideP, batch, d_info, d_bidx, s, 4);
+        return getrf_impl<float, 1>(ctx, n, (float2 *)A, lda, strideA, ipiv, strideP, batch, d_info, d_bidx, s, 4, ks);
     if (prec == WM_PREC_BF16X3)
-        return getrf_impl<float, 1>(ctx, n, (float2 *)A, lda, strideA, ipiv, strideP, batch, d_info, d_bidx, s, 2);
+        return getrf_impl<float, 1>(ctx, n, (float2 *)A, lda, strideA, ipiv, strideP, batch, d_info, d_bidx, s, 2, ks);
     set_error("getrf: precision %d not available", prec);
     return WM_ERR_ARG;
 }

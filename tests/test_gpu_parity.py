@@ -104,6 +104,36 @@ def test_trailing_update_step(wm, ctx, prec, bound):
     assert np.array_equal(got[:r], A64[:r]) and np.array_equal(got[:, :r], A64[:, :r])
 
 
+@pytest.mark.parametrize("panels", [1, 2, 8])
+def test_getrf_outer_block_depths(wm, ctx, oracle, panels):
+    """Every outer-block depth (panels per trailing update) gives LAPACK's pivots."""
+    n = 576
+    rng = np.random.RandomState(panels)
+    A = rng.randn(2, n, n) + 1j * rng.randn(2, n, n)
+    ctx.set_getrf_block("f64", n, panels)
+    try:
+        assert ctx.getrf_block("f64", n) == panels
+        dev, piv, info = wm.getrf_batched(ctx, A, "f64")
+    finally:
+        ctx.set_getrf_block("f64", n, 0)
+    LU, P = dev.download().transpose(0, 2, 1), piv.download()
+    for b in range(2):
+        LUo, pivo, _ = oracle.zgetrf(A[b])
+        assert np.array_equal(P[b], pivo), (panels, b)
+        assert np.abs(LU[b] - LUo).max() <= 1e-11 * np.abs(LUo).max()
+
+
+def test_tune_getrf(wm, ctx):
+    """The tuner times every depth on synthetic matrices and keeps the fastest."""
+    try:
+        best, ms = ctx.tune_getrf("f32", 1024)
+        print("tuned", best, ms)
+        assert best in (1, 2, 4, 8) and ms > 0
+        assert ctx.getrf_block("f32", 1024) == best
+    finally:
+        ctx.set_getrf_block("f32", 1024, 0)
+
+
 def test_getrf_f64_zero_pivot_info(wm, ctx):
     A = np.eye(128, dtype=complex)[None].repeat(2, axis=0)
     A[1, :, 70] = 0

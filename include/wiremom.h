@@ -114,6 +114,10 @@ int wm_zfill(wm_ctx *ctx, const wm_geom *g, int nfreq, const double *ka,
  * info (host, batch entries, may be NULL). */
 int wm_getrf_batched(wm_ctx *ctx, int prec, int n, void *A, int lda, int64_t strideA,
                      int *ipiv, int64_t strideP, int batch, int *info);
+/* Batched factorizations are captured into HIP graphs per (shape, buffers)
+ * and replayed (default on; the profiler, when enabled, times direct launches).
+ * wm_set_graphs(ctx, 0) launches directly and drops the captured graphs. */
+int wm_set_graphs(wm_ctx *ctx, int on);
 /* Factorization tuning.  The blocked LU applies a trailing update after every
  * `panels` panels of 64 columns (K = 64 * panels), per factor precision and
  * size class (n / 1024).  wm_set_getrf_block sets it (0 = default, 4; else

@@ -31,7 +31,7 @@ EXPORTS = [
     "wm_last_error", "wm_version", "wm_init", "wm_shutdown", "wm_sync", "wm_device_info", "wm_prof_enable",
     "wm_prof_get", "wm_dev_alloc",
     "wm_dev_free", "wm_memcpy_h2d", "wm_memcpy_d2h", "wm_memcpy_d2d", "wm_geom_create", "wm_geom_load", "wm_geom_destroy",
-    "wm_geom_npls", "wm_geom_ld", "wm_zfill", "wm_getrf_batched", "wm_getrs_batched", "wm_set_getrf_block", "wm_get_getrf_block", "wm_tune_getrf", "wm_solve_opts_default",
+    "wm_geom_npls", "wm_geom_ld", "wm_zfill", "wm_getrf_batched", "wm_getrs_batched", "wm_set_getrf_block", "wm_get_getrf_block", "wm_tune_getrf", "wm_set_graphs", "wm_solve_opts_default",
     "wm_solve_sweep", "wm_server_opts_default", "wm_server_create", "wm_server_submit", "wm_server_step",
     "wm_server_result", "wm_server_status", "wm_server_done", "wm_server_destroy", "geoinp", "config", "configwire", "zmatrix",
     "zmatrix_gpu", "zmatrix_gpu_rank", "LU_cuBLAS", "CalculateByFreq",
@@ -104,6 +104,7 @@ def lib() -> C.CDLL:
         L.wm_getrs_batched.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, C.c_int, C.c_int64, vp, C.c_int64, vp,
                                        C.c_int, C.c_int64, C.c_int]
         L.wm_set_getrf_block.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+        L.wm_set_graphs.argtypes = [vp, C.c_int]
         L.wm_get_getrf_block.argtypes = [vp, C.c_int, C.c_int]
         L.wm_tune_getrf.argtypes = [vp, C.c_int, C.c_int, C.c_int, ip, dp]
         L.wm_solve_opts_default.argtypes = [_p(SolveOpts)]
@@ -262,6 +263,10 @@ class Context:
         n = C.c_int64()
         _check(lib().wm_prof_get(self.h, kernel.encode(), C.byref(ms), C.byref(n), C.byref(fl), C.byref(by)))
         return ms.value, n.value, fl.value, by.value
+
+    def set_graphs(self, on: bool):
+        """Replay factorizations from captured HIP graphs (default) or launch directly."""
+        _check(lib().wm_set_graphs(self.h, int(on)))
 
     def set_getrf_block(self, prec, n: int, panels: int):
         """Panels per trailing update of the factorization for (prec, size class of n); 0 = default."""

@@ -240,6 +240,8 @@ int wm_init(int device, wm_ctx **out)
     WM_HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     WM_HIP_TRY(hipMalloc(&c->d_sync, sizeof(int) * wm_ctx::kSyncCap));
     WM_HIP_TRY(hipMalloc(&c->d_err, sizeof(int)));
+    WM_HIP_TRY(hipMalloc(&c->d_tag, sizeof(uint32_t)));
+    WM_HIP_TRY(hipMemset(c->d_tag, 0, sizeof(uint32_t)));
     WM_HIP_TRY(hipMemset(c->d_sync, 0, sizeof(int) * wm_ctx::kSyncCap));
     WM_HIP_TRY(hipMemset(c->d_err, 0, sizeof(int)));
     *out = c;
@@ -251,6 +253,8 @@ int wm_shutdown(wm_ctx *c)
     if (!c) return WM_OK;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
+    drop_graphs(c);
+    c->ws_info.release();
     c->ws_fill.release();
     c->ws_lu.release();
     c->ws_solve.release();
@@ -260,6 +264,7 @@ int wm_shutdown(wm_ctx *c)
     c->ws_sweep.release();
     (void)hipFree(c->d_sync);
     (void)hipFree(c->d_err);
+    (void)hipFree(c->d_tag);
     (void)hipStreamDestroy(c->stream);
     if (c == g_default_ctx) g_default_ctx = nullptr;
     delete c;
@@ -518,12 +523,14 @@ int wm_getrf_batched(wm_ctx *c, int prec, int n, void *A, int lda, int64_t strid
     if (!c) return WM_ERR_ARG;
     std::lock_guard<std::recursive_mutex> lk(c->mu);
     (void)hipSetDevice(c->device);
-    DevMem dinfo;
-    WM_TRY(dinfo.alloc(sizeof(int) * std::max(batch, 1)));
-    WM_TRY(getrf_batched(c, prec, n, A, lda, strideA, ipiv, strideP, batch, (int *)dinfo.p, c->stream));
+    // info words live in a context buffer, so repeated calls on the same
+    // buffers replay one captured factorization
+    WM_TRY(c->ws_info.ensure(sizeof(int) * std::max(batch, 1)));
+    int *dinfo = (int *)c->ws_info.ptr;
+    WM_TRY(getrf_batched(c, prec, n, A, lda, strideA, ipiv, strideP, batch, dinfo, c->stream));
     std::vector<int> h(batch);
     WM_TRY(check_device_error(c, c->stream));
-    WM_TRY(copy_d2h(h.data(), dinfo.p, sizeof(int) * batch, c->stream));
+    WM_TRY(copy_d2h(h.data(), dinfo, sizeof(int) * batch, c->stream));
     int first = 0;
     for (int b = 0; b < batch; ++b) {
         if (info) info[b] = h[b];
@@ -557,6 +564,17 @@ __global__ void k_tune_fill(void *A, int n, int esz, int batch)
 int valid_block(int panels) { return panels == 0 || panels == 1 || panels == 2 || panels == 4 || panels == 8; }
 
 }  // namespace
+
+int wm_set_graphs(wm_ctx *c, int on)
+{
+    if (!c) return WM_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    c->use_graphs = on != 0;
+    if (!on) drop_graphs(c);
+    return WM_OK;
+}
 
 int wm_set_getrf_block(wm_ctx *c, int prec, int n, int panels)
 {

@@ -3,9 +3,11 @@
 
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cstdarg>
 #include <cstddef>
 #include <cstdint>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -59,6 +61,12 @@ struct Prof {
 };
 }  // namespace wm
 
+namespace wm {
+// Captured factorization: precision, n, lda, strideA, A, ipiv, strideP, batch,
+// info, index table, outer-block depth, exchange buffer.
+typedef std::array<int64_t, 12> GraphKey;
+}  // namespace wm
+
 struct wm_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -71,6 +79,7 @@ struct wm_ctx {
     int *d_sync = nullptr;  // kSyncCap counters
     int *d_err = nullptr;   // 1 int
     static const int kSyncCap = 4096;
+    wm::DevBuf ws_info;     // info words of wm_getrf_batched
     wm::DevBuf ws_fill;     // per-frequency self-term integrals
     wm::DevBuf ws_lu;       // panel exchange buffers
     wm::DevBuf ws_solve;    // triangular-solve permutation and staging
@@ -79,7 +88,11 @@ struct wm_ctx {
     wm::DevBuf ws_resid;    // residual partial sums
     wm::DevBuf ws_sweep;    // matrices and vectors of a sweep batch
     int solve_epoch = 0;    // monotonic tag of triangular-solve flags
-    uint32_t panel_tag = 0; // base tag of the next panel launch (packets in ws_lu)
+    uint64_t panel_tag = 0; // packet tags used since the exchange buffer was last cleared
+    uint32_t *d_tag = nullptr;  // device tag base, bumped by every factorization launch chunk
+    bool use_graphs = true;     // replay captured factorizations (wm_set_graphs)
+    static const size_t kGraphCap = 64;
+    std::map<wm::GraphKey, hipGraphExec_t> graphs;
     // factorization outer-block depth (panels per trailing update) per factor
     // precision and size class (ld / 1024, capped), 0 = default; set by
     // wm_tune_getrf or wm_set_getrf_block
@@ -137,6 +150,11 @@ int residual_batched(wm_ctx *ctx, int n, int nrhs, const void *Z, int ldz, int64
                      hipStream_t s, const int *d_bidx = nullptr);
 
 int check_device_error(wm_ctx *ctx, hipStream_t s);
+
+// Captured factorizations of a context (see getrf_batched).
+void drop_graphs(wm_ctx *ctx);
+hipGraphExec_t find_graph(wm_ctx *ctx, const GraphKey &key);
+void keep_graph(wm_ctx *ctx, const GraphKey &key, hipGraphExec_t exec);
 
 // Profiler: bracket one launch of class k on stream s with events and record
 // its algorithmic flops and bytes.  No-ops unless ctx->prof.on.

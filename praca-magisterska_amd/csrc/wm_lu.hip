@@ -111,7 +111,8 @@ struct PanelArgs {
     int *err;
     char *xbuf;
     const int *bidx;  // launch member -> matrix index (null: identity)
-    uint32_t tag0;    // packet tags of this launch are tag0 + 1 .. tag0 + NB
+    const uint32_t *tagp;  // device tag base of the current factorization (bumped per launch chunk)
+    uint32_t toff;         // packet tags of this launch are *tagp + toff + 1 .. *tagp + toff + NB
 };
 
 // Argmax combine: larger value wins, equal values go to the lower row.
@@ -206,13 +207,14 @@ __global__ __launch_bounds__(256, 2) void k_panel(PanelArgs<T> a)
     __shared__ double s_gv;
     __shared__ int s_gr;
 
+    const uint32_t tag0 = *a.tagp + a.toff;
     // this row's value in the column of the next step (column 0 is in chunk 0)
     T nxr = re[0], nxi = im[0];
     for (int j = 0; j < NB; ++j) {
         const int par = j & 1;
         const int pr = k + j;
         const int cj = j / W;
-        const uint32_t tag = a.tag0 + (uint32_t)j + 1u;
+        const uint32_t tag = tag0 + (uint32_t)j + 1u;
 
         // 1. best row of each wave of the chunk holding column j
         if (c == cj) {
@@ -976,11 +978,23 @@ __global__ void k_zero_u64(uint64_t *p, size_t n)
     if (i < n) p[i] = 0;
 }
 
-// Exchange buffer of at least `bytes` for the panel launches of one call that
-// together use `ntags` tags.  It is cleared when (re)allocated and when the
-// tag counter could wrap during the call, so no stale packet can carry a
-// current tag.  Clearing is queued on s; work of earlier calls on other
-// streams has been joined into s.
+__global__ void k_tag_bump(uint32_t *tag, uint32_t inc)
+{
+    if (threadIdx.x == 0) *tag += inc;
+}
+
+__global__ void k_tag_set(uint32_t *tag, uint32_t v)
+{
+    if (threadIdx.x == 0) *tag = v;
+}
+
+// Exchange buffer of at least `bytes` for a factorization that uses `ntags`
+// packet tags.  Tags come from the device counter ctx->d_tag, which every launch
+// chunk bumps by n (so replays of a captured factorization get fresh tags); the
+// host tracks the same count.  The buffer is cleared, and the counter reset,
+// when the buffer is (re)allocated and before the counter could wrap, so no
+// stale packet can carry a current tag.  A reallocation drops the captured
+// factorizations, which hold the old buffer.
 int prepare_xbuf(wm_ctx *ctx, size_t bytes, uint64_t ntags, hipStream_t s)
 {
     if (ntags > 0x40000000u) {
@@ -989,21 +1003,16 @@ int prepare_xbuf(wm_ctx *ctx, size_t bytes, uint64_t ntags, hipStream_t s)
     }
     const size_t had = ctx->ws_lu.bytes;
     WM_TRY(ctx->ws_lu.ensure(bytes));
-    if (ctx->ws_lu.bytes != had || (uint64_t)ctx->panel_tag + ntags + NB > 0xFFFFFFFFull) {
+    const bool fresh = ctx->ws_lu.bytes != had;
+    if (fresh) drop_graphs(ctx);
+    if (fresh || ctx->panel_tag + ntags + NB > 0xFFFFFFFFull) {
         const size_t nw = ctx->ws_lu.bytes / sizeof(uint64_t);
         hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, s,
                            (uint64_t *)ctx->ws_lu.ptr, nw);
+        hipLaunchKernelGGL(k_tag_set, dim3(1), dim3(64), 0, s, ctx->d_tag, 0u);
         ctx->panel_tag = 0;
     }
     return WM_OK;
-}
-
-// Tag base of the next panel launch (tags base+1 .. base+NB).
-uint32_t next_tag(wm_ctx *ctx)
-{
-    const uint32_t t = ctx->panel_tag;
-    ctx->panel_tag += NB;
-    return t;
 }
 
 template <typename T, int TPR>
@@ -1017,15 +1026,19 @@ int occupancy_panel(int *blocks_per_cu)
     return WM_OK;
 }
 
+// Launch plan of a batched factorization: matrices per launch chunk so that
+// every panel cluster is co-resident, and the exchange buffer it needs.
+struct GetrfPlan {
+    int pmax = 0, per_launch = 0, nchunks = 0;
+    size_t xbytes = 0;
+};
+
 template <typename T, int TPR>
-int getrf_impl(wm_ctx *ctx, int n, typename CT<T>::V *A, int lda, int64_t strideA, int *ipiv, int64_t strideP,
-               int batch, int *d_info, const int *d_bidx, hipStream_t s, int gk, int kSub)
+int getrf_plan(wm_ctx *ctx, int n, int batch, GetrfPlan *pl)
 {
     using V = typename CT<T>::V;
     constexpr int RW = 256 / TPR;
-    constexpr int CB = (sizeof(T) == 8) ? 16 : 32;
     const int pmax = (n + RW - 1) / RW;
-    const XLayout<V> X(pmax);
     int bpc = 0;
     WM_TRY((occupancy_panel<T, TPR>(&bpc)));
     const int resident = bpc * ctx->num_cu;
@@ -1033,12 +1046,27 @@ int getrf_impl(wm_ctx *ctx, int n, typename CT<T>::V *A, int lda, int64_t stride
         set_error("getrf: panel cluster of %d workgroups exceeds %d resident", pmax, resident);
         return WM_ERR_ARG;
     }
-    // matrices per launch so that every cluster is co-resident
     int per_launch = resident / pmax;
     if (per_launch > batch) per_launch = batch;
     if (per_launch > wm_ctx::kSyncCap) per_launch = wm_ctx::kSyncCap;
-    const int nchunks = (batch + per_launch - 1) / per_launch;
-    WM_TRY(prepare_xbuf(ctx, (size_t)per_launch * X.mat_bytes, (uint64_t)nchunks * n, s));
+    pl->pmax = pmax;
+    pl->per_launch = per_launch;
+    pl->nchunks = (batch + per_launch - 1) / per_launch;
+    pl->xbytes = (size_t)per_launch * XLayout<V>(pmax).mat_bytes;
+    return WM_OK;
+}
+
+// Enqueues the launches of a planned factorization on s (kernel launches only,
+// so the sequence can be captured into a graph).
+template <typename T, int TPR>
+int getrf_impl(wm_ctx *ctx, const GetrfPlan &pl, int n, typename CT<T>::V *A, int lda, int64_t strideA, int *ipiv,
+               int64_t strideP, int batch, int *d_info, const int *d_bidx, hipStream_t s, int gk, int kSub)
+{
+    using V = typename CT<T>::V;
+    constexpr int RW = 256 / TPR;
+    constexpr int CB = (sizeof(T) == 8) ? 16 : 32;
+    const int pmax = pl.pmax, per_launch = pl.per_launch;
+    const XLayout<V> X(pmax);
     char *xbase = (char *)ctx->ws_lu.ptr;
     const double esz = 2.0 * sizeof(T);
     const int kPanelClass = sizeof(T) == 8 ? K_PANEL : K_PANEL32;
@@ -1059,7 +1087,7 @@ int getrf_impl(wm_ctx *ctx, int n, typename CT<T>::V *A, int lda, int64_t stride
         pa.A = G.Ab; pa.lda = lda; pa.strideA = strideA; pa.n = n; pa.k = c;
         pa.P = (n - c + RW - 1) / RW; pa.pmax = pmax;
         pa.ipiv = G.ipb; pa.strideP = strideP; pa.info = G.infob;
-        pa.err = ctx->d_err; pa.xbuf = G.xbuf; pa.bidx = G.bidx; pa.tag0 = next_tag(ctx);
+        pa.err = ctx->d_err; pa.xbuf = G.xbuf; pa.bidx = G.bidx; pa.tagp = ctx->d_tag; pa.toff = (uint32_t)c;
         const double mr = n - c;
         // per column j, (mr-j) scalings and (mr-j)(NB-j-1) complex multiply-adds
         prof_begin(ctx, kPanelClass, G.st);
@@ -1102,6 +1130,7 @@ int getrf_impl(wm_ctx *ctx, int n, typename CT<T>::V *A, int lda, int64_t stride
         G.bidx = d_bidx ? d_bidx + b0 : nullptr;
         G.xbuf = xbase;
         G.st = s;
+        hipLaunchKernelGGL(k_tag_bump, dim3(1), dim3(64), 0, s, ctx->d_tag, (uint32_t)n);
         // Outer blocks of kSub panels: inside a block the panels are applied to
         // the block's own columns at once, while the columns to the right get
         // their interchanges at once and their update from the whole block in
@@ -1149,6 +1178,80 @@ int getrf_width(wm_ctx *ctx, int prec, int n)
     return std::max(1, bpc * ctx->num_cu / pmax);
 }
 
+namespace {
+
+// Enqueues a whole factorization (info reset, tag bumps, all launch chunks).
+template <typename T, int TPR>
+int getrf_enqueue(wm_ctx *ctx, const GetrfPlan &pl, int n, void *A, int lda, int64_t strideA, int *ipiv,
+                  int64_t strideP, int batch, int *d_info, const int *d_bidx, hipStream_t s, int gk, int ks)
+{
+    if (!d_bidx) hipLaunchKernelGGL(k_zero_int, dim3((batch + 255) / 256), dim3(256), 0, s, d_info, batch);
+    else hipLaunchKernelGGL(k_zero_idx, dim3((batch + 255) / 256), dim3(256), 0, s, d_info, d_bidx, batch);
+    return getrf_impl<T, TPR>(ctx, pl, n, (typename CT<T>::V *)A, lda, strideA, ipiv, strideP, batch, d_info, d_bidx,
+                              s, gk, ks);
+}
+
+template <typename T, int TPR>
+int getrf_run(wm_ctx *ctx, int prec, int n, void *A, int lda, int64_t strideA, int *ipiv, int64_t strideP, int batch,
+              int *d_info, const int *d_bidx, hipStream_t s, int gk)
+{
+    const int ks = getrf_block(ctx, prec, n);
+    GetrfPlan pl;
+    WM_TRY((getrf_plan<T, TPR>(ctx, n, batch, &pl)));
+    const uint64_t ntags = (uint64_t)pl.nchunks * n;
+    WM_TRY(prepare_xbuf(ctx, pl.xbytes, ntags, s));
+    if (!ctx->use_graphs || ctx->prof.on) {
+        WM_TRY((getrf_enqueue<T, TPR>(ctx, pl, n, A, lda, strideA, ipiv, strideP, batch, d_info, d_bidx, s, gk, ks)));
+    } else {
+        // Replay a captured factorization of the same shape and buffers, or
+        // capture one.  Its kernels read their tags from the device counter, so
+        // every replay uses fresh tags.
+        const GraphKey key{(int64_t)prec, (int64_t)n, (int64_t)lda, strideA, (int64_t)(intptr_t)A,
+                           (int64_t)(intptr_t)ipiv, strideP, (int64_t)batch, (int64_t)(intptr_t)d_info,
+                           (int64_t)(intptr_t)d_bidx, (int64_t)ks, (int64_t)(intptr_t)ctx->ws_lu.ptr};
+        hipGraphExec_t exec = find_graph(ctx, key);
+        if (!exec) {
+            WM_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            const int rc =
+                getrf_enqueue<T, TPR>(ctx, pl, n, A, lda, strideA, ipiv, strideP, batch, d_info, d_bidx, s, gk, ks);
+            hipGraph_t graph = nullptr;
+            const hipError_t ec = hipStreamEndCapture(s, &graph);
+            if (rc != WM_OK) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return rc;
+            }
+            WM_HIP_TRY(ec);
+            const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            WM_HIP_TRY(ei);
+            keep_graph(ctx, key, exec);
+        }
+        WM_HIP_TRY(hipGraphLaunch(exec, s));
+    }
+    ctx->panel_tag += ntags;
+    return WM_OK;
+}
+
+}  // namespace
+
+void drop_graphs(wm_ctx *ctx)
+{
+    for (auto &kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
+    ctx->graphs.clear();
+}
+
+hipGraphExec_t find_graph(wm_ctx *ctx, const GraphKey &key)
+{
+    auto it = ctx->graphs.find(key);
+    return it == ctx->graphs.end() ? nullptr : it->second;
+}
+
+void keep_graph(wm_ctx *ctx, const GraphKey &key, hipGraphExec_t exec)
+{
+    if (ctx->graphs.size() >= wm_ctx::kGraphCap) drop_graphs(ctx);
+    ctx->graphs[key] = exec;
+}
+
 int getrf_batched(wm_ctx *ctx, int prec, int n, void *A, int lda, int64_t strideA, int *ipiv, int64_t strideP,
                   int batch, int *d_info, hipStream_t s, const int *d_bidx)
 {
@@ -1157,15 +1260,12 @@ int getrf_batched(wm_ctx *ctx, int prec, int n, void *A, int lda, int64_t stride
                   batch);
         return WM_ERR_ARG;
     }
-    if (!d_bidx) hipLaunchKernelGGL(k_zero_int, dim3((batch + 255) / 256), dim3(256), 0, s, d_info, batch);
-    else hipLaunchKernelGGL(k_zero_idx, dim3((batch + 255) / 256), dim3(256), 0, s, d_info, d_bidx, batch);
-    const int ks = getrf_block(ctx, prec, n);
     if (prec == WM_PREC_F64)
-        return getrf_impl<double, 2>(ctx, n, (double2 *)A, lda, strideA, ipiv, strideP, batch, d_info, d_bidx, s, 8, ks);
+        return getrf_run<double, 2>(ctx, prec, n, A, lda, strideA, ipiv, strideP, batch, d_info, d_bidx, s, 8);
     if (prec == WM_PREC_F32)
-        return getrf_impl<float, 1>(ctx, n, (float2 *)A, lda, strideA, ipiv, strideP, batch, d_info, d_bidx, s, 4, ks);
+        return getrf_run<float, 1>(ctx, prec, n, A, lda, strideA, ipiv, strideP, batch, d_info, d_bidx, s, 4);
     if (prec == WM_PREC_BF16X3)
-        return getrf_impl<float, 1>(ctx, n, (float2 *)A, lda, strideA, ipiv, strideP, batch, d_info, d_bidx, s, 2, ks);
+        return getrf_run<float, 1>(ctx, prec, n, A, lda, strideA, ipiv, strideP, batch, d_info, d_bidx, s, 2);
     set_error("getrf: precision %d not available", prec);
     return WM_ERR_ARG;
 }
@@ -1201,14 +1301,15 @@ extern "C" int wmx_panel_step(wm_ctx *ctx, int prec, int n, void *A, int *ipiv, 
         const int RW = 256 / TPRv;
         const int pmax = (n + RW - 1) / RW;
         const XLayout<V> X(pmax);
-        WM_TRY(prepare_xbuf(ctx, X.mat_bytes, NB, ctx->stream));
+        WM_TRY(prepare_xbuf(ctx, X.mat_bytes, (uint64_t)n, ctx->stream));
         char *xbuf = (char *)ctx->ws_lu.ptr;
-        const uint32_t tag0 = next_tag(ctx);
+        hipLaunchKernelGGL(k_tag_bump, dim3(1), dim3(64), 0, ctx->stream, ctx->d_tag, (uint32_t)n);
+        ctx->panel_tag += (uint64_t)n;
         PanelArgs<T> pa;
         pa.A = (V *)Ap; pa.lda = n; pa.strideA = 0; pa.n = n; pa.k = k;
         pa.P = (n - k + RW - 1) / RW; pa.pmax = pmax;
         pa.ipiv = ipiv; pa.strideP = 0; pa.info = info;
-        pa.err = ctx->d_err; pa.xbuf = xbuf; pa.bidx = nullptr; pa.tag0 = tag0;
+        pa.err = ctx->d_err; pa.xbuf = xbuf; pa.bidx = nullptr; pa.tagp = ctx->d_tag; pa.toff = (uint32_t)k;
         constexpr int CB = (sizeof(T) == 8) ? 16 : 32;
         if (sizeof(T) == 8)
             hipLaunchKernelGGL((k_panel<double, 2>), dim3(pa.P, 1), dim3(256), 0, ctx->stream, *(PanelArgs<double> *)&pa);

@@ -35,6 +35,8 @@ namespace {
 
 constexpr int NB = 64;
 constexpr int kSubDefault = 4;  // panels per outer block of the factorization (tunable per context)
+constexpr int kTprF32 = 1;      // parts per panel row (waves sharing a row) for complex64 factors: one
+                                // part keeps the cluster small enough for 16 matrices per launch at N = 8192
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v16f __attribute__((ext_vector_type(16)));
@@ -1172,8 +1174,8 @@ int getrf_width(wm_ctx *ctx, int prec, int n)
 {
     int bpc = 0;
     const bool f64 = prec == WM_PREC_F64;  // complex64 factors (FP32, BF16X3) share the FP32 panel
-    if ((f64 ? occupancy_panel<double, 2>(&bpc) : occupancy_panel<float, 1>(&bpc)) != WM_OK || bpc <= 0) return 1;
-    const int RW = f64 ? 128 : 256;
+    if ((f64 ? occupancy_panel<double, 2>(&bpc) : occupancy_panel<float, kTprF32>(&bpc)) != WM_OK || bpc <= 0) return 1;
+    const int RW = f64 ? 128 : 256 / kTprF32;
     const int pmax = (n + RW - 1) / RW;
     return std::max(1, bpc * ctx->num_cu / pmax);
 }
@@ -1263,9 +1265,9 @@ int getrf_batched(wm_ctx *ctx, int prec, int n, void *A, int lda, int64_t stride
     if (prec == WM_PREC_F64)
         return getrf_run<double, 2>(ctx, prec, n, A, lda, strideA, ipiv, strideP, batch, d_info, d_bidx, s, 8);
     if (prec == WM_PREC_F32)
-        return getrf_run<float, 1>(ctx, prec, n, A, lda, strideA, ipiv, strideP, batch, d_info, d_bidx, s, 4);
+        return getrf_run<float, kTprF32>(ctx, prec, n, A, lda, strideA, ipiv, strideP, batch, d_info, d_bidx, s, 4);
     if (prec == WM_PREC_BF16X3)
-        return getrf_run<float, 1>(ctx, prec, n, A, lda, strideA, ipiv, strideP, batch, d_info, d_bidx, s, 2);
+        return getrf_run<float, kTprF32>(ctx, prec, n, A, lda, strideA, ipiv, strideP, batch, d_info, d_bidx, s, 2);
     set_error("getrf: precision %d not available", prec);
     return WM_ERR_ARG;
 }
@@ -1314,7 +1316,7 @@ extern "C" int wmx_panel_step(wm_ctx *ctx, int prec, int n, void *A, int *ipiv, 
         if (sizeof(T) == 8)
             hipLaunchKernelGGL((k_panel<double, 2>), dim3(pa.P, 1), dim3(256), 0, ctx->stream, *(PanelArgs<double> *)&pa);
         else
-            hipLaunchKernelGGL((k_panel<float, 1>), dim3(pa.P, 1), dim3(256), 0, ctx->stream, *(PanelArgs<float> *)&pa);
+            hipLaunchKernelGGL((k_panel<float, kTprF32>), dim3(pa.P, 1), dim3(256), 0, ctx->stream, *(PanelArgs<float> *)&pa);
         if (do_swap) {
             SwapArgs sa;
             sa.n = n; sa.k = k; sa.pmax = pmax; sa.do_swap = 1;
@@ -1327,7 +1329,7 @@ extern "C" int wmx_panel_step(wm_ctx *ctx, int prec, int n, void *A, int *ipiv, 
         }
         return WM_OK;
     };
-    int rc = (prec == WM_PREC_F64) ? run(double(0), (double2 *)A, 2) : run(float(0), (float2 *)A, 1);
+    int rc = (prec == WM_PREC_F64) ? run(double(0), (double2 *)A, 2) : run(float(0), (float2 *)A, kTprF32);
     WM_HIP_TRY(hipStreamSynchronize(ctx->stream));
     (void)hipFree(info);
     if (rc != WM_OK) return rc;

@@ -142,6 +142,7 @@ typedef struct {
     int batch;             /* frequencies in flight, 0 = auto by HBM budget */
     double hbm_fraction;   /* share of free HBM the sweep may use, default 0.8 */
     int fallback_f64;      /* refactor in FP64 if refinement fails, default 1 */
+    int size_policy;       /* factor a batch in FP64 where that is cheaper at its size, default 1 */
 } wm_solve_opts;
 
 void wm_solve_opts_default(wm_solve_opts *o);
@@ -169,7 +170,9 @@ typedef struct {
  * scaled by (f_obs/f)^3, predict the rate at each lower frequency, and each is
  * factored in the cheapest precision predicted to reach tol within 3/4 of
  * max_iter: the requested one, else FP32 (for BF16X3), else FP64 directly
- * (fell_back = 2).  A split-BF16x3 attempt that does not converge is retried
+ * (fell_back = 2).  A batch whose size makes FP64 factors cheaper than
+ * complex64 factors plus refinement (small N, where both factorizations are
+ * bound by panel latency) is factored in FP64 directly as well.  A split-BF16x3 attempt that does not converge is retried
  * with FP32 factors, an FP32 one with FP64 factors (fell_back = 1).
  * Returns the first nonzero status. */
 int wm_solve_sweep(wm_ctx *ctx, const wm_geom *g, const wm_solve_opts *opts, int nfreq,
@@ -184,6 +187,7 @@ typedef struct {
     int max_inflight;      /* frequencies in flight at once, 0 = auto */
     int prec;              /* WM_PREC_* for every request */
     int streams;           /* device streams used for overlap, default 2 */
+    int size_policy;       /* as wm_solve_opts.size_policy, default 1 */
 } wm_server_opts;
 
 void wm_server_opts_default(wm_server_opts *o);

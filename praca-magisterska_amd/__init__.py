@@ -46,7 +46,7 @@ class WireMoMError(RuntimeError):
 
 class SolveOpts(C.Structure):
     _fields_ = [("prec", C.c_int), ("max_iter", C.c_int), ("tol", C.c_double), ("batch", C.c_int),
-                ("hbm_fraction", C.c_double), ("fallback_f64", C.c_int)]
+                ("hbm_fraction", C.c_double), ("fallback_f64", C.c_int), ("size_policy", C.c_int)]
 
 
 def _prec(p) -> int:
@@ -63,7 +63,8 @@ class FreqStats(C.Structure):
 
 
 class ServerOpts(C.Structure):
-    _fields_ = [("hbm_budget", C.c_size_t), ("max_inflight", C.c_int), ("prec", C.c_int), ("streams", C.c_int)]
+    _fields_ = [("hbm_budget", C.c_size_t), ("max_inflight", C.c_int), ("prec", C.c_int), ("streams", C.c_int),
+                ("size_policy", C.c_int)]
 
 
 _lib = None
@@ -372,8 +373,10 @@ class Geometry:
 
 
 def solve_sweep(geom: Geometry, freqs_mhz, gaps, prec="f64", ampl: float = 1.0, phas: float = 0.0,
-                batch: int = 0, max_iter: int = 20, tol: float = 1e-12, fallback: bool = True):
-    """Whole solves (wm_solve_sweep).  Returns (zin[nf, nrhs], I[nf, nrhs, N], stats list)."""
+                batch: int = 0, max_iter: int = 20, tol: float = 1e-12, fallback: bool = True,
+                size_policy: bool = True):
+    """Whole solves (wm_solve_sweep).  Returns (zin[nf, nrhs], I[nf, nrhs, N], stats list).
+    size_policy=False keeps the requested mixed precision at sizes where FP64 factors are cheaper."""
     f = np.ascontiguousarray(np.atleast_1d(freqs_mhz), np.float32)
     g = np.ascontiguousarray(np.atleast_1d(gaps), np.int32)
     nf, nr, n = len(f), len(g), geom.npls
@@ -381,6 +384,7 @@ def solve_sweep(geom: Geometry, freqs_mhz, gaps, prec="f64", ampl: float = 1.0, 
     lib().wm_solve_opts_default(C.byref(o))
     o.prec = PREC_NAMES[prec] if isinstance(prec, str) else int(prec)
     o.batch, o.max_iter, o.tol, o.fallback_f64 = batch, max_iter, tol, int(fallback)
+    o.size_policy = int(size_policy)
     I = np.zeros((nf, nr, n), np.complex128)
     zin = np.zeros((nf, nr), np.complex128)
     st = (FreqStats * nf)()
@@ -464,11 +468,12 @@ def CalculateByFreq(t: Topology, freq_mhz: float, ngap: int, ampl: float = 1.0, 
 class Server:
     """Continuous-batching request server (wm_server_*)."""
 
-    def __init__(self, ctx: Context, prec="f32", hbm_budget: int = 0, max_inflight: int = 0):
+    def __init__(self, ctx: Context, prec="f32", hbm_budget: int = 0, max_inflight: int = 0,
+                 size_policy: bool = True):
         o = ServerOpts()
         lib().wm_server_opts_default(C.byref(o))
         o.prec = PREC_NAMES[prec] if isinstance(prec, str) else int(prec)
-        o.hbm_budget, o.max_inflight = hbm_budget, max_inflight
+        o.hbm_budget, o.max_inflight, o.size_policy = hbm_budget, max_inflight, int(size_policy)
         h = C.c_void_p()
         _check(lib().wm_server_create(ctx.h, C.byref(o), C.byref(h)))
         self.h, self.ctx = h, ctx

@@ -666,6 +666,7 @@ void wm_solve_opts_default(wm_solve_opts *o)
     o->batch = 0;
     o->hbm_fraction = 0.8;
     o->fallback_f64 = 1;
+    o->size_policy = 1;
 }
 
 }  // extern "C"
@@ -714,7 +715,11 @@ int wm_solve_sweep(wm_ctx *c, const wm_geom *g, const wm_solve_opts *opts, int n
         // mixed sweeps go in batches one factorization launch wide, so that each
         // batch is routed with the rates of the frequencies just above it
         if (mixed) cnt = std::min(cnt, getrf_width(c, WM_PREC_F32, g->ld));
-        if (mixed && f0 > 0) route_items(o, items.data() + f0, cnt);
+        if (mixed && o.size_policy && prefer_f64(c, o.prec, g->ld, cnt)) {
+            for (int b = f0; b < f0 + cnt; ++b) items[b].fprec = WM_PREC_F64;
+        } else if (mixed && f0 > 0) {
+            route_items(o, items.data() + f0, cnt);
+        }
         const int rc = solve_items(c, o, items.data() + f0, cnt, c->stream);
         if (rc < 0) return rc;
         if (rc > 0 && status == WM_OK) status = rc;

@@ -208,6 +208,10 @@ void record_rates(const WorkItem *items, int count);
 // Routes the items of a batch to the cheapest factor precision (o.prec, then
 // FP32, then FP64) whose predicted rate reaches tol within max_iter.
 void route_items(const wm_solve_opts &o, WorkItem *items, int count);
+// True when a batch of `count` items of size n costs less with FP64 factors
+// than with complex64 factors plus refinement (small n: the factorization is
+// bound by the panels' per-column latency in both precisions).
+bool prefer_f64(wm_ctx *c, int prec, int n, int count);
 // Matrices per launch of the batched factorization (panel cluster residency).
 int getrf_width(wm_ctx *ctx, int prec, int n);
 // Panels per outer block for a factorization of size n in precision prec.

@@ -76,6 +76,7 @@ void wm_server_opts_default(wm_server_opts *o)
     o->max_inflight = 0;
     o->prec = WM_PREC_F32;
     o->streams = 1;
+    o->size_policy = 1;
 }
 
 int wm_server_create(wm_ctx *c, const wm_server_opts *o, wm_server **out)
@@ -211,7 +212,12 @@ int wm_server_step(wm_server *s, int wait, int *done)
         (void)hipSetDevice(c->device);
         // mixed precision: structures solved before carry refinement rates that
         // route frequencies which cannot refine to FP64 factors at once
-        if (o.prec != WM_PREC_F64) route_items(o, sorted.data(), (int)sorted.size());
+        if (o.prec != WM_PREC_F64) {
+            if (s->o.size_policy && prefer_f64(c, o.prec, ld, (int)sorted.size()))
+                for (auto &w : sorted) w.fprec = WM_PREC_F64;
+            else
+                route_items(o, sorted.data(), (int)sorted.size());
+        }
         rc = solve_items(c, o, sorted.data(), (int)sorted.size(), c->stream);
         if (rc >= 0 && o.prec != WM_PREC_F64) record_rates(sorted.data(), (int)sorted.size());
     }

@@ -251,7 +251,7 @@ def test_mixed_precision_matches_f64(wm, geometries, ctx, geo_dir, prec):
     g = wm.Geometry.load(ctx, p)
     f = [1.0, 20.0, 60.0, 100.0]
     z64, I64, _ = wm.solve_sweep(g, f, [5, 700], "f64")
-    z32, I32, st = wm.solve_sweep(g, f, [5, 700], prec)
+    z32, I32, st = wm.solve_sweep(g, f, [5, 700], prec, size_policy=False)
     print(st)
     rel = np.abs(z32 - z64).max(axis=1) / np.abs(z64).max(axis=1)
     print(rel)
@@ -285,7 +285,7 @@ def test_server_result_same_alone_and_in_mix(wm, geometries, ctx, geo_dir):
     geometries.planar_grid(p1, 512, seed=21)
     geometries.planar_grid(p2, 500, seed=22)  # same padded size, different structure
     g1, g2 = wm.Geometry.load(ctx, p1), wm.Geometry.load(ctx, p2)
-    s = wm.Server(ctx, "f32")
+    s = wm.Server(ctx, "f32", size_policy=False)  # keep the mixed path at this size
     rid = s.submit(g1, [3.0, 40.0], [7, 100])
     while s.step() == 0:
         pass
@@ -331,14 +331,14 @@ def test_mixed_precision_fallback_band(wm, geometries, ctx, geo_dir):
     gap = geometries.nearest_basis(t, (0, 0, 19.9))
     f = np.logspace(-2, 1, 12).astype(np.float32)
     z64, _, _ = wm.solve_sweep(g, f, [gap], "f64")
-    z32, _, st = wm.solve_sweep(g, f, [gap], "f32")
+    z32, _, st = wm.solve_sweep(g, f, [gap], "f32", size_policy=False)
     rel = np.abs(z32[:, 0] - z64[:, 0]) / np.abs(z64[:, 0])
     print([(float(fr), s["iters"], s["fell_back"], float(r)) for fr, s, r in zip(f, st, rel)])
     assert (rel <= 1e-10).all(), rel
     assert any(s["fell_back"] for s in st)
     # batches of 4: the 4 highest frequencies are refined first and their
     # contraction rates route the lowest ones to FP64 factors directly
-    z32b, _, stb = wm.solve_sweep(g, f, [gap], "f32", batch=4)
+    z32b, _, stb = wm.solve_sweep(g, f, [gap], "f32", batch=4, size_policy=False)
     relb = np.abs(z32b[:, 0] - z64[:, 0]) / np.abs(z64[:, 0])
     print([(float(fr), s["iters"], s["fell_back"], float(r)) for fr, s, r in zip(f, stb, relb)])
     assert (relb <= 1e-10).all(), relb
@@ -348,7 +348,7 @@ def test_mixed_precision_fallback_band(wm, geometries, ctx, geo_dir):
         if b["fell_back"] == 2:
             assert a["fell_back"] == 1 or a["iters"] >= 12, (a, b)
     # precision ladder: split-BF16x3 factors where they refine, then FP32, then FP64
-    zb, _, stl = wm.solve_sweep(g, f, [gap], "bf16x3", batch=4)
+    zb, _, stl = wm.solve_sweep(g, f, [gap], "bf16x3", batch=4, size_policy=False)
     rell = np.abs(zb[:, 0] - z64[:, 0]) / np.abs(z64[:, 0])
     print([(float(fr), s["iters"], s["prec"], s["fell_back"], float(r)) for fr, s, r in zip(f, stl, rell)])
     assert (rell <= 1e-10).all(), rell
@@ -368,12 +368,12 @@ def test_serving_replay_matches_sweeps(wm, geometries, ctx, geo_dir):
         geometries.planar_grid(p, r["n"], seed=r["seed"])
         geoms.append(wm.Geometry.load(ctx, p))
         feeds.append(np.random.RandomState(r["seed"]).choice(geoms[-1].npls, r["nrhs"], replace=False) + 1)
-    srv = wm.Server(ctx, "f32")
+    srv = wm.Server(ctx, "f32", size_policy=False)  # keep the mixed path at these sizes
     out = S.replay(srv, reqs, geoms, feeds)
     srv.close()
     assert all(t > 0 for t in out["latency"])
     for r, g, fd, (zin, I, st) in zip(reqs, geoms, feeds, out["results"]):
-        z32, _, _ = wm.solve_sweep(g, r["freqs"], fd, "f32")
+        z32, _, _ = wm.solve_sweep(g, r["freqs"], fd, "f32", size_policy=False)
         rel32 = np.abs(zin - z32) / np.abs(z32)
         assert rel32.max() <= 1e-12, (r["n"], rel32.max())
         z64, _, _ = wm.solve_sweep(g, r["freqs"], fd, "f64")

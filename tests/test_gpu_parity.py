@@ -384,3 +384,24 @@ def test_serving_replay_matches_sweeps(wm, geometries, ctx, geo_dir):
             bound = max(1e-10, 1e3 * s_["dx_rel"])
             print(r["n"], float(f), s_["iters"], s_["fell_back"], s_["dx_rel"], float(e))
             assert e <= bound, (r["n"], float(f), float(e), s_)
+
+
+def test_large_cage_multi_rhs(wm, geometries, ctx, geo_dir):
+    """The large single-GPU case: a 3-D cage at N = 14336 (panel clusters of 112
+    workgroups, 4 matrices per launch) with 8 feeds; mixed precision (FP32 and
+    split-BF16x3) matches the FP64 path, Zin within 1e-10 and currents within 1e-9."""
+    p = os.path.join(geo_dir, "cage.geo")
+    geometries.cage(p, 14336, seed=4)
+    g = wm.Geometry.load(ctx, p)
+    t = wm.configwire(p, cwd=geo_dir)
+    assert g.npls == 14336
+    gaps = list(np.random.RandomState(4).choice(t.npls, 8, replace=False) + 1)
+    f = [10.0, 25.0, 40.0, 50.0]
+    z64, I64, _ = wm.solve_sweep(g, f, gaps, "f64")
+    for prec in ("f32", "bf16x3"):
+        z, I, st = wm.solve_sweep(g, f, gaps, prec)
+        rel = np.abs(z - z64).max() / np.abs(z64).max()
+        relI = (np.abs(I - I64).max(axis=2) / np.abs(I64).max(axis=2)).max()
+        print(prec, [(s["prec"], s["iters"]) for s in st], rel, relI)
+        assert rel <= 1e-10 and relI <= 1e-9, (prec, rel, relI)
+        assert all(s["prec"] != wm.PREC_F64 for s in st)

@@ -224,11 +224,14 @@ def main():
         roof["traffic_source"] = "rocprofv3 FETCH_SIZE+WRITE_SIZE ratio, profiles/gemm_f64_traffic_c3_n8192.md"
 
     # whole-solve roofline fraction: t_ideal / t_measured per freq-solve, each
-    # frequency's LU priced at the peak of the precision its factors used
+    # frequency's LU priced at the peak of the precision its factors used, with
+    # the real flops of its products (3 real products per complex product for
+    # complex128/complex64 factors: 2 N^3; 4 for split-BF16x3: 8/3 N^3)
     N = g.ld
     iters = float(np.mean([s["iters"] for s in st]))
     peak_of = {0: PEAK_TFLOPS["f64"], 1: PEAK_TFLOPS["f32"], 2: PEAK_TFLOPS["bf16x3"]}
-    t_lu = float(np.mean([8 / 3 * N ** 3 / (peak_of[s["prec"]] * 1e12) for s in st]))
+    lu_flops = {0: 2.0 * N ** 3, 1: 2.0 * N ** 3, 2: 8 / 3 * N ** 3}
+    t_lu = float(np.mean([lu_flops[s["prec"]] / (peak_of[s["prec"]] * 1e12) for s in st]))
     t_ideal = (t_lu + 220 * N ** 2 / (PEAK_FP64_VALU * 1e12)
                + (16 * N ** 2 + iters * 24 * N ** 2) / (PEAK_HBM_GBS * 1e9))
     solve_frac = t_ideal / (dt / (a.steps * len(freqs)))

@@ -194,12 +194,13 @@ bool prefer_f64(wm_ctx *c, int prec, int n, int count)
     if (prec == WM_PREC_F64 || count <= 0) return false;
     // Measured on MI355X: a panel column costs about 7.2 us (complex128) and
     // 8.5 us (complex64) whatever the cluster size; trailing updates run at
-    // about 58 TF/s (FP64), 108 TF/s (FP32) and 190 TF/s FP32-equivalent
-    // (split-BF16x3); a refinement iteration costs about 0.1 us per unknown
-    // (triangular-solve latency chain) plus its 24 N^2 bytes; about 6 of them.
+    // about 72 TF/s (FP64), 108 TF/s (FP32) and 190 TF/s FP32-equivalent
+    // (split-BF16x3), all counted at 8 real flops per complex product; a
+    // refinement iteration costs about 0.1 us per unknown (triangular-solve
+    // latency chain) plus its 24 N^2 bytes; about 6 of them.
     const double nn = (double)n, flu = 8.0 / 3.0 * nn * nn * nn;
     const int w64 = getrf_width(c, WM_PREC_F64, n), w32 = getrf_width(c, prec, n);
-    const double t64 = nn * 7.2e-6 * ((count + w64 - 1) / w64) + count * flu / 58e12;
+    const double t64 = nn * 7.2e-6 * ((count + w64 - 1) / w64) + count * flu / 72e12;
     const double rate = prec == WM_PREC_BF16X3 ? 190e12 : 108e12;
     const double tmix = nn * 8.5e-6 * ((count + w32 - 1) / w32) +
                         count * (flu / rate + 6.0 * (1e-7 * nn + 24.0 * nn * nn / 4e12));

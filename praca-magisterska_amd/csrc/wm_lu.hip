@@ -562,10 +562,16 @@ __device__ inline void tile_coords(int t, int TI, int TJ, int &ti, int &tj)
     tj = u / rows;
 }
 
-// complex128: workgroup tile 128 (rows) x 64 (columns), four waves of 64 x 32,
-// v_mfma_f64_16x16x4f64.  Lane l holds srcA[l%16][l/16], srcB[l/16][l%16] and
-// D[l/16 + 4*v][l%16] (measured on gfx950).
-constexpr int GD_BM = 128, GD_BN = 64, GD_KC = 16;
+// complex128 with three real products per complex product (Gauss): over the
+// K terms, P = sum Br Ar, Q = sum Bi Ai and S = sum (Br + Bi)(Ar + Ai), so
+// Re(LU) = P - Q and Im(LU) = S - P - Q.  Three accumulators instead of two
+// make the wave tile 32 x 32 (2 x 2 tiles of v_mfma_f64_16x16x4f64, lane l
+// holding srcA[l%16][l/16], srcB[l/16][l%16] and D[l/16 + 4*v][l%16]);
+// workgroup tile 64 x 64.  Against four products this takes a quarter fewer
+// MFMA cycles per update.  The error bound per entry is that of the
+// four-product form with |a||b| replaced by (|Ar| + |Ai|)(|Br| + |Bi|), at
+// most twice as large.
+constexpr int GZ_BM = 64, GZ_BN = 64, GZ_KC = 16;
 
 __global__ __launch_bounds__(256, 2) void k_gemm_f64(double2 *Abase, int lda, int64_t strideA, GemmArgs g,
                                                      const int *bidx, int TI, int TJ)
@@ -577,82 +583,71 @@ __global__ __launch_bounds__(256, 2) void k_gemm_f64(double2 *Abase, int lda, in
     const int mrows = g.mrows, ncols = g.ncols, K = g.K;
     int ti, tj;
     tile_coords(t, TI, TJ, ti, tj);
-    const int i0 = ti * GD_BM, j0 = tj * GD_BN;
+    const int i0 = ti * GZ_BM, j0 = tj * GZ_BN;
     const double2 *A = M + g.r0 + (size_t)g.kk * lda;  // L block
     const double2 *B = M + g.kk + (size_t)g.c0 * lda;  // U block
-    double2 *C = M + g.r0 + (size_t)g.c0 * lda;        // updated block
+    double2 *C = M + g.r0 + (size_t)g.c0 * lda;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wi = (wave & 1) * 64, wj = (wave >> 1) * 32;
+    const int wi = (wave & 1) * 32, wj = (wave >> 1) * 32;
 
-    __shared__ double2 sA[GD_KC][GD_BM + 1];
-    __shared__ double2 sB[GD_KC][GD_BN + 1];
-    double2 pa[8], pb[4];
+    __shared__ double2 sA[GZ_KC][GZ_BM + 1];
+    __shared__ double2 sB[GZ_KC][GZ_BN + 1];
+    double2 pa[4], pb[4];
+    // chunk elements moved by this thread: L rows ai, k = ak + 4e; U k = bk, columns bj + 16e
+    const int ai = tid % GZ_BM, ak = tid / GZ_BM;
+    const int bk = tid % GZ_KC, bj = tid / GZ_KC;
+    const bool arow = i0 + ai < mrows;
     auto fetch = [&](int kc) {
+        const double2 *pA = A + (i0 + ai) + (size_t)(kc + ak) * lda;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const int idx = tid + 256 * e, i = idx % GD_BM, kk = idx / GD_BM, gi = i0 + i;
-            pa[e] = gi < mrows ? A[gi + (size_t)(kc + kk) * lda] : double2{0.0, 0.0};
-        }
+        for (int e = 0; e < 4; ++e) pa[e] = arow ? pA[(size_t)(4 * e) * lda] : double2{0.0, 0.0};
+        const double2 *pB = B + (kc + bk) + (size_t)(j0 + bj) * lda;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int idx = tid + 256 * e, kk = idx % GD_KC, j = idx / GD_KC, gj = j0 + j;
-            pb[e] = gj < ncols ? B[(kc + kk) + (size_t)gj * lda] : double2{0.0, 0.0};
-        }
+        for (int e = 0; e < 4; ++e) pb[e] = (j0 + bj + 16 * e < ncols) ? pB[(size_t)(16 * e) * lda] : double2{0.0, 0.0};
     };
     auto stash = [&]() {
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const int idx = tid + 256 * e;
-            sA[idx / GD_BM][idx % GD_BM] = pa[e];
-        }
+        for (int e = 0; e < 4; ++e) sA[ak + 4 * e][ai] = pa[e];
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int idx = tid + 256 * e;
-            sB[idx % GD_KC][idx / GD_KC] = pb[e];
-        }
+        for (int e = 0; e < 4; ++e) sB[bk][bj + 16 * e] = pb[e];
     };
-    v4d accR[2][4], accI[2][4];
+    v4d P[2][2], Q[2][2], S[2][2];
 #pragma unroll
     for (int x = 0; x < 2; ++x)
 #pragma unroll
-        for (int y = 0; y < 4; ++y) { accR[x][y] = v4d{0, 0, 0, 0}; accI[x][y] = v4d{0, 0, 0, 0}; }
+        for (int y = 0; y < 2; ++y) { P[x][y] = v4d{0, 0, 0, 0}; Q[x][y] = P[x][y]; S[x][y] = P[x][y]; }
 
     fetch(0);
     stash();
     __syncthreads();
-    for (int kc = 0; kc < K; kc += GD_KC) {
-        if (kc + GD_KC < K) fetch(kc + GD_KC);
+    for (int kc = 0; kc < K; kc += GZ_KC) {
+        if (kc + GZ_KC < K) fetch(kc + GZ_KC);
 #pragma unroll
-        for (int ks = 0; ks < GD_KC; ks += 4) {
+        for (int ks = 0; ks < GZ_KC; ks += 4) {
             const int kk = ks + lane / 16;
-            double2 bv[2], av[4];
+            double2 bv[2], av[2];
+            double bs[2], as[2];
 #pragma unroll
-            for (int tj = 0; tj < 2; ++tj) bv[tj] = sB[kk][wj + tj * 16 + lane % 16];
-#pragma unroll
-            for (int ti = 0; ti < 4; ++ti) av[ti] = sA[kk][wi + ti * 16 + lane % 16];
-            // four passes over the tiles so that consecutive MFMAs never share an accumulator
-#pragma unroll
-            for (int tj = 0; tj < 2; ++tj)
-#pragma unroll
-                for (int ti = 0; ti < 4; ++ti)
-                    accR[tj][ti] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[tj].x, av[ti].x, accR[tj][ti], 0, 0, 0);
+            for (int q = 0; q < 2; ++q) {
+                bv[q] = sB[kk][wj + q * 16 + lane % 16];
+                av[q] = sA[kk][wi + q * 16 + lane % 16];
+                bs[q] = bv[q].x + bv[q].y;
+                as[q] = av[q].x + av[q].y;
+            }
 #pragma unroll
             for (int tj = 0; tj < 2; ++tj)
 #pragma unroll
-                for (int ti = 0; ti < 4; ++ti)
-                    accI[tj][ti] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[tj].x, av[ti].y, accI[tj][ti], 0, 0, 0);
+                for (int ti = 0; ti < 2; ++ti) P[tj][ti] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[tj].x, av[ti].x, P[tj][ti], 0, 0, 0);
 #pragma unroll
             for (int tj = 0; tj < 2; ++tj)
 #pragma unroll
-                for (int ti = 0; ti < 4; ++ti)
-                    accR[tj][ti] = __builtin_amdgcn_mfma_f64_16x16x4f64(-bv[tj].y, av[ti].y, accR[tj][ti], 0, 0, 0);
+                for (int ti = 0; ti < 2; ++ti) Q[tj][ti] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[tj].y, av[ti].y, Q[tj][ti], 0, 0, 0);
 #pragma unroll
             for (int tj = 0; tj < 2; ++tj)
 #pragma unroll
-                for (int ti = 0; ti < 4; ++ti)
-                    accI[tj][ti] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[tj].y, av[ti].x, accI[tj][ti], 0, 0, 0);
+                for (int ti = 0; ti < 2; ++ti) S[tj][ti] = __builtin_amdgcn_mfma_f64_16x16x4f64(bs[tj], as[ti], S[tj][ti], 0, 0, 0);
         }
-        if (kc + GD_KC < K) {
+        if (kc + GZ_KC < K) {
             __syncthreads();
             stash();
             __syncthreads();
@@ -661,24 +656,27 @@ __global__ __launch_bounds__(256, 2) void k_gemm_f64(double2 *Abase, int lda, in
 #pragma unroll
     for (int tj = 0; tj < 2; ++tj)
 #pragma unroll
-        for (int ti = 0; ti < 4; ++ti)
+        for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int gi = i0 + wi + ti * 16 + lane % 16;
                 const int gj = j0 + wj + tj * 16 + (lane / 16) + 4 * v;
                 if (gi < mrows && gj < ncols) {
+                    const double p = P[tj][ti][v], q = Q[tj][ti][v];
                     double2 c = C[gi + (size_t)gj * lda];
-                    c.x -= accR[tj][ti][v];
-                    c.y -= accI[tj][ti][v];
+                    c.x -= p - q;
+                    c.y -= S[tj][ti][v] - p - q;
                     C[gi + (size_t)gj * lda] = c;
                 }
             }
 }
 
-// complex64: workgroup tile 128 x 128, four waves of 64 x 64 (2 x 2 MFMA
-// tiles), v_mfma_f32_32x32x2f32.  Lane l holds srcA[l%32][l/32],
-// srcB[l/32][l%32] and D[8*(v/4)+4*(l/32)+v%4][l%32].
-constexpr int GS_BM = 128, GS_BN = 128, GS_KC = 16;
+// complex64 with three real products per complex product, as k_gemm_f64:
+// P = sum Br Ar, Q = sum Bi Ai, S = sum (Br + Bi)(Ar + Ai).  Workgroup tile
+// 128 x 64, four waves of 64 x 32 (2 x 1 tiles of v_mfma_f32_32x32x2f32).
+// Lane l holds srcA[l%32][l/32], srcB[l/32][l%32] and
+// D[8*(v/4)+4*(l/32)+v%4][l%32].
+constexpr int GC_BM = 128, GC_BN = 64, GC_KC = 16;
 
 __global__ __launch_bounds__(256, 2) void k_gemm_f32(float2 *Abase, int lda, int64_t strideA, GemmArgs g,
                                                      const int *bidx, int TI, int TJ)
@@ -690,19 +688,19 @@ __global__ __launch_bounds__(256, 2) void k_gemm_f32(float2 *Abase, int lda, int
     const int mrows = g.mrows, ncols = g.ncols, K = g.K;
     int ti, tj;
     tile_coords(t, TI, TJ, ti, tj);
-    const int i0 = ti * GS_BM, j0 = tj * GS_BN;
+    const int i0 = ti * GC_BM, j0 = tj * GC_BN;
     const float2 *A = M + g.r0 + (size_t)g.kk * lda;
     const float2 *B = M + g.kk + (size_t)g.c0 * lda;
     float2 *C = M + g.r0 + (size_t)g.c0 * lda;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wi = (wave & 1) * 64, wj = (wave >> 1) * 64;
+    const int wi = (wave & 1) * 64, wj = (wave >> 1) * 32;
 
-    __shared__ float2 sA[GS_KC][GS_BM + 2];
-    __shared__ float2 sB[GS_KC][GS_BN + 2];
-    float2 pa[8], pb[8];
-    // per-thread fixed offsets of the chunk elements it moves
-    const int ai = tid % GS_BM, ak = tid / GS_BM;  // A: rows ai, k = ak + 2e
-    const int bk = tid % GS_KC, bj = tid / GS_KC;  // B: k = bk, columns bj + 16e
+    __shared__ float2 sA[GC_KC][GC_BM + 2];
+    __shared__ float2 sB[GC_KC][GC_BN + 2];
+    float2 pa[8], pb[4];
+    // chunk elements moved by this thread: L rows ai, k = ak + 2e; U k = bk, columns bj + 16e
+    const int ai = tid % GC_BM, ak = tid / GC_BM;
+    const int bk = tid % GC_KC, bj = tid / GC_KC;
     const bool arow = i0 + ai < mrows;
     auto fetch = [&](int kc) {
         const float2 *pA = A + (i0 + ai) + (size_t)(kc + ak) * lda;
@@ -710,78 +708,64 @@ __global__ __launch_bounds__(256, 2) void k_gemm_f32(float2 *Abase, int lda, int
         for (int e = 0; e < 8; ++e) pa[e] = arow ? pA[(size_t)(2 * e) * lda] : float2{0.f, 0.f};
         const float2 *pB = B + (kc + bk) + (size_t)(j0 + bj) * lda;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) pb[e] = (j0 + bj + 16 * e < ncols) ? pB[(size_t)(16 * e) * lda] : float2{0.f, 0.f};
+        for (int e = 0; e < 4; ++e) pb[e] = (j0 + bj + 16 * e < ncols) ? pB[(size_t)(16 * e) * lda] : float2{0.f, 0.f};
     };
     auto stash = [&]() {
 #pragma unroll
         for (int e = 0; e < 8; ++e) sA[ak + 2 * e][ai] = pa[e];
 #pragma unroll
-        for (int e = 0; e < 8; ++e) sB[bk][bj + 16 * e] = pb[e];
+        for (int e = 0; e < 4; ++e) sB[bk][bj + 16 * e] = pb[e];
     };
-    v16f accR[2][2], accI[2][2];
+    v16f P[2], Q[2], S[2];
 #pragma unroll
     for (int x = 0; x < 2; ++x)
 #pragma unroll
-        for (int y = 0; y < 2; ++y)
-#pragma unroll
-            for (int v = 0; v < 16; ++v) { accR[x][y][v] = 0.f; accI[x][y][v] = 0.f; }
+        for (int v = 0; v < 16; ++v) { P[x][v] = 0.f; Q[x][v] = 0.f; S[x][v] = 0.f; }
 
     fetch(0);
     stash();
     __syncthreads();
-    for (int kc = 0; kc < K; kc += GS_KC) {
-        if (kc + GS_KC < K) fetch(kc + GS_KC);
+    for (int kc = 0; kc < K; kc += GC_KC) {
+        if (kc + GC_KC < K) fetch(kc + GC_KC);
 #pragma unroll
-        for (int ks = 0; ks < GS_KC; ks += 2) {
+        for (int ks = 0; ks < GC_KC; ks += 2) {
             const int kk = ks + lane / 32;
-            float2 bv[2], av[2];
+            const float2 bv = sB[kk][wj + lane % 32];
+            const float bs = bv.x + bv.y;
+            float2 av[2];
+            float as[2];
 #pragma unroll
             for (int q = 0; q < 2; ++q) {
-                bv[q] = sB[kk][wj + q * 32 + lane % 32];
                 av[q] = sA[kk][wi + q * 32 + lane % 32];
+                as[q] = av[q].x + av[q].y;
             }
 #pragma unroll
-            for (int tj = 0; tj < 2; ++tj)
+            for (int q = 0; q < 2; ++q) P[q] = __builtin_amdgcn_mfma_f32_32x32x2f32(bv.x, av[q].x, P[q], 0, 0, 0);
 #pragma unroll
-                for (int ti = 0; ti < 2; ++ti)
-                    accR[tj][ti] = __builtin_amdgcn_mfma_f32_32x32x2f32(bv[tj].x, av[ti].x, accR[tj][ti], 0, 0, 0);
+            for (int q = 0; q < 2; ++q) Q[q] = __builtin_amdgcn_mfma_f32_32x32x2f32(bv.y, av[q].y, Q[q], 0, 0, 0);
 #pragma unroll
-            for (int tj = 0; tj < 2; ++tj)
-#pragma unroll
-                for (int ti = 0; ti < 2; ++ti)
-                    accI[tj][ti] = __builtin_amdgcn_mfma_f32_32x32x2f32(bv[tj].x, av[ti].y, accI[tj][ti], 0, 0, 0);
-#pragma unroll
-            for (int tj = 0; tj < 2; ++tj)
-#pragma unroll
-                for (int ti = 0; ti < 2; ++ti)
-                    accR[tj][ti] = __builtin_amdgcn_mfma_f32_32x32x2f32(-bv[tj].y, av[ti].y, accR[tj][ti], 0, 0, 0);
-#pragma unroll
-            for (int tj = 0; tj < 2; ++tj)
-#pragma unroll
-                for (int ti = 0; ti < 2; ++ti)
-                    accI[tj][ti] = __builtin_amdgcn_mfma_f32_32x32x2f32(bv[tj].y, av[ti].x, accI[tj][ti], 0, 0, 0);
+            for (int q = 0; q < 2; ++q) S[q] = __builtin_amdgcn_mfma_f32_32x32x2f32(bs, as[q], S[q], 0, 0, 0);
         }
-        if (kc + GS_KC < K) {
+        if (kc + GC_KC < K) {
             __syncthreads();
             stash();
             __syncthreads();
         }
     }
 #pragma unroll
-    for (int tj = 0; tj < 2; ++tj)
+    for (int q = 0; q < 2; ++q)
 #pragma unroll
-        for (int ti = 0; ti < 2; ++ti)
-#pragma unroll
-            for (int v = 0; v < 16; ++v) {
-                const int gi = i0 + wi + ti * 32 + lane % 32;
-                const int gj = j0 + wj + tj * 32 + 8 * (v / 4) + 4 * (lane / 32) + (v % 4);
-                if (gi < mrows && gj < ncols) {
-                    float2 c = C[gi + (size_t)gj * lda];
-                    c.x -= accR[tj][ti][v];
-                    c.y -= accI[tj][ti][v];
-                    C[gi + (size_t)gj * lda] = c;
-                }
+        for (int v = 0; v < 16; ++v) {
+            const int gi = i0 + wi + q * 32 + lane % 32;
+            const int gj = j0 + wj + 8 * (v / 4) + 4 * (lane / 32) + (v % 4);
+            if (gi < mrows && gj < ncols) {
+                const float p = P[q][v], r = Q[q][v];
+                float2 c = C[gi + (size_t)gj * lda];
+                c.x -= p - r;
+                c.y -= S[q][v] - p - r;
+                C[gi + (size_t)gj * lda] = c;
             }
+        }
 }
 
 // complex64 with split-BF16x3 products: every FP32 operand x is split into
@@ -948,11 +932,11 @@ void launch_gemm(int gk, void *A, int lda, int64_t strideA, const GemmArgs &g, i
 {
     if (g.mrows <= 0 || g.ncols <= 0 || g.K <= 0) return;
     if (gk == 8) {
-        const int TI = (g.mrows + GD_BM - 1) / GD_BM, TJ = (g.ncols + GD_BN - 1) / GD_BN;
+        const int TI = (g.mrows + GZ_BM - 1) / GZ_BM, TJ = (g.ncols + GZ_BN - 1) / GZ_BN;
         hipLaunchKernelGGL(k_gemm_f64, dim3(xcd_grid(TI * TJ), 1, nb_), dim3(256), 0, s, (double2 *)A, lda, strideA,
                            g, bidx, TI, TJ);
     } else if (gk == 4) {
-        const int TI = (g.mrows + GS_BM - 1) / GS_BM, TJ = (g.ncols + GS_BN - 1) / GS_BN;
+        const int TI = (g.mrows + GC_BM - 1) / GC_BM, TJ = (g.ncols + GC_BN - 1) / GC_BN;
         hipLaunchKernelGGL(k_gemm_f32, dim3(xcd_grid(TI * TJ), 1, nb_), dim3(256), 0, s, (float2 *)A, lda, strideA,
                            g, bidx, TI, TJ);
     } else {
@@ -1115,10 +1099,13 @@ int getrf_impl(wm_ctx *ctx, const GetrfPlan &pl, int n, typename CT<T>::V *A, in
     auto gemm = [&](const Grp &G, int r0, int c0, int mrows, int ncols, int kk, int K) {
         if (mrows <= 0 || ncols <= 0) return;
         const GemmArgs g{r0, c0, mrows, ncols, kk, K};
-        // C read and written, L and U read once
+        // C read and written, L and U read once; real flops of the products the
+        // kernel issues: 3 real products per complex product (complex128 and
+        // complex64), 4 x 3 bf16 products for split-BF16x3 (counted as 8)
+        const double fpc = gk == 2 ? 8.0 : 6.0;
         prof_begin(ctx, kGemmClass, G.st);
         launch_gemm(gk, G.Ab, lda, strideA, g, G.m, G.bidx, G.st);
-        prof_end(ctx, kGemmClass, G.st, G.m * 8.0 * mrows * ncols * K,
+        prof_end(ctx, kGemmClass, G.st, G.m * fpc * mrows * ncols * K,
                  G.m * (2.0 * mrows * ncols + (double)(mrows + ncols) * K) * esz);
     };
 
@@ -1278,6 +1265,18 @@ int getrf_batched(wm_ctx *ctx, int prec, int n, void *A, int lda, int64_t stride
 // Internal test hooks (not part of include/wiremom.h): run one step of the
 // blocked factorization on a device matrix so tests can check each kernel
 // against numpy.  A is n x n (lda = n), complex128 for prec 0, complex64 else.
+// C -= L U with C = A[K.., K..], L = A[K.., 0..K), U = A[0..K, K..] (no factorization).
+extern "C" int wmx_gemm_k(wm_ctx *ctx, int prec, int n, void *A, int K, int reps)
+{
+    using namespace wm;
+    if (K <= 0 || K >= n || reps <= 0) return WM_ERR_ARG;
+    const GemmArgs g{K, K, n - K, n - K, 0, K};
+    for (int r = 0; r < reps; ++r)
+        launch_gemm(prec == WM_PREC_F64 ? 8 : prec == WM_PREC_BF16X3 ? 2 : 4, A, n, 0, g, 1, nullptr, ctx->stream);
+    WM_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return WM_OK;
+}
+
 extern "C" int wmx_gemm_step(wm_ctx *ctx, int prec, int n, void *A, int k)
 {
     using namespace wm;

@@ -174,10 +174,14 @@ def test_getrf_f64_zero_pivot_info(wm, ctx):
     assert list(info) == [0, 71]
 
 
-@pytest.mark.parametrize("prec,bound", [("f32", 1e-5), ("bf16x3", 1e-3)])
+@pytest.mark.parametrize("prec,bound", [("f32", 4e-5), ("bf16x3", 1e-3)])
 def test_getrf_f32_backward_error(wm, ctx, prec, bound):
     """Complex64 factors: backward error of P A = L U at the working precision
-    (FP32 updates, or split-BF16x3 updates with about 16-bit products)."""
+    (FP32 updates, or split-BF16x3 updates with about 16-bit products).  The FP32
+    updates take three real products per complex product, whose imaginary part
+    sums (Ar + Ai)(Br + Bi): in a numpy emulation of the same blocked LU this
+    about doubles the backward error of four products (7e-6 -> 1.0-1.4e-5 here,
+    n u = 2.3e-5)."""
     n = 384
     rng = np.random.RandomState(5)
     A = (rng.randn(3, n, n) + 1j * rng.randn(3, n, n)).astype(np.complex64)

@@ -91,9 +91,23 @@ typedef struct wm_geom wm_geom;
  * basis functions 0..npls-1.  Free space (ienv = 1). */
 int wm_geom_create(wm_ctx *ctx, const float *x, const float *y, const float *z, const float *rad,
                    const int *ipc, const int *ips, const int *ise, int nsmax, int npls, wm_geom **out);
+/* As wm_geom_create with the environment arguments of the reference's
+ * zmatrix ("wersja referencyjna/Zmatrix.h":7-9): ienv 1 free space, 2 PEC
+ * ground z = 0 by images, whose source coordinates are scaled by
+ * (sgnx, sgny, sgnz) and whose terms are weighted by sgnenv (the reference
+ * uses 1, 1, -1, -1: "wersja referencyjna/Wire_MoM.cu":107-122).  With
+ * ienv 2 the fill evaluates twice the kernels (Zmatrix.cpp:94-145, 174-224).
+ * Other ienv values are rejected (WM_ERR_ARG). */
+int wm_geom_create_env(wm_ctx *ctx, const float *x, const float *y, const float *z, const float *rad,
+                       const int *ipc, const int *ips, const int *ise, int nsmax, int npls, int ienv,
+                       float sgnx, float sgny, float sgnz, float sgnenv, wm_geom **out);
 /* Reads a .geo file and runs the reference segmentation (configwire) on it;
  * writes configwire.txt to `wirefile` unless it is NULL or empty. */
 int wm_geom_load(wm_ctx *ctx, const char *geo_path, const char *wirefile, wm_geom **out);
+/* As wm_geom_load for environment ienv (2: PEC ground; configwire then adds
+ * the virtual ground wires at nodes with z = 0, "wersja referencyjna/
+ * Configuration.cpp":131-202, and the signs are the reference's). */
+int wm_geom_load_env(wm_ctx *ctx, const char *geo_path, const char *wirefile, int ienv, wm_geom **out);
 int wm_geom_destroy(wm_geom *g);
 /* Number of unknowns N, and the padded leading dimension used on device. */
 int wm_geom_npls(const wm_geom *g);
@@ -236,8 +250,8 @@ void configwire(int *npnt, int nnod, float *xnod, float *ynod, float *znod, floa
 
 /* Replaces zmatrix, "wersja referencyjna/Zmatrix.cpp":27-52 and Zmatrix.h:7-9:
  * fills cz (host, npls x npls, column-major) for wavenumber ka.  The fill
- * runs on the GPU.  ienv must be 1 (free space); sgn* are accepted and
- * ignored as they are in free space. */
+ * runs on the GPU.  ienv 1 (free space) or 2 (PEC ground z = 0 with the
+ * image signs sgn*, as wm_geom_create_env); other values print an error. */
 void zmatrix(wm_c128 *cz, float *x, float *y, float *z, int *ipc, int *ips, int *ise, float *rad,
              int npls, double ka, int nsmax, int ienv, float sgnx, float sgny, float sgnz, float sgnenv);
 

@@ -58,6 +58,8 @@ def _declare(L):
     L.wmo_self_integral.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, dp, dp]
     L.wmo_zmatrix_cols.argtypes = [dp, C.c_int, fp, fp, fp, ip, ip, ip, fp, C.c_int, C.c_double, C.c_int,
                                    C.c_int, C.c_int]
+    L.wmo_zmatrix_env_cols.argtypes = [dp, C.c_int, fp, fp, fp, ip, ip, ip, fp, C.c_int, C.c_double, C.c_int,
+                                       C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int, C.c_int]
     L.wmo_wavenumber.argtypes = [C.c_float]
     L.wmo_wavenumber.restype = C.c_double
     L.wmo_zgetrf.argtypes = [C.c_int, dp, C.c_int, ip]
@@ -198,17 +200,25 @@ def self_integral(ifun, ka, dl, an) -> complex:
     return complex(re.value, im.value)
 
 
-def zmatrix(t: Topology, ka: float, cols: tuple[int, int] | None = None) -> np.ndarray:
-    """zmatrix ("wersja referencyjna/Zmatrix.cpp":27-52), free space.
+# Environment arguments of zmatrix for ienv = 1 (free space) and 2 (PEC
+# ground z = 0): sgnx, sgny, sgnz, sgnenv ("wersja referencyjna/Wire_MoM.cu":107-122).
+ENV_SIGNS = {1: (1.0, 1.0, 1.0, 1.0), 2: (1.0, 1.0, -1.0, -1.0)}
+
+
+def zmatrix(t: Topology, ka: float, cols: tuple[int, int] | None = None, ienv: int = 1) -> np.ndarray:
+    """zmatrix ("wersja referencyjna/Zmatrix.cpp":27-52); ienv 2 adds the
+    PEC-ground image terms (cwire1_im / cwire2_im, Zmatrix.cpp:342-446, 559-663).
 
     Returns Z as a Fortran-ordered complex128 (npls, npls) array; with
     ``cols=(n0, n1)`` only those columns are filled (others are zero)."""
     n = t.npls
     Z = np.zeros((n, n), np.complex128, order="F")
     n0, n1 = cols if cols else (0, n)
-    lib().wmo_zmatrix_cols(Z.ctypes.data_as(_p(C.c_double)), n, _ptr(t.x, C.c_float), _ptr(t.y, C.c_float),
-                           _ptr(t.z, C.c_float), _ptr(t.ipc, C.c_int), _ptr(t.ips, C.c_int),
-                           _ptr(t.ise, C.c_int), _ptr(t.rad, C.c_float), n, float(ka), t.nsmax, n0, n1)
+    sx, sy, sz, se = ENV_SIGNS[ienv]
+    lib().wmo_zmatrix_env_cols(Z.ctypes.data_as(_p(C.c_double)), n, _ptr(t.x, C.c_float), _ptr(t.y, C.c_float),
+                               _ptr(t.z, C.c_float), _ptr(t.ipc, C.c_int), _ptr(t.ips, C.c_int),
+                               _ptr(t.ise, C.c_int), _ptr(t.rad, C.c_float), n, float(ka), t.nsmax, ienv,
+                               sx, sy, sz, se, n0, n1)
     return Z
 
 
@@ -249,12 +259,13 @@ def excitation(npls: int, gaps, ampl: float = 1.0, phas: float = 0.0) -> np.ndar
     return B
 
 
-def solve_freq(t: Topology, freq_mhz: float, gaps=(100,), ampl: float = 1.0, phas: float = 0.0):
+def solve_freq(t: Topology, freq_mhz: float, gaps=(100,), ampl: float = 1.0, phas: float = 0.0,
+               ienv: int = 1):
     """One frequency of the reference loop ("wersja referencyjna/Wire_MoM.cu":335-395).
 
     Returns (zin[nrhs], I[npls, nrhs], Z)."""
     ka = wavenumber(freq_mhz)
-    Z = zmatrix(t, ka)
+    Z = zmatrix(t, ka, ienv=ienv)
     LU, ipiv, info = zgetrf(Z)
     if info:
         raise np.linalg.LinAlgError(f"zgetrf info={info}")

@@ -485,27 +485,95 @@ static void wmo_obs_point(const float *x, const float *y, const float *z, const 
     *mr = ise[ms + nsmax - 1];
 }
 
+/* Image of a source half in the PEC ground (cwire1_im / cwire2_im,
+ * Zmatrix.cpp:342-446, 559-663): every coordinate of the half is scaled by
+ * (sgnx, sgny, sgnz), so the start point and unit vector flip sign in the
+ * mirrored axes while the length, radius and node ids stay.              */
+static void wmo_image_half(const wmo_half *h, float sgnx, float sgny, float sgnz, wmo_half *im)
+{
+    *im = *h;
+    im->rx = (double)sgnx * h->rx; im->ry = (double)sgny * h->ry; im->rz = (double)sgnz * h->rz;
+    im->dcx = (double)sgnx * h->dcx; im->dcy = (double)sgny * h->dcy; im->dcz = (double)sgnz * h->dcz;
+}
+
+/* cwire1_im / cwire2_im integral: the self rule applies to the image of the
+ * observation segment itself only when both its nodes lie in the ground
+ * plane (z < 1e-6, Zmatrix.cpp:384); otherwise the 2-point rule with the
+ * mirrored half.                                                          */
+static void wmo_cwire_im(const wmo_half *im, int ifun, int ml, int mr, const float *z, double ka,
+                         double rcx, double rcy, double rcz, double *re, double *imv)
+{
+    const int same = (im->n1 == ml && im->n2 == mr) || (im->n1 == mr && im->n2 == ml);
+    if (same && z[ml - 1] < 1.e-6 && z[mr - 1] < 1.e-6) {
+        wmo_self_integral(ifun, ka, im->dl, im->an, re, imv);
+        return;
+    }
+    double a = 0.5 * im->dl, b = im->dl, c = GL2_C * b;
+    double r1, i1, r2, i2;
+    wmo_ckmn(a + c, ifun, ka, rcx, rcy, rcz, im->rx, im->ry, im->rz, im->dcx, im->dcy, im->dcz, im->an, im->dl, &r1, &i1);
+    wmo_ckmn(a - c, ifun, ka, rcx, rcy, rcz, im->rx, im->ry, im->rz, im->dcx, im->dcy, im->dcz, im->an, im->dl, &r2, &i2);
+    *re = (r1 + r2) * (b * 0.5);
+    *imv = (i1 + i2) * (b * 0.5);
+}
+
+/* Environment of the fill: ienv 1 free space, 2 PEC ground by images with
+ * source coordinates scaled by sgn[0..2] and image terms by sgnenv
+ * (Wire_MoM.cu:107-122 sets 1, 1, -1 and -1 for ienv = 2).             */
+typedef struct {
+    int ienv;
+    float sgnx, sgny, sgnz, sgnenv;
+} wmo_env;
+
 /* One matrix entry before the j*eta0/(4 pi k) scale: the wim() body for
- * test basis m and source basis n, free space (Zmatrix.cpp:86-151, 167-231). */
+ * test basis m and source basis n (Zmatrix.cpp:86-151, 167-231), image
+ * terms in the reference's order when env->ienv == 2.                    */
 static void wmo_entry(const float *x, const float *y, const float *z, const int *ipc,
                       const int *ips, const int *ise, const float *rad, int nsmax,
                       double ka, int m, int n, const wmo_half *h1, const wmo_half *h2,
+                      const wmo_half *i1, const wmo_half *i2, const wmo_env *env,
                       double *zr, double *zi)
 {
+    const int img = env->ienv == 2;
+    const double se = (double)env->sgnenv;
     double accr = 0.0, acci = 0.0;
     for (int half = 1; half <= 2; half++) {
         double rc[3], lm[3];
         int ml, mr;
         wmo_obs_point(x, y, z, ipc, ips, ise, nsmax, m, half, rc, lm, &ml, &mr);
-        double v1r, v1i, v2r, v2i, s1r, s1i, s2r, s2i;
-        wmo_cwire(h1, 1, ml, mr, ka, rc[0], rc[1], rc[2], &v1r, &v1i);
-        double t1 = h1->dcx * lm[0] + h1->dcy * lm[1] + h1->dcz * lm[2];
-        wmo_cwire(h2, 2, ml, mr, ka, rc[0], rc[1], rc[2], &v2r, &v2i);
-        double t2 = h2->dcx * lm[0] + h2->dcy * lm[1] + h2->dcz * lm[2];
-        double cvr = v1r * t1 + v2r * t2, cvi = v1i * t1 + v2i * t2;
-        wmo_cwire(h1, 3, ml, mr, ka, rc[0], rc[1], rc[2], &s1r, &s1i);
-        wmo_cwire(h2, 3, ml, mr, ka, rc[0], rc[1], rc[2], &s2r, &s2i);
-        double csr = s1r / h1->dl - s2r / h2->dl, csi = s1i / h1->dl - s2i / h2->dl;
+        double vr, vi, t;
+        /* vector potential: n+ (and its image), n- (and its image) */
+        wmo_cwire(h1, 1, ml, mr, ka, rc[0], rc[1], rc[2], &vr, &vi);
+        t = h1->dcx * lm[0] + h1->dcy * lm[1] + h1->dcz * lm[2];
+        double cvr = vr * t, cvi = vi * t;
+        if (img) {
+            wmo_cwire_im(i1, 1, ml, mr, z, ka, rc[0], rc[1], rc[2], &vr, &vi);
+            t = (i1->dcx * lm[0] + i1->dcy * lm[1] + i1->dcz * lm[2]) * se;
+            cvr += vr * t; cvi += vi * t;
+        }
+        wmo_cwire(h2, 2, ml, mr, ka, rc[0], rc[1], rc[2], &vr, &vi);
+        t = h2->dcx * lm[0] + h2->dcy * lm[1] + h2->dcz * lm[2];
+        cvr += vr * t; cvi += vi * t;
+        if (img) {
+            wmo_cwire_im(i2, 2, ml, mr, z, ka, rc[0], rc[1], rc[2], &vr, &vi);
+            t = (i2->dcx * lm[0] + i2->dcy * lm[1] + i2->dcz * lm[2]) * se;
+            cvr += vr * t; cvi += vi * t;
+        }
+        /* scalar potential: + n+ / dl, - n- / dl, images scaled by sgnenv */
+        double sr, si;
+        wmo_cwire(h1, 3, ml, mr, ka, rc[0], rc[1], rc[2], &sr, &si);
+        double csr = sr / h1->dl, csi = si / h1->dl;
+        if (img) {
+            wmo_cwire_im(i1, 3, ml, mr, z, ka, rc[0], rc[1], rc[2], &sr, &si);
+            const double w = se / i1->dl;
+            csr += sr * w; csi += si * w;
+        }
+        wmo_cwire(h2, 3, ml, mr, ka, rc[0], rc[1], rc[2], &sr, &si);
+        csr -= sr / h2->dl; csi -= si / h2->dl;
+        if (img) {
+            wmo_cwire_im(i2, 3, ml, mr, z, ka, rc[0], rc[1], rc[2], &sr, &si);
+            const double w = se / i2->dl;
+            csr -= sr * w; csi -= si * w;
+        }
         double kk = ka * ka;
         if (half == 1) { accr += cvr * kk - csr; acci += cvi * kk - csi; }
         else           { accr += cvr * kk + csr; acci += cvi * kk + csi; }
@@ -514,28 +582,40 @@ static void wmo_entry(const float *x, const float *y, const float *z, const int 
     *zi = acci;
 }
 
-/* zmatrix (Zmatrix.cpp:27-52), free space (ienv == 1).  cz is column-major
- * complex128 with leading dimension ldz >= npls: cz[2*(m + n*ldz) + {0,1}].
- * Columns [n0, n1) only (n0=0, n1=npls for the whole matrix).           */
-void wmo_zmatrix_cols(double *cz, int ldz, const float *x, const float *y, const float *z,
-                      const int *ipc, const int *ips, const int *ise, const float *rad,
-                      int npls, double ka, int nsmax, int n0, int n1)
+/* zmatrix (Zmatrix.cpp:27-52) with its environment arguments.  cz is
+ * column-major complex128 with leading dimension ldz >= npls:
+ * cz[2*(m + n*ldz) + {0,1}].  Columns [n0, n1) only.                     */
+void wmo_zmatrix_env_cols(double *cz, int ldz, const float *x, const float *y, const float *z,
+                          const int *ipc, const int *ips, const int *ise, const float *rad,
+                          int npls, double ka, int nsmax, int ienv, float sgnx, float sgny,
+                          float sgnz, float sgnenv, int n0, int n1)
 {
     const double eta0 = 120.0 * WMO_PI;
     const double q = eta0 / (4.0 * WMO_PI * ka);
+    const wmo_env env = {ienv, sgnx, sgny, sgnz, sgnenv};
 #pragma omp parallel for schedule(dynamic, 4)
     for (int n = n0; n < n1; n++) {
-        wmo_half h1, h2;
+        wmo_half h1, h2, i1, i2;
         wmo_source_half(x, y, z, ipc, ips, ise, rad, nsmax, n, 1, &h1);
         wmo_source_half(x, y, z, ipc, ips, ise, rad, nsmax, n, 2, &h2);
+        wmo_image_half(&h1, sgnx, sgny, sgnz, &i1);
+        wmo_image_half(&h2, sgnx, sgny, sgnz, &i2);
         for (int m = 0; m < npls; m++) {
             double zr, zi;
-            wmo_entry(x, y, z, ipc, ips, ise, rad, nsmax, ka, m, n, &h1, &h2, &zr, &zi);
+            wmo_entry(x, y, z, ipc, ips, ise, rad, nsmax, ka, m, n, &h1, &h2, &i1, &i2, &env, &zr, &zi);
             /* multiply by cjq = (0, q) */
             cz[2 * ((size_t)m + (size_t)n * ldz)] = -q * zi;
             cz[2 * ((size_t)m + (size_t)n * ldz) + 1] = q * zr;
         }
     }
+}
+
+/* Free space (ienv == 1). */
+void wmo_zmatrix_cols(double *cz, int ldz, const float *x, const float *y, const float *z,
+                      const int *ipc, const int *ips, const int *ise, const float *rad,
+                      int npls, double ka, int nsmax, int n0, int n1)
+{
+    wmo_zmatrix_env_cols(cz, ldz, x, y, z, ipc, ips, ise, rad, npls, ka, nsmax, 1, 1.f, 1.f, 1.f, 1.f, n0, n1);
 }
 
 void wmo_zmatrix(double *cz, const float *x, const float *y, const float *z,

@@ -26,11 +26,14 @@ PREC_NAMES = {"f64": PREC_F64, "f32": PREC_F32, "bf16x3": PREC_BF16X3}
 # capacities of the reference driver ("wersja referencyjna/Wire_MoM.cu":43-44)
 NNMAX, NTMAX, NWMAX, NPWMAX = 3000, 5000, 10000, 20480
 NSMAX = NWMAX + NPWMAX
+# zmatrix's environment signs (sgnx, sgny, sgnz, sgnenv) for ienv 1 (free
+# space) and 2 (PEC ground z = 0), "wersja referencyjna/Wire_MoM.cu":107-122
+ENV_SIGNS = {1: (1.0, 1.0, 1.0, 1.0), 2: (1.0, 1.0, -1.0, -1.0)}
 
 EXPORTS = [
     "wm_last_error", "wm_version", "wm_init", "wm_shutdown", "wm_sync", "wm_device_info", "wm_prof_enable",
     "wm_prof_get", "wm_dev_alloc",
-    "wm_dev_free", "wm_memcpy_h2d", "wm_memcpy_d2h", "wm_memcpy_d2d", "wm_geom_create", "wm_geom_load", "wm_geom_destroy",
+    "wm_dev_free", "wm_memcpy_h2d", "wm_memcpy_d2h", "wm_memcpy_d2d", "wm_geom_create", "wm_geom_create_env", "wm_geom_load", "wm_geom_load_env", "wm_geom_destroy",
     "wm_geom_npls", "wm_geom_ld", "wm_zfill", "wm_getrf_batched", "wm_getrs_batched", "wm_set_getrf_block", "wm_get_getrf_block", "wm_tune_getrf", "wm_set_graphs", "wm_solve_opts_default",
     "wm_solve_sweep", "wm_server_opts_default", "wm_server_create", "wm_server_submit", "wm_server_step",
     "wm_server_result", "wm_server_status", "wm_server_done", "wm_server_destroy", "geoinp", "config", "configwire", "zmatrix",
@@ -97,6 +100,9 @@ def lib() -> C.CDLL:
         L.wm_memcpy_d2d.argtypes = [vp, vp, vp, C.c_size_t]
         L.wm_geom_create.argtypes = [vp, fp, fp, fp, fp, ip, ip, ip, C.c_int, C.c_int, _p(vp)]
         L.wm_geom_load.argtypes = [vp, C.c_char_p, C.c_char_p, _p(vp)]
+        L.wm_geom_create_env.argtypes = [vp, fp, fp, fp, fp, ip, ip, ip, C.c_int, C.c_int, C.c_int, C.c_float,
+                                         C.c_float, C.c_float, C.c_float, _p(vp)]
+        L.wm_geom_load_env.argtypes = [vp, C.c_char_p, C.c_char_p, C.c_int, _p(vp)]
         L.wm_geom_destroy.argtypes = [vp]
         L.wm_geom_npls.argtypes = [vp]
         L.wm_geom_ld.argtypes = [vp]
@@ -332,17 +338,21 @@ class Geometry:
         self.ld = lib().wm_geom_ld(handle)
 
     @classmethod
-    def from_topology(cls, ctx: Context, t: Topology) -> "Geometry":
+    def from_topology(cls, ctx: Context, t: Topology, ienv: int = 1) -> "Geometry":
+        """ienv 1 free space, 2 PEC ground z = 0 (the reference's image signs)."""
         h = C.c_void_p()
-        _check(lib().wm_geom_create(ctx.h, _ptr(t.x, C.c_float), _ptr(t.y, C.c_float), _ptr(t.z, C.c_float),
-                                    _ptr(t.rad, C.c_float), _ptr(t.ipc, C.c_int), _ptr(t.ips, C.c_int),
-                                    _ptr(t.ise, C.c_int), t.nsmax, t.npls, C.byref(h)))
+        sx, sy, sz, se = ENV_SIGNS[ienv] if ienv in ENV_SIGNS else (1.0, 1.0, 1.0, 1.0)
+        _check(lib().wm_geom_create_env(ctx.h, _ptr(t.x, C.c_float), _ptr(t.y, C.c_float), _ptr(t.z, C.c_float),
+                                        _ptr(t.rad, C.c_float), _ptr(t.ipc, C.c_int), _ptr(t.ips, C.c_int),
+                                        _ptr(t.ise, C.c_int), t.nsmax, t.npls, int(ienv), sx, sy, sz, se,
+                                        C.byref(h)))
         return cls(ctx, h)
 
     @classmethod
-    def load(cls, ctx: Context, geo_path: str, wirefile: str | None = None) -> "Geometry":
+    def load(cls, ctx: Context, geo_path: str, wirefile: str | None = None, ienv: int = 1) -> "Geometry":
+        """Reads a .geo and segments it (configwire) for environment ienv."""
         h = C.c_void_p()
-        _check(lib().wm_geom_load(ctx.h, geo_path.encode(), (wirefile or "").encode(), C.byref(h)))
+        _check(lib().wm_geom_load_env(ctx.h, geo_path.encode(), (wirefile or "").encode(), int(ienv), C.byref(h)))
         return cls(ctx, h)
 
     def close(self):
@@ -426,13 +436,14 @@ def getrs_batched(ctx: Context, dev: DeviceArray, piv: DeviceArray, B: np.ndarra
 
 
 # ---------------------------------------------------------------------------- drop-in wrappers
-def zmatrix(t: Topology, ka: float) -> np.ndarray:
+def zmatrix(t: Topology, ka: float, ienv: int = 1) -> np.ndarray:
     """zmatrix with the reference's arguments; returns Z (npls x npls, numpy row/col indexing)."""
     n = t.npls
     cz = np.zeros((n, n), np.complex128)  # filled column-major: cz.T is Z
+    sx, sy, sz, se = ENV_SIGNS[ienv]
     lib().zmatrix(cz.ctypes.data, _ptr(t.x, C.c_float), _ptr(t.y, C.c_float), _ptr(t.z, C.c_float),
                   _ptr(t.ipc, C.c_int), _ptr(t.ips, C.c_int), _ptr(t.ise, C.c_int), _ptr(t.rad, C.c_float), n,
-                  float(ka), t.nsmax, 1, 1.0, 1.0, 1.0, 1.0)
+                  float(ka), t.nsmax, int(ienv), sx, sy, sz, se)
     return cz.T.copy()
 
 
@@ -452,15 +463,16 @@ def LU_cuBLAS(Z: np.ndarray, b: np.ndarray) -> np.ndarray:
 
 
 def CalculateByFreq(t: Topology, freq_mhz: float, ngap: int, ampl: float = 1.0, phas: float = 0.0,
-                    iload: int = 2, rank: int = 0):
+                    iload: int = 2, rank: int = 0, ienv: int = 1):
     """One frequency through the drop-in CalculateByFreq; returns (Zin, I, Z)."""
     n = t.npls
+    sx, sy, sz, se = ENV_SIGNS[ienv]
     cz = np.zeros((n, n), np.complex128)
     cu = np.zeros(n, np.complex128)
     ci = np.zeros(n, np.complex128)
     lib().CalculateByFreq(0, n, 1, float(freq_mhz), _ptr(t.x, C.c_float), _ptr(t.y, C.c_float),
                           _ptr(t.z, C.c_float), _ptr(t.rad, C.c_float), _ptr(t.ipc, C.c_int), _ptr(t.ips, C.c_int),
-                          _ptr(t.ise, C.c_int), t.nsmax, 1, 1.0, 1.0, 1.0, 1.0, iload, float(phas), int(ngap),
+                          _ptr(t.ise, C.c_int), t.nsmax, int(ienv), sx, sy, sz, se, iload, float(phas), int(ngap),
                           float(ampl), cz.ctypes.data, cu.ctypes.data, ci.ctypes.data, rank)
     return cu[ngap - 1] / ci[ngap - 1], ci, cz.T.copy()
 

@@ -380,15 +380,32 @@ int wm_memcpy_d2d(wm_ctx *c, void *dst, const void *src, size_t bytes)
 int wm_geom_create(wm_ctx *c, const float *x, const float *y, const float *z, const float *rad, const int *ipc,
                    const int *ips, const int *ise, int nsmax, int npls, wm_geom **out)
 {
+    return wm_geom_create_env(c, x, y, z, rad, ipc, ips, ise, nsmax, npls, 1, 1.f, 1.f, 1.f, 1.f, out);
+}
+
+int wm_geom_create_env(wm_ctx *c, const float *x, const float *y, const float *z, const float *rad, const int *ipc,
+                       const int *ips, const int *ise, int nsmax, int npls, int ienv, float sgnx, float sgny,
+                       float sgnz, float sgnenv, wm_geom **out)
+{
     if (!c || !x || !y || !z || !rad || !ipc || !ips || !ise || !out || npls <= 0 || nsmax < npls) {
         set_error("wm_geom_create: bad arguments");
+        return WM_ERR_ARG;
+    }
+    if (ienv != 1 && ienv != 2) {
+        set_error("wm_geom_create: ienv must be 1 (free space) or 2 (PEC ground), got %d", ienv);
         return WM_ERR_ARG;
     }
     std::lock_guard<std::recursive_mutex> lk(c->mu);
     (void)hipSetDevice(c->device);
     const int ld = round_up(npls, 128);
-    std::vector<double> obs((size_t)12 * ld, 0.0), src((size_t)16 * ld, 0.0);
-    std::vector<int> okey((size_t)2 * ld, -1), skey((size_t)2 * ld, -2);
+    std::vector<double> obs((size_t)12 * ld, 0.0), src((size_t)16 * ld, 0.0), img;
+    std::vector<int> okey((size_t)2 * ld, -1), skey((size_t)2 * ld, -2), ikey;
+    const bool pec = ienv == 2;
+    if (pec) {
+        img.assign((size_t)16 * ld, 0.0);
+        ikey.assign((size_t)2 * ld, -3);
+    }
+    const double sg[3] = {(double)sgnx, (double)sgny, (double)sgnz};
     std::map<long long, int> keys;  // unordered node pair -> segment key
     auto seg_key = [&](int s) {
         const int a = ise[s - 1], b = ise[s - 1 + nsmax];
@@ -439,11 +456,30 @@ int wm_geom_create(wm_ctx *c, const float *x, const float *y, const float *z, co
             r[8 * g + 3] = rdx / dl; r[8 * g + 4] = rdy / dl; r[8 * g + 5] = rdz / dl;
             r[8 * g + 6] = dl;
             r[8 * g + 7] = (double)rad[segs[g] - 1];
+            if (pec) {
+                // image half (cwire1_im / cwire2_im, "wersja referencyjna/Zmatrix.cpp":342-446,
+                // 559-663): coordinates scaled by the signs, so the start point and the unit
+                // vector flip in the mirrored axes and dl stays; the self rule applies to it
+                // when both nodes of its segment lie in the plane (Zmatrix.cpp:384)
+                double *q = &img[(size_t)p * 16];
+                for (int d = 0; d < 3; ++d) {
+                    q[8 * g + d] = sg[d] * r[8 * g + d];
+                    q[8 * g + 3 + d] = sg[d] * r[8 * g + 3 + d];
+                }
+                q[8 * g + 6] = r[8 * g + 6];
+                q[8 * g + 7] = r[8 * g + 7];
+                const int s = segs[g], na = ise[s - 1], nb = ise[s - 1 + nsmax];
+                if ((double)z[na - 1] < 1.e-6 && (double)z[nb - 1] < 1.e-6) ikey[2 * p + g] = skey[2 * p + g];
+            }
         }
     }
     for (int p = npls; p < ld; ++p) {  // padding: finite dummy data, overwritten by the identity
         double *r = &src[(size_t)p * 16];
         for (int g = 0; g < 2; ++g) { r[8 * g + 3] = 1.0; r[8 * g + 6] = 1.0; r[8 * g + 7] = 1.0; }
+        if (pec) {
+            double *q = &img[(size_t)p * 16];
+            for (int g = 0; g < 2; ++g) { q[8 * g + 3] = 1.0; q[8 * g + 6] = 1.0; q[8 * g + 7] = 1.0; }
+        }
         for (int d = 0; d < 12; ++d) obs[(size_t)d * ld + p] = 1.0e3;
     }
     auto *g = new wm_geom();
@@ -451,13 +487,21 @@ int wm_geom_create(wm_ctx *c, const float *x, const float *y, const float *z, co
     g->npls = npls;
     g->ld = ld;
     g->h_src = src;
+    g->ienv = ienv;
+    g->sgnenv = (double)sgnenv;
     if (hipMalloc(&g->d_obs, obs.size() * sizeof(double)) != hipSuccess ||
         hipMalloc(&g->d_src, src.size() * sizeof(double)) != hipSuccess ||
         hipMalloc(&g->d_obs_key, okey.size() * sizeof(int)) != hipSuccess ||
-        hipMalloc(&g->d_src_key, skey.size() * sizeof(int)) != hipSuccess) {
+        hipMalloc(&g->d_src_key, skey.size() * sizeof(int)) != hipSuccess ||
+        (pec && (hipMalloc(&g->d_img, img.size() * sizeof(double)) != hipSuccess ||
+                 hipMalloc(&g->d_img_key, ikey.size() * sizeof(int)) != hipSuccess))) {
         wm_geom_destroy(g);
         set_error("wm_geom_create: device allocation failed");
         return WM_ERR_NOMEM;
+    }
+    if (pec) {
+        WM_HIP_TRY(hipMemcpy(g->d_img, img.data(), img.size() * sizeof(double), hipMemcpyHostToDevice));
+        WM_HIP_TRY(hipMemcpy(g->d_img_key, ikey.data(), ikey.size() * sizeof(int), hipMemcpyHostToDevice));
     }
     WM_HIP_TRY(hipMemcpy(g->d_obs, obs.data(), obs.size() * sizeof(double), hipMemcpyHostToDevice));
     WM_HIP_TRY(hipMemcpy(g->d_src, src.data(), src.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -469,6 +513,15 @@ int wm_geom_create(wm_ctx *c, const float *x, const float *y, const float *z, co
 
 int wm_geom_load(wm_ctx *c, const char *geo_path, const char *wirefile, wm_geom **out)
 {
+    return wm_geom_load_env(c, geo_path, wirefile, 1, out);
+}
+
+int wm_geom_load_env(wm_ctx *c, const char *geo_path, const char *wirefile, int ienv, wm_geom **out)
+{
+    if (ienv != 1 && ienv != 2) {
+        set_error("wm_geom_load: ienv must be 1 (free space) or 2 (PEC ground), got %d", ienv);
+        return WM_ERR_ARG;
+    }
     GeoInput gi;
     int rc = geo_read(geo_path, gi);
     if (rc != WM_OK) {
@@ -478,7 +531,7 @@ int wm_geom_load(wm_ctx *c, const char *geo_path, const char *wirefile, wm_geom 
     Topology t;
     std::string text;
     const bool want = wirefile && wirefile[0];
-    rc = configwire_build(gi, 1, kNsmax, kNwmax, t, want ? &text : nullptr);
+    rc = configwire_build(gi, ienv, kNsmax, kNwmax, t, want ? &text : nullptr);
     if (rc != WM_OK) {
         set_error("wm_geom_load: segmentation of %s failed (capacity or topology)", geo_path);
         return rc;
@@ -489,8 +542,10 @@ int wm_geom_load(wm_ctx *c, const char *geo_path, const char *wirefile, wm_geom 
         std::fwrite(text.data(), 1, text.size(), f);
         std::fclose(f);
     }
-    return wm_geom_create(c, t.x.data(), t.y.data(), t.z.data(), t.rad.data(), t.ipc.data(), t.ips.data(),
-                          t.ise.data(), t.nsmax, t.npls, out);
+    // the reference's signs for its two environments ("wersja referencyjna/Wire_MoM.cu":107-122)
+    const float sz = ienv == 2 ? -1.f : 1.f;
+    return wm_geom_create_env(c, t.x.data(), t.y.data(), t.z.data(), t.rad.data(), t.ipc.data(), t.ips.data(),
+                              t.ise.data(), t.nsmax, t.npls, ienv, 1.f, 1.f, sz, sz, out);
 }
 
 int wm_geom_destroy(wm_geom *g)
@@ -500,6 +555,8 @@ int wm_geom_destroy(wm_geom *g)
     if (g->d_src) (void)hipFree(g->d_src);
     if (g->d_obs_key) (void)hipFree(g->d_obs_key);
     if (g->d_src_key) (void)hipFree(g->d_src_key);
+    if (g->d_img) (void)hipFree(g->d_img);
+    if (g->d_img_key) (void)hipFree(g->d_img_key);
     delete g;
     return WM_OK;
 }
@@ -808,17 +865,13 @@ void config(int *npnt, int nnod, float *xnod, float *ynod, float *znod, float *x
 }
 
 static int fill_to_host(wm_c128 *cz, float *x, float *y, float *z, float *rad, int *ipc, int *ips, int *ise,
-                        int nsmax, double ka, int npls, int ienv)
+                        int nsmax, double ka, int npls, int ienv, float sgnx, float sgny, float sgnz, float sgnenv)
 {
-    if (ienv != 1) {
-        set_error("only free space (ienv = 1) is available");
-        return WM_ERR_ARG;
-    }
     wm_ctx *c = default_ctx();
     if (!c) return WM_ERR_HIP;
     std::lock_guard<std::recursive_mutex> lk(c->mu);
     wm_geom *g = nullptr;
-    WM_TRY(wm_geom_create(c, x, y, z, rad, ipc, ips, ise, nsmax, npls, &g));
+    WM_TRY(wm_geom_create_env(c, x, y, z, rad, ipc, ips, ise, nsmax, npls, ienv, sgnx, sgny, sgnz, sgnenv, &g));
     std::unique_ptr<wm_geom, int (*)(wm_geom *)> guard(g, wm_geom_destroy);
     DevMem Zd;
     WM_TRY(Zd.alloc((size_t)g->ld * g->ld * sizeof(double2)));
@@ -830,16 +883,15 @@ static int fill_to_host(wm_c128 *cz, float *x, float *y, float *z, float *rad, i
 void zmatrix(wm_c128 *cz, float *x, float *y, float *z, int *ipc, int *ips, int *ise, float *rad, int npls, double ka,
              int nsmax, int ienv, float sgnx, float sgny, float sgnz, float sgnenv)
 {
-    (void)sgnx; (void)sgny; (void)sgnz; (void)sgnenv;
-    if (fill_to_host(cz, x, y, z, rad, ipc, ips, ise, nsmax, ka, npls, ienv) != WM_OK)
+    if (fill_to_host(cz, x, y, z, rad, ipc, ips, ise, nsmax, ka, npls, ienv, sgnx, sgny, sgnz, sgnenv) != WM_OK)
         std::fprintf(stderr, "zmatrix: %s\n", wm_last_error());
 }
 
 void zmatrix_gpu(wm_c128 *cz_h, float *x_h, float *y_h, float *z_h, float *rad_h, int *ipc_h, int *ips_h, int *ise_h,
                  int nsmax, double ka, int npls, int ienv, float sgnx, float sgny, float sgnz, float sgnenv)
 {
-    (void)sgnx; (void)sgny; (void)sgnz; (void)sgnenv;
-    if (fill_to_host(cz_h, x_h, y_h, z_h, rad_h, ipc_h, ips_h, ise_h, nsmax, ka, npls, ienv) != WM_OK)
+    if (fill_to_host(cz_h, x_h, y_h, z_h, rad_h, ipc_h, ips_h, ise_h, nsmax, ka, npls, ienv, sgnx, sgny, sgnz,
+                     sgnenv) != WM_OK)
         std::fprintf(stderr, "zmatrix_gpu: %s\n", wm_last_error());
 }
 

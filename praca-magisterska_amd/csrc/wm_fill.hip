@@ -19,6 +19,10 @@
 // column store is one coalesced 1 KiB run), each wave walks kColsPerWave
 // source bases n; n is wave-uniform, so the source records arrive through
 // scalar loads.  R and 1/R are shared by every frequency of the launch.
+//
+// PEC ground (ienv 2, "wersja referencyjna/Zmatrix.cpp":94-145, 174-224): every
+// source half is followed by its image in z = 0 (coordinates scaled by the
+// environment signs, weight sgnenv), which doubles the kernel evaluations.
 #include "wm_internal.h"
 
 namespace wm {
@@ -26,6 +30,11 @@ namespace wm {
 namespace {
 
 constexpr int kMaxF = 4;          // frequencies per launch
+// Frequencies per launch with the PEC image terms.  At four the image kernel
+// needs 256 VGPRs plus 98 AGPRs of spill space and gave wrong values in one
+// frequency slot on gfx950 (5.7e-4 relative, the others exact); at two it
+// fits in 213 VGPRs with no spill and matches the oracle.
+constexpr int kMaxFImg = 2;
 constexpr int kColsPerWave = 4;   // source bases per wave
 constexpr int kWaves = 4;         // waves per block
 constexpr double kPi = 3.14159265358979323846;
@@ -114,11 +123,22 @@ __global__ void k_self_terms(const double *__restrict__ src, int npls, int ld, i
     }
 }
 
-template <int F, bool kW32>
+// Image source halves (PEC ground, IMG): after each source half g its image
+// is taken with the same rule -- the self rule on the image of the
+// observation segment itself when that segment lies in the ground plane --
+// and added with weight sgnenv, in the reference's order ("wersja
+// referencyjna/Zmatrix.cpp":86-151, 167-231).
+struct ImgArgs {
+    const double *rec;  // image records, as src
+    const int *key;     // image self keys
+    double sgnenv;
+};
+
+template <int F, bool kW32, bool IMG>
 __global__ __launch_bounds__(256) void k_fill(const double *__restrict__ obs, const int *__restrict__ obs_key,
                                               const double *__restrict__ src, const int *__restrict__ src_key,
-                                              const double2 *__restrict__ self, int npls, int ld, FreqArgs fa,
-                                              double2 *__restrict__ Z, int64_t strideZ,
+                                              ImgArgs im, const double2 *__restrict__ self, int npls, int ld,
+                                              FreqArgs fa, double2 *__restrict__ Z, int64_t strideZ,
                                               float2 *__restrict__ Z32, int64_t strideZ32)
 {
     const int lane = threadIdx.x & 63;
@@ -137,7 +157,6 @@ __global__ __launch_bounds__(256) void k_fill(const double *__restrict__ obs, co
 
     for (int c = 0; c < kColsPerWave; ++c) {
         const int n = (blockIdx.y * kWaves + wave) * kColsPerWave + c;  // wave-uniform
-        const double *sr = src + (size_t)n * 16;
         double2 acc[F];
 #pragma unroll
         for (int f = 0; f < F; ++f) acc[f] = make_double2(0.0, 0.0);
@@ -149,42 +168,54 @@ __global__ __launch_bounds__(256) void k_fill(const double *__restrict__ obs, co
             for (int f = 0; f < F; ++f) { cv[f] = make_double2(0.0, 0.0); cs[f] = make_double2(0.0, 0.0); }
 #pragma unroll
             for (int g = 0; g < 2; ++g) {
-                const double rsx = sr[8 * g + 0], rsy = sr[8 * g + 1], rsz = sr[8 * g + 2];
-                const double dcx = sr[8 * g + 3], dcy = sr[8 * g + 4], dcz = sr[8 * g + 5];
-                const double dl = sr[8 * g + 6], an = sr[8 * g + 7];
-                const bool is_self = (okey[h] == src_key[2 * n + g]);
-                const double half = 0.5 * dl, off = kGL2 * dl;
-                const double bx = rc[h][0] - rsx, by = rc[h][1] - rsy, bz = rc[h][2] - rsz;
-                double2 V[F], S[F];
 #pragma unroll
-                for (int f = 0; f < F; ++f) { V[f] = make_double2(0.0, 0.0); S[f] = make_double2(0.0, 0.0); }
+                for (int pass = 0; pass < (IMG ? 2 : 1); ++pass) {
+                    const double *sr = (pass ? im.rec : src) + (size_t)n * 16;
+                    const int skey = pass ? im.key[2 * n + g] : src_key[2 * n + g];
+                    const double rsx = sr[8 * g + 0], rsy = sr[8 * g + 1], rsz = sr[8 * g + 2];
+                    const double dcx = sr[8 * g + 3], dcy = sr[8 * g + 4], dcz = sr[8 * g + 5];
+                    const double dl = sr[8 * g + 6], an = sr[8 * g + 7];
+                    const bool is_self = (okey[h] == skey);
+                    const double half = 0.5 * dl, off = kGL2 * dl;
+                    const double bx = rc[h][0] - rsx, by = rc[h][1] - rsy, bz = rc[h][2] - rsz;
+                    double2 V[F], S[F];
 #pragma unroll
-                for (int qp = 0; qp < 2; ++qp) {
-                    const double s = qp == 0 ? half + off : half - off;
-                    const double dx = bx - s * dcx, dy = by - s * dcy, dz = bz - s * dcz;
-                    const double R = sqrt(an * an + dx * dx + dy * dy + dz * dz);
-                    const double w = (g == 0 ? s : dl - s) / dl;
+                    for (int f = 0; f < F; ++f) { V[f] = make_double2(0.0, 0.0); S[f] = make_double2(0.0, 0.0); }
+#pragma unroll
+                    for (int qp = 0; qp < 2; ++qp) {
+                        const double s = qp == 0 ? half + off : half - off;
+                        const double dx = bx - s * dcx, dy = by - s * dcy, dz = bz - s * dcz;
+                        const double R = sqrt(an * an + dx * dx + dy * dy + dz * dz);
+                        const double w = (g == 0 ? s : dl - s) / dl;
+#pragma unroll
+                        for (int f = 0; f < F; ++f) {
+                            double sn, cn;
+                            sincos(fa.ka[f] * R, &sn, &cn);
+                            const double gr = cn / R, gi = -sn / R;
+                            V[f].x += w * gr; V[f].y += w * gi;
+                            S[f].x += gr;     S[f].y += gi;
+                        }
+                    }
+                    double t = dcx * lm[h][0] + dcy * lm[h][1] + dcz * lm[h][2];
+                    if (pass) t *= im.sgnenv;
 #pragma unroll
                     for (int f = 0; f < F; ++f) {
-                        double sn, cn;
-                        sincos(fa.ka[f] * R, &sn, &cn);
-                        const double gr = cn / R, gi = -sn / R;
-                        V[f].x += w * gr; V[f].y += w * gi;
-                        S[f].x += gr;     S[f].y += gi;
+                        double2 v = make_double2(V[f].x * half, V[f].y * half);
+                        double2 sc = make_double2(S[f].x * half, S[f].y * half);
+                        if (is_self) {
+                            v = self[((size_t)f * ld + n) * 4 + 2 * g];
+                            sc = self[((size_t)f * ld + n) * 4 + 2 * g + 1];
+                        }
+                        cv[f].x += v.x * t; cv[f].y += v.y * t;
+                        if (pass) {
+                            const double wq = im.sgnenv / dl;
+                            if (g == 0) { cs[f].x += sc.x * wq; cs[f].y += sc.y * wq; }
+                            else        { cs[f].x -= sc.x * wq; cs[f].y -= sc.y * wq; }
+                        } else {
+                            if (g == 0) { cs[f].x += sc.x / dl; cs[f].y += sc.y / dl; }
+                            else        { cs[f].x -= sc.x / dl; cs[f].y -= sc.y / dl; }
+                        }
                     }
-                }
-                const double t = dcx * lm[h][0] + dcy * lm[h][1] + dcz * lm[h][2];
-#pragma unroll
-                for (int f = 0; f < F; ++f) {
-                    double2 v = make_double2(V[f].x * half, V[f].y * half);
-                    double2 sc = make_double2(S[f].x * half, S[f].y * half);
-                    if (is_self) {
-                        v = self[((size_t)f * ld + n) * 4 + 2 * g];
-                        sc = self[((size_t)f * ld + n) * 4 + 2 * g + 1];
-                    }
-                    cv[f].x += v.x * t; cv[f].y += v.y * t;
-                    if (g == 0) { cs[f].x += sc.x / dl; cs[f].y += sc.y / dl; }
-                    else        { cs[f].x -= sc.x / dl; cs[f].y -= sc.y / dl; }
                 }
             }
 #pragma unroll
@@ -205,16 +236,30 @@ __global__ __launch_bounds__(256) void k_fill(const double *__restrict__ obs, co
     }
 }
 
+template <int F, bool IMG>
+void launch_fill_fi(dim3 grid, hipStream_t s, const wm_geom *g, const double2 *self, const FreqArgs &fa,
+                    double2 *Z, int64_t sZ, float2 *Z32, int64_t sZ32)
+{
+    const ImgArgs im{g->d_img, g->d_img_key, g->sgnenv};
+    if (Z32)
+        hipLaunchKernelGGL((k_fill<F, true, IMG>), grid, dim3(256), 0, s, g->d_obs, g->d_obs_key, g->d_src,
+                           g->d_src_key, im, self, g->npls, g->ld, fa, Z, sZ, Z32, sZ32);
+    else
+        hipLaunchKernelGGL((k_fill<F, false, IMG>), grid, dim3(256), 0, s, g->d_obs, g->d_obs_key, g->d_src,
+                           g->d_src_key, im, self, g->npls, g->ld, fa, Z, sZ, (float2 *)nullptr, (int64_t)0);
+}
+
 template <int F>
 void launch_fill_f(dim3 grid, hipStream_t s, const wm_geom *g, const double2 *self, const FreqArgs &fa,
                    double2 *Z, int64_t sZ, float2 *Z32, int64_t sZ32)
 {
-    if (Z32)
-        hipLaunchKernelGGL((k_fill<F, true>), grid, dim3(256), 0, s, g->d_obs, g->d_obs_key, g->d_src,
-                           g->d_src_key, self, g->npls, g->ld, fa, Z, sZ, Z32, sZ32);
-    else
-        hipLaunchKernelGGL((k_fill<F, false>), grid, dim3(256), 0, s, g->d_obs, g->d_obs_key, g->d_src,
-                           g->d_src_key, self, g->npls, g->ld, fa, Z, sZ, (float2 *)nullptr, (int64_t)0);
+    if constexpr (F <= kMaxFImg) {
+        if (g->ienv == 2) {
+            launch_fill_fi<F, true>(grid, s, g, self, fa, Z, sZ, Z32, sZ32);
+            return;
+        }
+    }
+    launch_fill_fi<F, false>(grid, s, g, self, fa, Z, sZ, Z32, sZ32);
 }
 
 }  // namespace
@@ -237,8 +282,9 @@ int fill_launch(wm_ctx *ctx, const wm_geom *g, int nf, const double *ka, void *Z
     WM_TRY(ws.ensure(self_bytes));
     double2 *self = (double2 *)ws.ptr;
     const dim3 grid(ld / 64, ld / (kWaves * kColsPerWave));
-    for (int f0 = 0; f0 < nf; f0 += kMaxF) {
-        const int F = (nf - f0 < kMaxF) ? nf - f0 : kMaxF;
+    const int fmax = g->ienv == 2 ? kMaxFImg : kMaxF;
+    for (int f0 = 0; f0 < nf; f0 += fmax) {
+        const int F = (nf - f0 < fmax) ? nf - f0 : fmax;
         FreqArgs fa;
         for (int f = 0; f < kMaxF; ++f) {
             const double k = ka[f0 + (f < F ? f : 0)];
@@ -250,8 +296,10 @@ int fill_launch(wm_ctx *ctx, const wm_geom *g, int nf, const double *ka, void *Z
                            fa, self);
         double2 *Zf = (double2 *)Z + (size_t)f0 * strideZ;
         float2 *Z32f = Z32 ? (float2 *)Z32 + (size_t)f0 * strideZ32 : nullptr;
-        // algorithmic count per frequency: 220 N^2 FP64 operations, 16 N^2 bytes written
+        // algorithmic count per frequency: 220 N^2 FP64 operations (440 N^2 with
+        // the image terms of the PEC ground), 16 N^2 bytes written
         const double nn = (double)g->npls * g->npls;
+        const double ops = g->ienv == 2 ? 440.0 : 220.0;
         prof_begin(ctx, K_FILL, s);
         switch (F) {
             case 1: launch_fill_f<1>(grid, s, g, self, fa, Zf, strideZ, Z32f, strideZ32); break;
@@ -259,7 +307,7 @@ int fill_launch(wm_ctx *ctx, const wm_geom *g, int nf, const double *ka, void *Z
             case 3: launch_fill_f<3>(grid, s, g, self, fa, Zf, strideZ, Z32f, strideZ32); break;
             default: launch_fill_f<4>(grid, s, g, self, fa, Zf, strideZ, Z32f, strideZ32); break;
         }
-        prof_end(ctx, K_FILL, s, 220.0 * nn * F, (Z32 ? 24.0 : 16.0) * nn * F);
+        prof_end(ctx, K_FILL, s, ops * nn * F, (Z32 ? 24.0 : 16.0) * nn * F);
         WM_HIP_TRY(hipGetLastError());
     }
     return WM_OK;

@@ -111,6 +111,14 @@ struct wm_geom {
     int *d_obs_key = nullptr;  // 2 arrays of ld ints: segment key of obs half 1, 2
     double *d_src = nullptr;   // ld records of 16 doubles: rs1[3] dc1[3] dl1 an1 rs2[3] dc2[3] dl2 an2
     int *d_src_key = nullptr;  // ld records of 2 ints
+    // PEC ground (ienv 2): image records (coordinates and directions scaled by
+    // sgnx, sgny, sgnz) laid out as d_src, image terms weighted by sgnenv; an
+    // image half's key is its segment key when the self rule applies to it
+    // (both nodes in the ground plane), else -3 (never an observation key)
+    int ienv = 1;
+    double sgnenv = 1.0;
+    double *d_img = nullptr;
+    int *d_img_key = nullptr;
     std::vector<double> h_src; // host copy of the source records (self-term precompute)
     // observed refinement contraction rates of mixed-precision solves on this
     // structure; see wm::predict_rate

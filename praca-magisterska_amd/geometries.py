@@ -17,6 +17,8 @@ Structures (sizes as used by the benchmarks and tests):
   dipole          straight wire along z (the reference's dipol.geo family)
   planar_grid     jittered planar lattice in z = 0, random segment lengths
   lightning       4 down-conductors, roof mesh at 1 m pitch, ground ring
+                  (depth=0 puts the ring on the PEC ground plane z = 0)
+  monopole        vertical wire standing on the ground plane
   cage            jittered cubic lattice of wires
 """
 from __future__ import annotations
@@ -52,6 +54,12 @@ def dipole(path: str, nseg: int, length: float = 25.0, radius: float = 0.0005) -
     wires = [(1, 2, radius, nseg)]
     write_geo(path, nodes, wires, "Dipole")
     return nseg - 1
+
+
+def monopole(path: str, nseg: int, height: float = 1.0, radius: float = 0.001) -> None:
+    """Vertical wire from the ground plane (0,0,0) to (0,0,height); with the PEC
+    ground (ienv 2) configwire adds the virtual ground wire at its base."""
+    write_geo(path, [(0.0, 0.0, 0.0), (0.0, 0.0, height)], [(1, 2, radius, nseg)], "Monopole")
 
 
 def _fit_nseg(lengths, target_sum, lo_len, hi_len, rng):
@@ -117,9 +125,12 @@ def planar_grid(path: str, n_target: int = 4096, seed: int = 2, size: float = 10
 
 
 def lightning(path: str, n_target: int = 8192, radius: float = 0.005, height: float = 20.0, depth: float = 0.5,
-              size: float = 10.0, pitch: float = 1.0) -> int:
+              size: float = 10.0, pitch: float = 1.0, ring: bool = True) -> int:
     """Lightning-protection structure: roof mesh at z=height, four corner down-conductors
-    to z=-depth and a ring joining their bottoms.  Down-conductor 1 starts at (0,0,height)."""
+    to z=-depth and (ring=True) a ring joining their bottoms.  Down-conductor 1 starts
+    at (0,0,height).  With depth=0 and ring=False the down-conductors end on the PEC
+    ground plane (grounded rods); a ring lying in the plane would be shorted by its
+    image (zero columns of Z under ienv 2)."""
     k = int(round(size / pitch)) + 1
     nodes = [(i * pitch, j * pitch, height) for j in range(k) for i in range(k)]
     corners = [1, k, k * (k - 1) + 1, k * k]
@@ -137,7 +148,8 @@ def lightning(path: str, n_target: int = 8192, radius: float = 0.005, height: fl
             if j + 1 < k:
                 edges.append((v, v + k))
     b0, b1, b2, b3 = bottoms  # ring: (0,0)-(10,0)-(10,10)-(0,10)
-    edges += [(b0, b1), (b1, b3), (b3, b2), (b2, b0)]
+    if ring:
+        edges += [(b0, b1), (b1, b3), (b3, b2), (b2, b0)]
     P = np.array(nodes, float)
     lens = np.array([np.linalg.norm(P[a - 1] - P[b - 1]) for a, b in edges])
     target_sum = n_target - len(edges) + len(nodes)

@@ -50,13 +50,20 @@ def _cmp_topology(t, to):
     assert (t.npnt, t.nseg, t.npls) == (to.npnt, to.nseg, to.npls)
 
 
-@pytest.mark.parametrize("which", ["dipole768", "grid", "lightning", "cage_small", "tjunction"])
+@pytest.mark.parametrize("which", ["dipole768", "grid", "lightning", "cage_small", "tjunction",
+                                   "monopole_pec", "lightning_pec"])
 def test_front_end_matches_oracle(wm, geometries, oracle, geo_dir, which):
     """configwire in libwiremom (product) and in the oracle agree bit for bit:
-    basis order, node ids and float32 coordinates, and configwire.txt text."""
+    basis order, node ids and float32 coordinates, and configwire.txt text;
+    the *_pec cases segment for the PEC ground (ienv 2, virtual ground wires)."""
     G = geometries
     p = os.path.join(geo_dir, f"fe_{which}.geo")
-    if which == "dipole768":
+    ienv = 2 if which.endswith("_pec") else 1
+    if which == "monopole_pec":
+        G.monopole(p, 40)
+    elif which == "lightning_pec":
+        G.lightning(p, 1024, depth=0.0)
+    elif which == "dipole768":
         G.dipole(p, 768)
     elif which == "grid":
         G.planar_grid(p, 1536, seed=11)
@@ -69,7 +76,9 @@ def test_front_end_matches_oracle(wm, geometries, oracle, geo_dir, which):
                     [(1, 2, 0.001, 10), (2, 3, 0.001, 10), (4, 2, 0.001, 10)])
     d = os.path.join(geo_dir, f"fe_{which}")
     os.makedirs(d, exist_ok=True)
-    t = wm.configwire(p, cwd=d)
-    to = oracle.load(p, wirefile=os.path.join(d, "oracle_cw.txt"))
+    t = wm.configwire(p, ienv=ienv, cwd=d)
+    to = oracle.load(p, ienv=ienv, wirefile=os.path.join(d, "oracle_cw.txt"))
     _cmp_topology(t, to)
     assert open(os.path.join(d, "configwire.txt"), "rb").read() == open(os.path.join(d, "oracle_cw.txt"), "rb").read()
+    if ienv == 2:  # the virtual ground wires add bases
+        assert t.npls > oracle.load(p, ienv=1).npls

@@ -53,6 +53,95 @@ def test_fill_matches_oracle(wm, geometries, oracle, ctx, geo_dir, case):
         assert err <= 1e-12, (case, f, err)
 
 
+def _pec_case(G, d, case):
+    """Structures on and above the PEC ground plane (ienv 2)."""
+    p = os.path.join(d, f"pec_{case}.geo")
+    if case == "monopole":
+        G.monopole(p, 200, 1.0, 0.001)
+    elif case == "lightning":  # ring lying in the ground plane: shorted, zero columns (fill only)
+        G.lightning(p, 1024, depth=0.0)
+    elif case == "rods":  # down-conductors ending on the ground plane
+        G.lightning(p, 1024, depth=0.0, ring=False)
+    else:  # bent wire 1-2.5 m above the ground, no ground contact
+        G.write_geo(p, [(0.0, 0.0, 1.0), (1.0, 0.0, 1.0), (1.0, 0.5, 2.0), (0.3, 0.8, 2.5)],
+                    [(1, 2, 0.002, 40), (2, 3, 0.002, 45), (3, 4, 0.0015, 35)])
+    return p
+
+
+@pytest.mark.parametrize("case", ["monopole", "lightning", "rods", "raised"])
+def test_pec_fill_matches_oracle(wm, geometries, oracle, ctx, geo_dir, case):
+    """PEC ground (ienv 2): fill with the image terms, the self rule on in-plane
+    segments and the virtual ground wires, against the oracle.  Five
+    frequencies in one request: every frequency slot of the image kernel's
+    launches (two frequencies per launch) is compared."""
+    p = _pec_case(geometries, geo_dir, case)
+    t = oracle.load(p, ienv=2)
+    g = wm.Geometry.load(ctx, p, ienv=2)
+    assert g.npls == t.npls
+    freqs = [40.0, 25.0, 10.0, 2.0, 0.5]
+    Zg = g.fill(freqs)
+    for i, f in enumerate(freqs):
+        Zo = oracle.zmatrix(t, oracle.wavenumber(f), ienv=2)
+        err = np.abs(Zg[i] - Zo).max() / np.abs(Zo).max()
+        print(case, f, err)
+        assert err <= 1e-12, (case, f, err)
+
+
+def test_pec_zin_matches_oracle_f64(wm, geometries, oracle, ctx, geo_dir):
+    """FP64 GPU path on the PEC ground vs the oracle: Zin within 1e-10; the
+    monopole's Zin is half the free-space dipole's (image theory)."""
+    for case, gap_at, freqs in [("monopole", (0, 0, 0), [50.0, 75.0, 120.0]),
+                                ("rods", (0, 0, 20), [0.5, 3.0, 25.0])]:
+        p = _pec_case(geometries, geo_dir, case)
+        t = oracle.load(p, ienv=2)
+        g = wm.Geometry.load(ctx, p, ienv=2)
+        gap = geometries.nearest_basis(t, gap_at)
+        zin, I, _ = wm.solve_sweep(g, freqs, [gap], "f64")
+        for i, f in enumerate(freqs):
+            zo, Io, _ = oracle.solve_freq(t, f, gaps=(gap,), ienv=2)
+            rel = abs(zin[i, 0] - zo[0]) / abs(zo[0])
+            print(case, f, zin[i, 0], rel)
+            assert rel <= 1e-10, (case, f, rel)
+            assert np.abs(I[i, 0] - Io[:, 0]).max() <= 1e-9 * np.abs(Io).max()
+    pd = os.path.join(geo_dir, "pec_dipole.geo")
+    geometries.write_geo(pd, [(0.0, 0.0, -1.0), (0.0, 0.0, 1.0)], [(1, 2, 0.001, 400)])
+    gd = wm.Geometry.load(ctx, pd)
+    zd, _, _ = wm.solve_sweep(gd, [50.0, 75.0, 120.0], [200], "f64")
+    gm = wm.Geometry.load(ctx, _pec_case(geometries, geo_dir, "monopole"), ienv=2)
+    zm, _, _ = wm.solve_sweep(gm, [50.0, 75.0, 120.0], [1], "f64")
+    assert (np.abs(zm[:, 0] - zd[:, 0] / 2) <= 2e-7 * np.abs(zd[:, 0])).all(), (zm, zd)
+
+
+@pytest.mark.parametrize("prec", ["f32", "bf16x3"])
+def test_pec_mixed_precision_matches_f64(wm, geometries, ctx, geo_dir, prec):
+    """Complex64 factors + FP64 refinement on a PEC-ground structure reach the
+    FP64 path's Zin within 1e-10 (or route to FP64 when they cannot)."""
+    p = _pec_case(geometries, geo_dir, "rods")
+    g = wm.Geometry.load(ctx, p, ienv=2)
+    f = [2.0, 10.0, 25.0, 40.0]
+    z64, _, _ = wm.solve_sweep(g, f, [1, 500], "f64")
+    zm, _, st = wm.solve_sweep(g, f, [1, 500], prec, size_policy=False)
+    print(st)
+    rel = np.abs(zm - z64).max(axis=1) / np.abs(z64).max(axis=1)
+    print(rel)
+    assert (rel <= 1e-10).all(), rel
+
+
+def test_pec_dropin_zmatrix(wm, geometries, oracle, ctx, geo_dir):
+    """The drop-in zmatrix with the reference's PEC arguments (ienv 2, signs
+    1, 1, -1, -1) and CalculateByFreq on the PEC ground."""
+    p = _pec_case(geometries, geo_dir, "monopole")
+    t = wm.configwire(p, ienv=2, cwd=geo_dir)
+    to = oracle.load(p, ienv=2)
+    ka = wm.wavenumber(75.0)
+    Z = wm.zmatrix(t, ka, ienv=2)
+    Zo = oracle.zmatrix(to, oracle.wavenumber(75.0), ienv=2)
+    assert np.abs(Z - Zo).max() <= 1e-12 * np.abs(Zo).max()
+    zin, _, _ = wm.CalculateByFreq(t, 75.0, 1, ienv=2)
+    zo, _, _ = oracle.solve_freq(to, 75.0, gaps=(1,), ienv=2)
+    assert abs(zin - zo[0]) <= 1e-10 * abs(zo[0])
+
+
 def test_fill_f32_copy_is_rounded_f64(wm, geometries, ctx, geo_dir):
     p = _dipole(geometries, geo_dir, 512, 1.0, 0.001)
     g = wm.Geometry.load(ctx, p)

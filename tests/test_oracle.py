@@ -189,3 +189,75 @@ def test_junction_connections_reduce_to_single_wire(oracle, geometries, geo_dir,
         assert abs(z1[0] - z2[0]) <= 1e-9 * abs(z1[0]), (kind, zf, z1, z2)
         a1, a2 = np.abs(I1[order1, 0]), np.abs(I2[order2, 0])
         assert np.abs(a1 - a2).max() <= 1e-9 * a1.max(), (kind, zf)
+
+
+def _raised_structure(d, G, mirrored=False):
+    """Bent three-wire structure 1-2.5 m above z = 0 (every direction
+    component non-zero somewhere); with mirrored=True the wires of its mirror
+    image in z = 0 follow, with the same node order."""
+    nodes = [(0.0, 0.0, 1.0), (1.0, 0.0, 1.0), (1.0, 0.5, 2.0), (0.3, 0.8, 2.5)]
+    wires = [(1, 2, 0.002, 8), (2, 3, 0.002, 9), (3, 4, 0.0015, 7)]
+    if mirrored:
+        nodes = nodes + [(x, y, -z) for x, y, z in nodes]
+        wires = wires + [(a + 4, b + 4, r, s) for a, b, r, s in wires]
+    p = os.path.join(d, f"raised_{int(mirrored)}.geo")
+    G.write_geo(p, nodes, wires)
+    return p
+
+
+def test_pec_ground_is_image_theory(oracle, geometries, geo_dir):
+    """PEC ground (ienv = 2) by images: for a structure above the ground, the
+    matrix equals the free-space matrix of the structure plus its mirror image
+    with the image currents taking the mirrored basis functions with weight
+    sgnenv = -1, i.e. Z_pec[m, n] = Zd[m, n] - Zd[m, n + N]."""
+    t = oracle.load(_raised_structure(geo_dir, geometries), ienv=2)
+    t2 = oracle.load(_raised_structure(geo_dir, geometries, True), ienv=1)
+    N = t.npls
+    assert t2.npls == 2 * N
+
+    def peak(tt, n):
+        i = abs(int(tt.ipc[n])) - 1
+        return float(tt.x[i]), float(tt.y[i]), float(tt.z[i])
+
+    for n in range(N):  # the doubled structure numbers the mirrored bases N..2N-1
+        x, y, z = peak(t, n)
+        assert peak(t2, n) == (x, y, z) and peak(t2, n + N) == (x, y, -z)
+    for f in (30.0, 150.0):
+        ka = oracle.wavenumber(f)
+        Zp = oracle.zmatrix(t, ka, ienv=2)
+        Zd = oracle.zmatrix(t2, ka, ienv=1)
+        scale = np.abs(Zd).max()
+        assert np.abs(Zp - (Zd[:N, :N] - Zd[:N, N:])).max() <= 1e-13 * scale
+        assert np.abs(Zp - Zd[:N, :N]).max() >= 1e-5 * scale  # the images matter
+
+
+def test_pec_wire_in_ground_plane_cancels(oracle, geometries, geo_dir):
+    """A wire lying in the ground plane is cancelled by its image: every term,
+    the self terms of segments in the plane included (Zmatrix.cpp:384), has an
+    equal and opposite image, so Z vanishes to rounding."""
+    p = os.path.join(geo_dir, "in_plane.geo")
+    geometries.write_geo(p, [(0.0, 0.0, 0.0), (2.0, 0.5, 0.0)], [(1, 2, 0.001, 16)])
+    t = oracle.load(p, ienv=2)
+    assert t.npls == 17  # 15 wire bases + one per grounded end over the virtual ground wires
+    ka = oracle.wavenumber(100.0)
+    Zf = oracle.zmatrix(t, ka, ienv=1)
+    Zp = oracle.zmatrix(t, ka, ienv=2)
+    assert np.abs(Zp).max() <= 1e-14 * np.abs(Zf).max()
+
+
+@pytest.mark.parametrize("nseg", [40, 80])
+def test_pec_monopole_is_half_dipole(oracle, geometries, geo_dir, nseg):
+    """Image theory for the whole path (front end with its virtual ground wire,
+    fill, LU): a vertical monopole of height h on the PEC ground, fed at its
+    base, has half the input impedance of the free-space dipole of length 2h
+    fed at its centre."""
+    h = 1.0
+    pm, pd = os.path.join(geo_dir, f"mono_{nseg}.geo"), os.path.join(geo_dir, f"dip_{nseg}.geo")
+    geometries.write_geo(pm, [(0.0, 0.0, 0.0), (0.0, 0.0, h)], [(1, 2, 0.001, nseg)])
+    geometries.write_geo(pd, [(0.0, 0.0, -h), (0.0, 0.0, h)], [(1, 2, 0.001, 2 * nseg)])
+    tm, td = oracle.load(pm, ienv=2), oracle.load(pd, ienv=1)
+    gm, gd = geometries.nearest_basis(tm, (0, 0, 0)), geometries.nearest_basis(td, (0, 0, 0))
+    for f in (50.0, 75.0, 120.0):
+        zm, _, _ = oracle.solve_freq(tm, f, gaps=(gm,), ienv=2)
+        zd, _, _ = oracle.solve_freq(td, f, gaps=(gd,))
+        assert abs(zm[0] - zd[0] / 2) <= 2e-7 * abs(zd[0]), (nseg, f, zm, zd)

@@ -136,6 +136,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-bf16x3", action="store_true")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch factorizations directly instead of replaying captured HIP graphs "
+                         "(for rocprofv3, whose tracing of graph capture faults on this ROCm)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run a plain device copy of MIB MiB (three times), the reference for HBM counters")
     ap.add_argument("--requests", type=int, default=48)
@@ -154,6 +157,8 @@ def main():
     G.lightning(geo, n_target)
     freqs = G.freqs_c3()[: a.nfreq]
     ctx = wm.Context(0)
+    if a.no_graphs:
+        ctx.set_graphs(False)
     name, ncu, hbm = ctx.info()
     g = wm.Geometry.load(ctx, geo, os.path.join(tmp, "configwire.txt"))
     t = wm.configwire(geo, cwd=tmp)
@@ -215,8 +220,13 @@ def main():
         ach = by / (ms * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": ach, "peak": peak, "unit": "GB/s", "frac": ach / peak}
     else:
+        # algorithmic flops (8 real per complex multiply-add); the complex128 and
+        # complex64 products issue 6 of them on the matrix cores (three real
+        # products per complex product), so the issued-MFMA utilisation is 3/4 of frac
         ach = fl / (ms * 1e-3) / 1e12
         roof = {"bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak}
+        if dom in ("gemm", "gemm_f32"):
+            roof["mfma_issued_frac"] = 0.75 * ach / peak
     traffic = GEMM_F64_HBM_RATIO * by / max(nl, 1) if dom == "gemm" else None
     roof.update({"kernel": dom, "launches": nl, "avg_launch_us": 1e3 * ms / max(nl, 1), "traffic": traffic,
                  "algorithmic_bytes_per_launch": by / max(nl, 1)})
@@ -224,14 +234,12 @@ def main():
         roof["traffic_source"] = "rocprofv3 FETCH_SIZE+WRITE_SIZE ratio, profiles/gemm_f64_traffic_c3_n8192.md"
 
     # whole-solve roofline fraction: t_ideal / t_measured per freq-solve, each
-    # frequency's LU priced at the peak of the precision its factors used, with
-    # the real flops of its products (3 real products per complex product for
-    # complex128/complex64 factors: 2 N^3; 4 for split-BF16x3: 8/3 N^3)
+    # frequency's LU (8/3 N^3 real flops, the complex getrf count) priced at the
+    # peak of the precision its factors used
     N = g.ld
     iters = float(np.mean([s["iters"] for s in st]))
     peak_of = {0: PEAK_TFLOPS["f64"], 1: PEAK_TFLOPS["f32"], 2: PEAK_TFLOPS["bf16x3"]}
-    lu_flops = {0: 2.0 * N ** 3, 1: 2.0 * N ** 3, 2: 8 / 3 * N ** 3}
-    t_lu = float(np.mean([lu_flops[s["prec"]] / (peak_of[s["prec"]] * 1e12) for s in st]))
+    t_lu = float(np.mean([8 / 3 * N ** 3 / (peak_of[s["prec"]] * 1e12) for s in st]))
     t_ideal = (t_lu + 220 * N ** 2 / (PEAK_FP64_VALU * 1e12)
                + (16 * N ** 2 + iters * 24 * N ** 2) / (PEAK_HBM_GBS * 1e9))
     solve_frac = t_ideal / (dt / (a.steps * len(freqs)))
@@ -276,6 +284,7 @@ def main():
         "cpu_baseline": cpu,
         "reference_best_published_solves_per_s": REF_BEST_PUBLISHED,
         "device": f"{name} x{ncu} CU",
+        "hip_graphs": not a.no_graphs,
     }
     if a.calibrate_copy:
         # plain device-to-device copies of known size (the runtime's copy kernel)

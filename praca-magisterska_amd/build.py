@@ -1,4 +1,4 @@
-"""Builds libwiremom.so (HIP for gfx950) in this directory.
+"""Builds libwiremom.so (HIP for gfx950) and the wire_mom driver in this directory.
 
     python praca-magisterska_amd/build.py [--jobs N]
 
@@ -19,6 +19,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libwiremom.so")
+DRIVER = os.path.join(HERE, "wire_mom")
 BUILD = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("WM_OFFLOAD_ARCH", "gfx950")
@@ -56,6 +57,14 @@ def build(jobs: int = 8) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed\n{' '.join(cmd)}\n{r.stderr}")
     os.replace(tmp, OUT)
+    # the reference program's driver: host C++ only, linked against the library beside it
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
+           os.path.join(CSRC, "wire_mom.cpp"), "-L", HERE, "-lwiremom", "-Wl,-rpath,$ORIGIN",
+           "-o", DRIVER + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"driver build failed\n{' '.join(cmd)}\n{r.stderr}")
+    os.replace(DRIVER + ".tmp", DRIVER)
     return OUT
 
 

@@ -1103,13 +1103,12 @@ int getrf_impl(wm_ctx *ctx, const GetrfPlan &pl, int n, typename CT<T>::V *A, in
     auto gemm = [&](const Grp &G, int r0, int c0, int mrows, int ncols, int kk, int K) {
         if (mrows <= 0 || ncols <= 0) return;
         const GemmArgs g{r0, c0, mrows, ncols, kk, K};
-        // C read and written, L and U read once; real flops of the products the
-        // kernel issues: 3 real products per complex product (complex128 and
-        // complex64), 4 x 3 bf16 products for split-BF16x3 (counted as 8)
-        const double fpc = gk == 2 ? 8.0 : 6.0;
+        // C read and written, L and U read once; algorithmic flops of a complex
+        // multiply-add (8 real).  The kernels issue 6 (three real products) for
+        // complex128 and complex64, and 4 x 3 bf16 products for split-BF16x3.
         prof_begin(ctx, kGemmClass, G.st);
         launch_gemm(gk, G.Ab, lda, strideA, g, G.m, G.bidx, G.st);
-        prof_end(ctx, kGemmClass, G.st, G.m * fpc * mrows * ncols * K,
+        prof_end(ctx, kGemmClass, G.st, G.m * 8.0 * mrows * ncols * K,
                  G.m * (2.0 * mrows * ncols + (double)(mrows + ncols) * K) * esz);
     };
 

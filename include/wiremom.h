@@ -193,6 +193,36 @@ int wm_solve_sweep(wm_ctx *ctx, const wm_geom *g, const wm_solve_opts *opts, int
                    const float *freq_mhz, int nrhs, const int *gaps, float ampl, float phas,
                    wm_c128 *I, wm_c128 *zin, wm_freq_stats *st);
 
+/* Lumped series load on basis function `basis` (1-based), added to Z's
+ * diagonal before the factorization: Z_L = r + j(2*pi*f*l - 1/(2*pi*f*c)) with
+ * f in MHz, l in uH and c in uF (c = 0: no capacitor) -- the units of the
+ * reference's hard-wired load block, "wersja referencyjna/Wire_MoM.cu":346-363
+ * (iload == 1: 1 + j*2*pi*f*4.5 on bases 592..2581, 30 on 2613, 1e12 on 2582,
+ * which is this list with l = 4.5, r = 30 and r = 1e12).  Loads on one basis
+ * are summed before they are added. */
+typedef struct {
+    int basis;
+    double r_ohm, l_uh, c_uf;
+} wm_load;
+
+/* Delta-gap generator: voltage ampl*exp(j*pi*phas/180) across basis `basis`
+ * (1-based) in right-hand side `rhs` (0-based).  Several generators may share
+ * a right-hand side (an array driven at several points); the reference has
+ * one generator ("wersja referencyjna/Wire_MoM.cu":183, 300-309). */
+typedef struct {
+    int basis, rhs;
+    float ampl, phas;
+} wm_source;
+
+/* wm_solve_sweep with general generators and lumped loads: nrhs right-hand
+ * sides built from nsrc generators, nload loads (may be 0 / NULL).
+ *   I   : complex128 [nfreq][nrhs][npls]
+ *   zin : complex128 [nfreq][nsrc]   V_s / I at generator s (its active impedance)
+ * Routing, batching and statistics as wm_solve_sweep. */
+int wm_solve_sweep_ex(wm_ctx *ctx, const wm_geom *g, const wm_solve_opts *opts, int nfreq,
+                      const float *freq_mhz, int nrhs, int nsrc, const wm_source *src, int nload,
+                      const wm_load *loads, wm_c128 *I, wm_c128 *zin, wm_freq_stats *st);
+
 /* ---- serving: continuous batching ------------------------------------------------ */
 typedef struct wm_server wm_server;
 

@@ -259,6 +259,43 @@ def excitation(npls: int, gaps, ampl: float = 1.0, phas: float = 0.0) -> np.ndar
     return B
 
 
+def load_impedance(freq_mhz: float, r_ohm: float, l_uh: float, c_uf: float) -> complex:
+    """Series load r + j(2 pi f l - 1/(2 pi f c)), f in MHz, l in uH, c in uF (c = 0: none);
+    the reference's hard-wired loads use 1 + j 2 pi f 4.5 ("wersja referencyjna/Wire_MoM.cu":349)."""
+    f = float(np.float32(freq_mhz))
+    x = 2.0 * np.pi * f * l_uh
+    if c_uf != 0.0:
+        x -= 1.0 / (2.0 * np.pi * f * c_uf)
+    return complex(r_ohm, x)
+
+
+def solve_freq_ex(t: Topology, freq_mhz: float, nrhs: int, sources, loads=(), ienv: int = 1):
+    """One frequency with generators (basis, rhs, ampl, phas) and series loads
+    (basis, r_ohm, l_uh, c_uf) added to Z's diagonal before the LU, as the
+    reference's load block does ("wersja referencyjna/Wire_MoM.cu":344-363).
+    Loads on one basis are summed first.  Returns (zin[nsrc] = V/I at each
+    generator, I[npls, nrhs])."""
+    ka = wavenumber(freq_mhz)
+    Z = zmatrix(t, ka, ienv=ienv)
+    per = {}
+    for b, r, l, c in loads:
+        per[int(b)] = per.get(int(b), 0j) + load_impedance(freq_mhz, r, l, c)
+    for b, zl in per.items():
+        Z[b - 1, b - 1] += zl
+    B = np.zeros((t.npls, nrhs), np.complex128, order="F")
+    vals = []
+    for b, r, a, p in sources:
+        v = excitation(1, [1], a, p)[0, 0]
+        B[int(b) - 1, int(r)] += v
+        vals.append(v)
+    LU, ipiv, info = zgetrf(Z)
+    if info:
+        raise np.linalg.LinAlgError(f"zgetrf info={info}")
+    X = zgetrs(LU, ipiv, B)
+    zin = np.array([cdiv(v, X[int(b) - 1, int(r)]) for v, (b, r, _, _) in zip(vals, sources)])
+    return zin, X
+
+
 def solve_freq(t: Topology, freq_mhz: float, gaps=(100,), ampl: float = 1.0, phas: float = 0.0,
                ienv: int = 1):
     """One frequency of the reference loop ("wersja referencyjna/Wire_MoM.cu":335-395).

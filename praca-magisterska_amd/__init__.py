@@ -35,7 +35,7 @@ EXPORTS = [
     "wm_prof_get", "wm_dev_alloc",
     "wm_dev_free", "wm_memcpy_h2d", "wm_memcpy_d2h", "wm_memcpy_d2d", "wm_geom_create", "wm_geom_create_env", "wm_geom_load", "wm_geom_load_env", "wm_geom_destroy",
     "wm_geom_npls", "wm_geom_ld", "wm_zfill", "wm_getrf_batched", "wm_getrs_batched", "wm_set_getrf_block", "wm_get_getrf_block", "wm_tune_getrf", "wm_set_graphs", "wm_solve_opts_default",
-    "wm_solve_sweep", "wm_server_opts_default", "wm_server_create", "wm_server_submit", "wm_server_step",
+    "wm_solve_sweep", "wm_solve_sweep_ex", "wm_server_opts_default", "wm_server_create", "wm_server_submit", "wm_server_step",
     "wm_server_result", "wm_server_status", "wm_server_done", "wm_server_destroy", "geoinp", "config", "configwire", "zmatrix",
     "zmatrix_gpu", "zmatrix_gpu_rank", "LU_cuBLAS", "CalculateByFreq",
 ]
@@ -63,6 +63,16 @@ def _stats(s) -> dict:
 class FreqStats(C.Structure):
     _fields_ = [("iters", C.c_int), ("info", C.c_int), ("fell_back", C.c_int), ("prec", C.c_int),
                 ("rate", C.c_double), ("dx_rel", C.c_double)]
+
+
+class Load(C.Structure):
+    """wm_load: series load r + j(2 pi f l - 1/(2 pi f c)) on a basis (f MHz, l uH, c uF)."""
+    _fields_ = [("basis", C.c_int), ("r_ohm", C.c_double), ("l_uh", C.c_double), ("c_uf", C.c_double)]
+
+
+class Source(C.Structure):
+    """wm_source: delta-gap generator ampl*exp(j pi phas/180) on a basis, in right-hand side rhs."""
+    _fields_ = [("basis", C.c_int), ("rhs", C.c_int), ("ampl", C.c_float), ("phas", C.c_float)]
 
 
 class ServerOpts(C.Structure):
@@ -118,6 +128,8 @@ def lib() -> C.CDLL:
         L.wm_solve_opts_default.restype = None
         L.wm_solve_sweep.argtypes = [vp, vp, _p(SolveOpts), C.c_int, fp, C.c_int, ip, C.c_float, C.c_float, vp, vp,
                                      _p(FreqStats)]
+        L.wm_solve_sweep_ex.argtypes = [vp, vp, _p(SolveOpts), C.c_int, fp, C.c_int, C.c_int, _p(Source), C.c_int,
+                                        _p(Load), vp, vp, _p(FreqStats)]
         L.wm_server_opts_default.argtypes = [_p(ServerOpts)]
         L.wm_server_opts_default.restype = None
         L.wm_server_create.argtypes = [vp, _p(ServerOpts), _p(vp)]
@@ -403,6 +415,30 @@ def solve_sweep(geom: Geometry, freqs_mhz, gaps, prec="f64", ampl: float = 1.0, 
     _check(rc)
     stats = [_stats(s) for s in st]
     return zin, I, stats
+
+
+def solve_sweep_ex(geom: Geometry, freqs_mhz, nrhs: int, sources, loads=(), prec="f64", batch: int = 0,
+                   max_iter: int = 20, tol: float = 1e-12, fallback: bool = True, size_policy: bool = True):
+    """Sweep with general generators and lumped loads (wm_solve_sweep_ex).
+
+    sources: (basis, rhs, ampl, phas) per generator; loads: (basis, r_ohm, l_uh, c_uf) per load.
+    Returns (zin[nf, nsrc] = V/I at each generator, I[nf, nrhs, N], stats list)."""
+    f = np.ascontiguousarray(np.atleast_1d(freqs_mhz), np.float32)
+    nf, n = len(f), geom.npls
+    src = (Source * len(sources))(*[Source(int(b), int(r), float(a), float(p)) for b, r, a, p in sources])
+    lds = (Load * max(1, len(loads)))(*[Load(int(b), float(r), float(l), float(c)) for b, r, l, c in loads])
+    o = SolveOpts()
+    lib().wm_solve_opts_default(C.byref(o))
+    o.prec = PREC_NAMES[prec] if isinstance(prec, str) else int(prec)
+    o.batch, o.max_iter, o.tol, o.fallback_f64 = batch, max_iter, tol, int(fallback)
+    o.size_policy = int(size_policy)
+    I = np.zeros((nf, nrhs, n), np.complex128)
+    zin = np.zeros((nf, len(sources)), np.complex128)
+    st = (FreqStats * nf)()
+    rc = lib().wm_solve_sweep_ex(geom.ctx.h, geom.h, C.byref(o), nf, _ptr(f, C.c_float), int(nrhs), len(sources),
+                                 src, len(loads), lds, I.ctypes.data, zin.ctypes.data, st)
+    _check(rc)
+    return zin, I, [_stats(s) for s in st]
 
 
 def getrf_batched(ctx: Context, A: np.ndarray, prec="f64"):

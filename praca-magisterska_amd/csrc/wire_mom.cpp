@@ -17,13 +17,18 @@
 //   -a V      source magnitude [V]            default 1
 //   -p DEG    source phase [degrees]          default 0
 //   -l N      lumped loadings: 1 the reference's hard-wired loads, 2 none  default 2
+//   -L FILE   lumped series loads, one per line: basis R[ohm] L[uH] C[uF] (C = 0: none);
+//             the sweep adds them to Z's diagonal on the GPU
+//   -S FILE   generators, one per line: basis magnitude[V] phase[deg], all driving one
+//             excitation; the printed Zin is V/I at the first one (replaces -s/-a/-p)
 //   -P PREC   factor precision of the sweep: f64 (drop-in), f32, bf16x3   default f64
 //   -r N      process number printed in the lines                     default 0
 //   -t        write "GPU time.dat" (three lines of ms, as zmatrix_gpu does)
 //
-// Without loads the whole frequency list is one batched sweep on the GPU
-// (wm_solve_sweep); with loads each frequency goes through the drop-in
-// CalculateByFreq, which adds them to Z before the LU.
+// Without the reference's hard-wired loads the whole frequency list is one
+// batched sweep on the GPU (wm_solve_sweep, or wm_solve_sweep_ex with -L/-S);
+// with them (-l 1) each frequency goes through the drop-in CalculateByFreq,
+// which adds them to Z before the LU.
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -32,6 +37,7 @@
 #include <fstream>
 #include <iomanip>
 #include <iostream>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -42,7 +48,7 @@ namespace {
 constexpr int kNnmax = 3000, kNtmax = 5000, kNwmax = 10000, kNpwmax = 20480, kNsmax = kNwmax + kNpwmax;
 
 struct Opts {
-    std::string geo = "dipol.geo", freq = "freq.txt";
+    std::string geo = "dipol.geo", freq = "freq.txt", loads, sources;
     int ienv = 1, ngap = 100, iload = 2, rank = 0, prec = WM_PREC_F64;
     float ampl = 1.f, phas = 0.f;
     bool timefile = false;
@@ -52,7 +58,7 @@ struct Opts {
 {
     std::fprintf(stderr,
                  "usage: %s [-g geo] [-f freq.txt] [-e 1|2] [-s source] [-a volts] [-p degrees] [-l 1|2]\n"
-                 "          [-P f64|f32|bf16x3] [-r rank] [-t]\n",
+                 "          [-L loads.txt] [-S sources.txt] [-P f64|f32|bf16x3] [-r rank] [-t]\n",
                  argv0);
     std::exit(2);
 }
@@ -75,6 +81,8 @@ Opts parse(int argc, char **argv)
         else if (a == "-l") o.iload = std::atoi(val());
         else if (a == "-r") o.rank = std::atoi(val());
         else if (a == "-t") o.timefile = true;
+        else if (a == "-L") o.loads = val();
+        else if (a == "-S") o.sources = val();
         else if (a == "-P") {
             const std::string p = val();
             if (p == "f64") o.prec = WM_PREC_F64;
@@ -84,7 +92,37 @@ Opts parse(int argc, char **argv)
         } else usage(argv[0]);
     }
     if ((o.ienv != 1 && o.ienv != 2) || (o.iload != 1 && o.iload != 2)) usage(argv[0]);
+    if (o.iload == 1 && (!o.loads.empty() || !o.sources.empty())) usage(argv[0]);
     return o;
+}
+
+// Whitespace-separated numbers per line; '#' starts a comment.  Returns false
+// (with a message) if a line does not hold exactly `cols` numbers.
+bool read_table(const std::string &path, int cols, std::vector<std::vector<double>> &rows)
+{
+    std::ifstream f(path);
+    if (!f) {
+        std::fprintf(stderr, "wire_mom: cannot open %s\n", path.c_str());
+        return false;
+    }
+    std::string line;
+    int ln = 0;
+    while (std::getline(f, line)) {
+        ++ln;
+        const size_t h = line.find('#');
+        if (h != std::string::npos) line.resize(h);
+        std::istringstream is(line);
+        std::vector<double> v;
+        double x;
+        while (is >> x) v.push_back(x);
+        if (v.empty()) continue;
+        if ((int)v.size() != cols || !is.eof()) {
+            std::fprintf(stderr, "wire_mom: %s:%d: expected %d numbers\n", path.c_str(), ln, cols);
+            return false;
+        }
+        rows.push_back(v);
+    }
+    return true;
 }
 
 // Prints " Process - <rank> <info> took <s>s" when it goes out of scope, like
@@ -164,7 +202,7 @@ int main(int argc, char **argv)
               << " Select 1 or 2: " << 1 << std::endl
               << " ------------------------------" << std::endl
               << " Source (source zone on wire): " << o.ngap << std::endl;
-    if (o.ngap <= 0 || o.ngap > npls) {
+    if (o.sources.empty() && (o.ngap <= 0 || o.ngap > npls)) {
         std::cout << "    ERROR (wrong number of source zone)" << std::endl;
         return 1;
     }
@@ -194,6 +232,26 @@ int main(int argc, char **argv)
               << "  Number of frequency points: " << nfreq << std::endl
               << "  -------------------------------------" << std::endl;
 
+    // generators and loads files, read before the GPU is opened
+    std::vector<wm_source> src;
+    std::vector<wm_load> lds;
+    if (!o.sources.empty()) {
+        std::vector<std::vector<double>> rows;
+        if (!read_table(o.sources, 3, rows)) return 1;
+        if (rows.empty()) {
+            std::fprintf(stderr, "wire_mom: %s holds no generator\n", o.sources.c_str());
+            return 1;
+        }
+        for (const auto &r : rows) src.push_back(wm_source{(int)r[0], 0, (float)r[1], (float)r[2]});
+    } else {
+        src.push_back(wm_source{o.ngap, 0, ampl, o.phas});
+    }
+    if (!o.loads.empty()) {
+        std::vector<std::vector<double>> rows;
+        if (!read_table(o.loads, 4, rows)) return 1;
+        for (const auto &r : rows) lds.push_back(wm_load{(int)r[0], r[1], r[2], r[3]});
+    }
+
     std::vector<wm_c128> zin(nfreq);
     double t_kernels_ms = 0.0, t_call_ms = 0.0;
     if (o.iload == 1) {
@@ -219,14 +277,20 @@ int main(int argc, char **argv)
         wm_solve_opts_default(&so);
         so.prec = o.prec;
         std::vector<wm_freq_stats> st(nfreq);
+        std::vector<wm_c128> zall((size_t)nfreq * src.size());
         wm_prof_enable(ctx, 1);
         {
             Timer t(o.rank, "GPU sweep (fill + LU + solve)");
-            if (wm_solve_sweep(ctx, g, &so, nfreq, freqs.data(), 1, &o.ngap, ampl, o.phas, nullptr, zin.data(),
-                               st.data()) != WM_OK)
-                return fail("solving the sweep");
+            const int rc = (o.loads.empty() && o.sources.empty())
+                               ? wm_solve_sweep(ctx, g, &so, nfreq, freqs.data(), 1, &o.ngap, ampl, o.phas, nullptr,
+                                                zall.data(), st.data())
+                               : wm_solve_sweep_ex(ctx, g, &so, nfreq, freqs.data(), 1, (int)src.size(), src.data(),
+                                                   (int)lds.size(), lds.empty() ? nullptr : lds.data(), nullptr,
+                                                   zall.data(), st.data());
+            if (rc != WM_OK) return fail("solving the sweep");
             t_call_ms = 1e3 * t.seconds();
         }
+        for (int i = 0; i < nfreq; ++i) zin[i] = zall[(size_t)i * src.size()];
         static const char *classes[] = {"fill", "panel", "swap_trsm", "gemm", "panel_f32", "swap_trsm_f32",
                                         "gemm_f32", "gemm_bf16x3", "trsv", "resid"};
         for (const char *c : classes) {

@@ -730,6 +730,66 @@ void wm_solve_opts_default(wm_solve_opts *o)
 
 extern "C" {
 
+namespace {
+
+// Solves the items of one geometry (one per frequency) in batches: FP64 sweeps
+// as wide as the HBM budget allows, mixed sweeps highest frequency first with
+// routing (see below).
+int run_sweep(wm_ctx *c, const wm_geom *g, const wm_solve_opts &o, std::vector<WorkItem> &items, int nrhs)
+{
+    const int nfreq = (int)items.size();
+    int batch = o.batch > 0 ? o.batch : auto_batch(c, g->ld, nrhs, o.prec, o.hbm_fraction);
+    int status = WM_OK;
+    const bool mixed = o.prec != WM_PREC_F64;
+    if (mixed) {
+        // Highest frequencies first: their refinement rates, recorded on the
+        // geometry (cleared here), route the lower ones, whose conditioning is
+        // worse, to FP64 factors when refinement cannot reach tol in max_iter.
+        std::stable_sort(items.begin(), items.end(), [](const WorkItem &a, const WorkItem &b) { return a.freq > b.freq; });
+        g->rate_obs.clear();
+    }
+    int f0 = 0;
+    while (f0 < nfreq) {
+        int cnt = std::min(batch, nfreq - f0);
+        if (mixed) {
+            // Mixed sweeps go in batches at most one factorization launch wide,
+            // so that each batch is routed with the rates of the frequencies just
+            // above it, and cut where the routed precision changes: a run of
+            // frequencies that all route to FP64 factors (the lower ones once
+            // refinement stops converging) goes as one batch, whose members
+            // share the panel launches.
+            const int w = std::min(cnt, getrf_width(c, WM_PREC_F32, g->ld));
+            if (o.size_policy && prefer_f64(c, o.prec, g->ld, w)) {
+                for (int b = f0; b < f0 + w; ++b) items[b].fprec = WM_PREC_F64;
+                cnt = w;
+            } else if (f0 > 0) {
+                route_items(o, items.data() + f0, cnt);
+                const bool f64 = items[f0].fprec == WM_PREC_F64;
+                int run = 1;
+                while (run < (f64 ? cnt : w) && (items[f0 + run].fprec == WM_PREC_F64) == f64) ++run;
+                // a short complex64 run just above the FP64 ones joins their batch
+                // when its own panel launches and refinement would cost more
+                if (!f64 && run < cnt && items[f0 + run].fprec == WM_PREC_F64 && o.size_policy &&
+                    join_f64(c, o.prec, g->ld, run)) {
+                    for (int b = f0; b < f0 + run; ++b) items[b].fprec = WM_PREC_F64;
+                    while (run < cnt && items[f0 + run].fprec == WM_PREC_F64) ++run;
+                }
+                cnt = run;
+            } else {
+                cnt = w;
+            }
+        }
+        const int rc = solve_items(c, o, items.data() + f0, cnt, c->stream);
+        if (rc < 0) return rc;
+        if (rc > 0 && status == WM_OK) status = rc;
+        if (mixed) record_rates(items.data() + f0, cnt);
+        f0 += cnt;
+    }
+    return status;
+}
+
+}  // namespace
+
 int wm_solve_sweep(wm_ctx *c, const wm_geom *g, const wm_solve_opts *opts, int nfreq, const float *freq_mhz, int nrhs,
                    const int *gaps, float ampl, float phas, wm_c128 *I, wm_c128 *zin, wm_freq_stats *st)
 {
@@ -756,34 +816,38 @@ int wm_solve_sweep(wm_ctx *c, const wm_geom *g, const wm_solve_opts *opts, int n
         w.zin = zin ? zin + (size_t)f * nrhs : nullptr;
         w.st = st ? st + f : nullptr;
     }
-    int batch = o.batch > 0 ? o.batch : auto_batch(c, g->ld, nrhs, o.prec, o.hbm_fraction);
-    int status = WM_OK;
-    const bool mixed = o.prec != WM_PREC_F64;
-    if (mixed) {
-        // Highest frequencies first: their refinement rates, recorded on the
-        // geometry (cleared here), route the lower ones, whose conditioning is
-        // worse, to FP64 factors when refinement cannot reach tol in max_iter.
-        std::stable_sort(items.begin(), items.end(), [](const WorkItem &a, const WorkItem &b) { return a.freq > b.freq; });
-        g->rate_obs.clear();
+    return run_sweep(c, g, o, items, nrhs);
+}
+
+int wm_solve_sweep_ex(wm_ctx *c, const wm_geom *g, const wm_solve_opts *opts, int nfreq, const float *freq_mhz,
+                      int nrhs, int nsrc, const wm_source *src, int nload, const wm_load *loads, wm_c128 *I,
+                      wm_c128 *zin, wm_freq_stats *st)
+{
+    if (!c || !g || nfreq <= 0 || !freq_mhz || nrhs <= 0 || nsrc <= 0 || !src || nload < 0 || (nload > 0 && !loads)) {
+        set_error("wm_solve_sweep_ex: bad arguments");
+        return WM_ERR_ARG;
     }
-    int f0 = 0;
-    while (f0 < nfreq) {
-        int cnt = std::min(batch, nfreq - f0);
-        // mixed sweeps go in batches one factorization launch wide, so that each
-        // batch is routed with the rates of the frequencies just above it
-        if (mixed) cnt = std::min(cnt, getrf_width(c, WM_PREC_F32, g->ld));
-        if (mixed && o.size_policy && prefer_f64(c, o.prec, g->ld, cnt)) {
-            for (int b = f0; b < f0 + cnt; ++b) items[b].fprec = WM_PREC_F64;
-        } else if (mixed && f0 > 0) {
-            route_items(o, items.data() + f0, cnt);
-        }
-        const int rc = solve_items(c, o, items.data() + f0, cnt, c->stream);
-        if (rc < 0) return rc;
-        if (rc > 0 && status == WM_OK) status = rc;
-        if (mixed) record_rates(items.data() + f0, cnt);
-        f0 += cnt;
+    wm_solve_opts o;
+    wm_solve_opts_default(&o);
+    if (opts) o = *opts;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    (void)hipSetDevice(c->device);
+    const int N = g->npls;
+    std::vector<WorkItem> items(nfreq);
+    for (int f = 0; f < nfreq; ++f) {
+        WorkItem &w = items[f];
+        w.g = g;
+        w.freq = freq_mhz[f];
+        w.nrhs = nrhs;
+        w.src = src;
+        w.nsrc = nsrc;
+        w.loads = loads;
+        w.nload = nload;
+        w.I = I ? I + (size_t)f * nrhs * N : nullptr;
+        w.zin = zin ? zin + (size_t)f * nsrc : nullptr;
+        w.st = st ? st + f : nullptr;
     }
-    return status;
+    return run_sweep(c, g, o, items, nrhs);
 }
 
 // ---- drop-in entry points ------------------------------------------------------------------

@@ -38,14 +38,48 @@ __global__ void k_zero2(double2 *p, int64_t n)
     if (i < n) p[i] = make_double2(0.0, 0.0);
 }
 
-// Delta-gap right-hand sides: entries (index, value) per (item, rhs).
-__global__ void k_set_rhs(double2 *B, int64_t strideB, int ldb, int nr, const int *idx, const double2 *val, int count)
+// Delta-gap right-hand sides: up to ne entries (column, index, value) per
+// item, index -1 for none; entries of one item name distinct positions.
+__global__ void k_set_rhs(double2 *B, int64_t strideB, int ldb, int ne, const int *col, const int *idx,
+                          const double2 *val, int count)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= count * nr) return;
-    const int b = t / nr, r = t % nr;
+    if (t >= count * ne) return;
+    const int b = t / ne;
     const int i = idx[t];
-    if (i >= 0) B[(size_t)b * strideB + (size_t)r * ldb + i] = val[t];
+    if (i >= 0) B[(size_t)b * strideB + (size_t)col[t] * ldb + i] = val[t];
+}
+
+// Lumped loads: Z[b][i][i] += val for up to nl entries (index, value) per
+// item (index -1 for none, distinct indices per item); the complex64 copy,
+// if any, takes the rounded new diagonal.
+__global__ void k_add_diag(double2 *Z, int64_t strideZ, int ld, float2 *Z32, const int *idx, const double2 *val,
+                           int nl, int count)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count * nl) return;
+    const int b = t / nl, i = idx[t];
+    if (i < 0) return;
+    const size_t o = (size_t)b * strideZ + (size_t)i * ld + i;
+    double2 z = Z[o];
+    z.x += val[t].x;
+    z.y += val[t].y;
+    Z[o] = z;
+    if (Z32) Z32[o] = make_float2((float)z.x, (float)z.y);
+}
+
+double2 source_value(float ampl, float phas)  // ampl*exp(j*pi*phas/180), "wersja referencyjna/Wire_MoM.cu":307-309
+{
+    const double arg = kPi * (double)phas / 180.;
+    return make_double2((double)ampl * std::cos(arg), (double)ampl * std::sin(arg));
+}
+
+// Series load impedance at f MHz: r + j(2 pi f l - 1/(2 pi f c)), l in uH, c in uF.
+double2 load_value(const wm_load &l, float f)
+{
+    double x = 2. * kPi * (double)f * l.l_uh;
+    if (l.c_uf != 0.0) x -= 1.0 / (2. * kPi * (double)f * l.c_uf);
+    return make_double2(l.r_ohm, x);
 }
 
 // X += D for the listed items; max |D|, |X| per (item, rhs) as ordered bit patterns.
@@ -189,22 +223,51 @@ void record_rates(const WorkItem *items, int count)
     }
 }
 
+// Panel launches of a lockstep factorization of `count` matrices of size n
+// when `width` full-height panels fit in one launch: the panel at row s*64
+// (height n - 64 s) goes in ceil(count (n - 64 s) / (width n)) launches.
+double panel_launches(int n, int count, int width)
+{
+    double t = 0.0;
+    for (int h = n; h > 0; h -= 64) t += std::ceil((double)count * h / ((double)width * n));
+    return t;
+}
+
 bool prefer_f64(wm_ctx *c, int prec, int n, int count)
 {
     if (prec == WM_PREC_F64 || count <= 0) return false;
-    // Measured on MI355X: a panel column costs about 7.2 us (complex128) and
-    // 8.5 us (complex64) whatever the cluster size; trailing updates run at
-    // about 72 TF/s (FP64), 108 TF/s (FP32) and 190 TF/s FP32-equivalent
-    // (split-BF16x3), all counted at 8 real flops per complex product; a
-    // refinement iteration costs about 0.1 us per unknown (triangular-solve
-    // latency chain) plus its 24 N^2 bytes; about 6 of them.
+    // Measured on MI355X: a panel launch costs about 64 columns x 7.2 us
+    // (complex128) and x 8.5 us (complex64) whatever the cluster size; trailing
+    // updates run at about 72 TF/s (FP64), 108 TF/s (FP32) and 190 TF/s
+    // FP32-equivalent (split-BF16x3), all counted at 8 real flops per complex
+    // product; a refinement iteration costs about 0.1 us per unknown
+    // (triangular-solve latency chain) plus its 24 N^2 bytes; about 6 of them.
     const double nn = (double)n, flu = 8.0 / 3.0 * nn * nn * nn;
     const int w64 = getrf_width(c, WM_PREC_F64, n), w32 = getrf_width(c, prec, n);
-    const double t64 = nn * 7.2e-6 * ((count + w64 - 1) / w64) + count * flu / 72e12;
+    const double t64 = 64 * 7.2e-6 * panel_launches(n, count, w64) + count * flu / 72e12;
     const double rate = prec == WM_PREC_BF16X3 ? 190e12 : 108e12;
-    const double tmix = nn * 8.5e-6 * ((count + w32 - 1) / w32) +
+    const double tmix = 64 * 8.5e-6 * panel_launches(n, count, w32) +
                         count * (flu / rate + 6.0 * (1e-7 * nn + 24.0 * nn * nn / 4e12));
     return t64 <= tmix;
+}
+
+bool join_f64(wm_ctx *c, int prec, int n, int r)
+{
+    if (prec == WM_PREC_F64 || r <= 0) return false;
+    // Measured on MI355X at N = 8192: complex128 members of a large lockstep
+    // batch cost their product (78 TF/s) plus a share of the panel launches
+    // (n x 7.2 us / (2 x launch width)); a separate complex64 batch pays its
+    // own panel launches (64 columns x 8.5 us each), its products (130 TF/s
+    // FP32, 190 TF/s split-BF16x3) and about 6 refinement iterations of two
+    // triangular-solve latency chains (n/64 blocks x 8 us) plus 24 N^2 bytes
+    // per member.
+    const double nn = (double)n, flu = 8.0 / 3.0 * nn * nn * nn;
+    const int w64 = getrf_width(c, WM_PREC_F64, n), w32 = getrf_width(c, prec, n);
+    const double t_join = r * (flu / 78e12 + nn * 7.2e-6 / (2.0 * w64));
+    const double rate = prec == WM_PREC_BF16X3 ? 190e12 : 130e12;
+    const double t_sep = 64 * 8.5e-6 * panel_launches(n, r, w32) + r * flu / rate +
+                         6.0 * (2.0 * nn / 64 * 8e-6 + r * 24.0 * nn * nn / 5e12);
+    return t_join <= t_sep;
 }
 
 void route_items(const wm_solve_opts &o, WorkItem *items, int count)
@@ -233,16 +296,62 @@ int solve_items(wm_ctx *c, const wm_solve_opts &o, WorkItem *items, int count, h
     const int n = items[0].g->ld;
     int nr = 0;
     for (int b = 0; b < count; ++b) {
-        if (items[b].g->ld != n) {
+        const WorkItem &w = items[b];
+        const int np = w.g->npls;
+        if (w.g->ld != n) {
             set_error("solve_items: items of different sizes in one batch");
             return WM_ERR_ARG;
         }
-        for (int r = 0; r < items[b].nrhs; ++r)
-            if (items[b].gaps[r] < 1 || items[b].gaps[r] > items[b].g->npls) {
-                set_error("feed basis %d outside 1..%d", items[b].gaps[r], items[b].g->npls);
+        if (w.src) {
+            for (int q = 0; q < w.nsrc; ++q)
+                if (w.src[q].basis < 1 || w.src[q].basis > np || w.src[q].rhs < 0 || w.src[q].rhs >= w.nrhs) {
+                    set_error("generator %d: basis %d outside 1..%d or right-hand side %d outside 0..%d", q,
+                              w.src[q].basis, np, w.src[q].rhs, w.nrhs - 1);
+                    return WM_ERR_ARG;
+                }
+        } else {
+            for (int r = 0; r < w.nrhs; ++r)
+                if (w.gaps[r] < 1 || w.gaps[r] > np) {
+                    set_error("feed basis %d outside 1..%d", w.gaps[r], np);
+                    return WM_ERR_ARG;
+                }
+        }
+        for (int q = 0; q < w.nload; ++q)
+            if (w.loads[q].basis < 1 || w.loads[q].basis > np) {
+                set_error("load %d on basis %d outside 1..%d", q, w.loads[q].basis, np);
                 return WM_ERR_ARG;
             }
-        nr = std::max(nr, items[b].nrhs);
+        nr = std::max(nr, w.nrhs);
+    }
+    // Right-hand-side entries per item (generators on one basis and column
+    // summed), the ports whose V/I is reported, and the loads per basis (summed).
+    struct Port {
+        int col, idx;
+        double2 v;
+    };
+    std::vector<std::vector<Port>> ent(count), ports(count), lds(count);
+    auto merge = [](std::vector<Port> &v, const Port &p) {
+        for (auto &e : v)
+            if (e.col == p.col && e.idx == p.idx) {
+                e.v.x += p.v.x;
+                e.v.y += p.v.y;
+                return;
+            }
+        v.push_back(p);
+    };
+    int ne = 1, nl = 0;
+    for (int b = 0; b < count; ++b) {
+        const WorkItem &w = items[b];
+        if (w.src) {
+            for (int q = 0; q < w.nsrc; ++q)
+                ports[b].push_back(Port{w.src[q].rhs, w.src[q].basis - 1, source_value(w.src[q].ampl, w.src[q].phas)});
+        } else {
+            for (int r = 0; r < w.nrhs; ++r) ports[b].push_back(Port{r, w.gaps[r] - 1, source_value(w.ampl, w.phas)});
+        }
+        for (const Port &p : ports[b]) merge(ent[b], p);
+        for (int q = 0; q < w.nload; ++q) merge(lds[b], Port{0, w.loads[q].basis - 1, load_value(w.loads[q], w.freq)});
+        ne = std::max(ne, (int)ent[b].size());
+        nl = std::max(nl, (int)lds[b].size());
     }
     const bool mixed = o.prec != WM_PREC_F64;
     if (o.prec != WM_PREC_F64 && o.prec != WM_PREC_F32 && o.prec != WM_PREC_BF16X3) {
@@ -261,7 +370,8 @@ int solve_items(wm_ctx *c, const wm_solve_opts &o, WorkItem *items, int count, h
     const size_t oM = carve((size_t)2 * count * nr * sizeof(unsigned long long));
     const size_t oA = carve(count * sizeof(int));
     const size_t oQ = carve((size_t)n * count * sizeof(int));
-    const size_t oG = carve((size_t)count * nr * (sizeof(int) + sizeof(double2)));
+    const size_t oG = carve((size_t)count * ne * (2 * sizeof(int) + sizeof(double2)));
+    const size_t oL = carve((size_t)count * nl * (sizeof(int) + sizeof(double2)));
     WM_TRY(c->ws_sweep.ensure(off));
     char *base = (char *)c->ws_sweep.ptr;
     double2 *Z = (double2 *)(base + oZ);
@@ -271,25 +381,33 @@ int solve_items(wm_ctx *c, const wm_solve_opts &o, WorkItem *items, int count, h
     int *ipiv = (int *)(base + oP), *dinfo = (int *)(base + oI), *dact = (int *)(base + oA);
     int *perm = (int *)(base + oQ);
     unsigned long long *mx = (unsigned long long *)(base + oM);
-    int *gidx = (int *)(base + oG);
-    double2 *gval = (double2 *)(base + oG + (size_t)count * nr * sizeof(int));
+    const size_t cne = (size_t)count * ne, cnl = (size_t)count * nl;
+    int *gcol = (int *)(base + oG), *gidx = gcol + cne;
+    double2 *gval = (double2 *)(base + oG + cne * 2 * sizeof(int));
+    int *lidx = (int *)(base + oL);
+    double2 *lval = (double2 *)(base + oL + cnl * sizeof(int));
 
     // excitation: ampl*exp(j*pi*phas/180) at each feed ("wersja referencyjna/Wire_MoM.cu":307-309)
-    std::vector<int> hidx((size_t)count * nr, -1);
-    std::vector<double2> hval((size_t)count * nr, make_double2(0.0, 0.0));
+    std::vector<int> hcol(cne, 0), hidx(cne, -1), hlidx(cnl, -1);
+    std::vector<double2> hval(cne, make_double2(0.0, 0.0)), hlval(cnl, make_double2(0.0, 0.0));
     for (int b = 0; b < count; ++b) {
-        const double arg = kPi * (double)items[b].phas / 180.;
-        const double2 e = make_double2((double)items[b].ampl * std::cos(arg), (double)items[b].ampl * std::sin(arg));
-        for (int r = 0; r < items[b].nrhs; ++r) {
-            hidx[(size_t)b * nr + r] = items[b].gaps[r] - 1;
-            hval[(size_t)b * nr + r] = e;
+        for (size_t e = 0; e < ent[b].size(); ++e) {
+            hcol[(size_t)b * ne + e] = ent[b][e].col;
+            hidx[(size_t)b * ne + e] = ent[b][e].idx;
+            hval[(size_t)b * ne + e] = ent[b][e].v;
+        }
+        for (size_t e = 0; e < lds[b].size(); ++e) {
+            hlidx[(size_t)b * nl + e] = lds[b][e].idx;
+            hlval[(size_t)b * nl + e] = lds[b][e].v;
         }
     }
-    WM_TRY(copy_h2d(gidx, hidx.data(), hidx.size() * sizeof(int), s));
-    WM_TRY(copy_h2d(gval, hval.data(), hval.size() * sizeof(double2), s));
+    WM_TRY(copy_h2d(gcol, hcol.data(), cne * sizeof(int), s));
+    WM_TRY(copy_h2d(gidx, hidx.data(), cne * sizeof(int), s));
+    WM_TRY(copy_h2d(gval, hval.data(), cne * sizeof(double2), s));
     const int64_t nv = sV * count;
     hipLaunchKernelGGL(k_zero2, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, s, B, nv);
-    hipLaunchKernelGGL(k_set_rhs, dim3((count * nr + 255) / 256), dim3(256), 0, s, B, sV, n, nr, gidx, gval, count);
+    hipLaunchKernelGGL(k_set_rhs, dim3((unsigned)((cne + 255) / 256)), dim3(256), 0, s, B, sV, n, ne, gcol, gidx, gval,
+                       count);
 
     // fill, one launch group per run of items sharing a geometry
     for (int b0 = 0; b0 < count;) {
@@ -300,6 +418,13 @@ int solve_items(wm_ctx *c, const wm_solve_opts &o, WorkItem *items, int count, h
         WM_TRY(fill_launch(c, items[b0].g, b1 - b0, ka.data(), Z + (size_t)b0 * sZ, sZ,
                            mixed ? (void *)(Z32 + (size_t)b0 * sZ) : nullptr, sZ, s));
         b0 = b1;
+    }
+    // lumped loads on the diagonals ("wersja referencyjna/Wire_MoM.cu":344-363)
+    if (nl > 0) {
+        WM_TRY(copy_h2d(lidx, hlidx.data(), cnl * sizeof(int), s));
+        WM_TRY(copy_h2d(lval, hlval.data(), cnl * sizeof(double2), s));
+        hipLaunchKernelGGL(k_add_diag, dim3((unsigned)((cnl + 255) / 256)), dim3(256), 0, s, Z, sZ, n, Z32, lidx, lval,
+                           nl, count);
     }
 
     std::vector<int> hinfo(count, 0), iters(count, 0), fell(count, 0);
@@ -458,15 +583,17 @@ int solve_items(wm_ctx *c, const wm_solve_opts &o, WorkItem *items, int count, h
             set_error("zero pivot at column %d (%g MHz)", hinfo[b], (double)w.freq);
             status = hinfo[b];
         }
-        for (int r = 0; r < w.nrhs; ++r) {
-            const double2 *xv = &hX[(size_t)b * sV + (size_t)r * n];
-            if (w.I)
+        if (w.I)
+            for (int r = 0; r < w.nrhs; ++r) {
+                const double2 *xv = &hX[(size_t)b * sV + (size_t)r * n];
                 for (int i = 0; i < N; ++i) w.I[(size_t)r * N + i] = wm_c128{xv[i].x, xv[i].y};
-            if (w.zin) {
-                const double2 e = hval[(size_t)b * nr + r];
-                w.zin[r] = cdiv(wm_c128{e.x, e.y}, wm_c128{xv[w.gaps[r] - 1].x, xv[w.gaps[r] - 1].y});
             }
-        }
+        if (w.zin)
+            for (size_t q = 0; q < ports[b].size(); ++q) {
+                const Port &p = ports[b][q];
+                const double2 x = hX[(size_t)b * sV + (size_t)p.col * n + p.idx];
+                w.zin[q] = cdiv(wm_c128{p.v.x, p.v.y}, wm_c128{x.x, x.y});
+            }
         if (w.st) {
             w.st->iters = iters[b];
             w.st->info = hinfo[b];

@@ -99,6 +99,9 @@ struct wm_ctx {
     static const int kTuneSizes = 32;
     int getrf_block[3][kTuneSizes] = {};
     wm::Prof prof;          // per-kernel-class event timing (off by default)
+    // panel workgroups assumed co-resident (0 = the occupancy query's count);
+    // lowered only by tests, to split panel launches at small sizes
+    int panel_resident_cap = 0;
 };
 
 // Geometry as the fill kernel reads it.  Observation data per basis (SoA,
@@ -186,7 +189,11 @@ struct WorkItem {
     const int *gaps = nullptr; // 1-based feed bases, nrhs of them
     float ampl = 1.f, phas = 0.f;
     wm_c128 *I = nullptr;      // [nrhs][npls]
-    wm_c128 *zin = nullptr;    // [nrhs]
+    wm_c128 *zin = nullptr;    // [nrhs], or [nsrc] with generators
+    const wm_source *src = nullptr;  // generators (null: one delta gap per column, gaps/ampl/phas)
+    int nsrc = 0;
+    const wm_load *loads = nullptr;  // lumped series loads on Z's diagonal
+    int nload = 0;
     wm_freq_stats *st = nullptr;
     int fprec = -1;            // mixed precision: factor precision chosen by routing (-1: the batch's);
                                // out: the precision of the factors that produced the result
@@ -220,6 +227,9 @@ void route_items(const wm_solve_opts &o, WorkItem *items, int count);
 // than with complex64 factors plus refinement (small n: the factorization is
 // bound by the panels' per-column latency in both precisions).
 bool prefer_f64(wm_ctx *c, int prec, int n, int count);
+// A run of r complex64-routed frequencies followed by FP64-routed ones: true if
+// factoring the run with the FP64 batch is predicted cheaper than on its own.
+bool join_f64(wm_ctx *c, int prec, int n, int r);
 // Matrices per launch of the batched factorization (panel cluster residency).
 int getrf_width(wm_ctx *ctx, int prec, int n);
 // Panels per outer block for a factorization of size n in precision prec.

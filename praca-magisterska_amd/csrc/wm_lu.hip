@@ -1016,10 +1016,13 @@ int occupancy_panel(int *blocks_per_cu)
     return WM_OK;
 }
 
-// Launch plan of a batched factorization: matrices per launch chunk so that
-// every panel cluster is co-resident, and the exchange buffer it needs.
+// Launch plan of a batched factorization.  All members step through the
+// panels together; a panel launch holds as many members as keep every panel
+// cluster co-resident at that panel's height (resident / P workgroups), so
+// tall panels go in several launches and short ones in one, and each member
+// has its own exchange buffer.
 struct GetrfPlan {
-    int pmax = 0, per_launch = 0, nchunks = 0;
+    int pmax = 0, resident = 0, per_launch = 0;
     size_t xbytes = 0;
 };
 
@@ -1031,18 +1034,16 @@ int getrf_plan(wm_ctx *ctx, int n, int batch, GetrfPlan *pl)
     const int pmax = (n + RW - 1) / RW;
     int bpc = 0;
     WM_TRY((occupancy_panel<T, TPR>(&bpc)));
-    const int resident = bpc * ctx->num_cu;
+    int resident = bpc * ctx->num_cu;
+    if (ctx->panel_resident_cap > 0 && ctx->panel_resident_cap < resident) resident = ctx->panel_resident_cap;
     if (pmax > resident || bpc <= 0) {
         set_error("getrf: panel cluster of %d workgroups exceeds %d resident", pmax, resident);
         return WM_ERR_ARG;
     }
-    int per_launch = resident / pmax;
-    if (per_launch > batch) per_launch = batch;
-    if (per_launch > wm_ctx::kSyncCap) per_launch = wm_ctx::kSyncCap;
     pl->pmax = pmax;
-    pl->per_launch = per_launch;
-    pl->nchunks = (batch + per_launch - 1) / per_launch;
-    pl->xbytes = (size_t)per_launch * XLayout<V>(pmax).mat_bytes;
+    pl->resident = resident;
+    pl->per_launch = std::min(batch, resident / pmax);
+    pl->xbytes = (size_t)batch * XLayout<V>(pmax).mat_bytes;
     return WM_OK;
 }
 
@@ -1055,7 +1056,7 @@ int getrf_impl(wm_ctx *ctx, const GetrfPlan &pl, int n, typename CT<T>::V *A, in
     using V = typename CT<T>::V;
     constexpr int RW = 256 / TPR;
     constexpr int CB = (sizeof(T) == 8) ? 16 : 32;
-    const int pmax = pl.pmax, per_launch = pl.per_launch;
+    const int pmax = pl.pmax;
     const XLayout<V> X(pmax);
     char *xbase = (char *)ctx->ws_lu.ptr;
     const double esz = 2.0 * sizeof(T);
@@ -1072,18 +1073,32 @@ int getrf_impl(wm_ctx *ctx, const GetrfPlan &pl, int n, typename CT<T>::V *A, in
         char *xbuf;
         hipStream_t st;
     };
+    // panel at column c for every member, in launches of as many members as
+    // keep their clusters co-resident; members b0.. of a launch use exchange
+    // buffers b0.. (the interchange kernel reads member b's permutation there)
     auto panel = [&](const Grp &G, int c) {
-        PanelArgs<T> pa;
-        pa.A = G.Ab; pa.lda = lda; pa.strideA = strideA; pa.n = n; pa.k = c;
-        pa.P = (n - c + RW - 1) / RW; pa.pmax = pmax;
-        pa.ipiv = G.ipb; pa.strideP = strideP; pa.info = G.infob;
-        pa.err = ctx->d_err; pa.xbuf = G.xbuf; pa.bidx = G.bidx; pa.tagp = ctx->d_tag; pa.toff = (uint32_t)c;
+        const int P = (n - c + RW - 1) / RW;
+        const int per = std::max(1, std::min(G.m, pl.resident / P));
         const double mr = n - c;
-        // per column j, (mr-j) scalings and (mr-j)(NB-j-1) complex multiply-adds
-        prof_begin(ctx, kPanelClass, G.st);
-        hipLaunchKernelGGL((k_panel<T, TPR>), dim3(pa.P, G.m), dim3(256), 0, G.st, pa);
-        prof_end(ctx, kPanelClass, G.st, G.m * 8.0 * (mr * NB * NB / 2.0 - (double)NB * NB * NB / 6.0),
-                 G.m * 2.0 * mr * NB * esz);
+        for (int b0 = 0; b0 < G.m; b0 += per) {
+            const int m = std::min(per, G.m - b0);
+            PanelArgs<T> pa;
+            pa.A = G.bidx ? G.Ab : G.Ab + (size_t)b0 * strideA;
+            pa.lda = lda; pa.strideA = strideA; pa.n = n; pa.k = c;
+            pa.P = P; pa.pmax = pmax;
+            pa.ipiv = G.bidx ? G.ipb : G.ipb + (size_t)b0 * strideP;
+            pa.strideP = strideP;
+            pa.info = G.bidx ? G.infob : G.infob + b0;
+            pa.err = ctx->d_err;
+            pa.xbuf = G.xbuf + (size_t)b0 * X.mat_bytes;
+            pa.bidx = G.bidx ? G.bidx + b0 : nullptr;
+            pa.tagp = ctx->d_tag; pa.toff = (uint32_t)c;
+            // per column j, (mr-j) scalings and (mr-j)(NB-j-1) complex multiply-adds
+            prof_begin(ctx, kPanelClass, G.st);
+            hipLaunchKernelGGL((k_panel<T, TPR>), dim3(P, m), dim3(256), 0, G.st, pa);
+            prof_end(ctx, kPanelClass, G.st, m * 8.0 * (mr * NB * NB / 2.0 - (double)NB * NB * NB / 6.0),
+                     m * 2.0 * mr * NB * esz);
+        }
     };
     // interchanges of the panel at c left of c; solve for [tr0, tr1); interchanges only for [tr1, n)
     auto swap = [&](const Grp &G, int c, int tr1, int do_swap, int tr0) {
@@ -1112,14 +1127,13 @@ int getrf_impl(wm_ctx *ctx, const GetrfPlan &pl, int n, typename CT<T>::V *A, in
                  G.m * (2.0 * mrows * ncols + (double)(mrows + ncols) * K) * esz);
     };
 
-    for (int b0 = 0; b0 < batch; b0 += per_launch) {
+    {
         Grp G;
-        G.m = (batch - b0 < per_launch) ? batch - b0 : per_launch;
-        // with an index table the bases stay put and members are looked up
-        G.Ab = d_bidx ? A : A + (size_t)b0 * strideA;
-        G.ipb = d_bidx ? ipiv : ipiv + (size_t)b0 * strideP;
-        G.infob = d_bidx ? d_info : d_info + b0;
-        G.bidx = d_bidx ? d_bidx + b0 : nullptr;
+        G.m = batch;
+        G.Ab = A;
+        G.ipb = ipiv;
+        G.infob = d_info;
+        G.bidx = d_bidx;
         G.xbuf = xbase;
         G.st = s;
         hipLaunchKernelGGL(k_tag_bump, dim3(1), dim3(64), 0, s, ctx->d_tag, (uint32_t)n);
@@ -1191,7 +1205,7 @@ int getrf_run(wm_ctx *ctx, int prec, int n, void *A, int lda, int64_t strideA, i
     const int ks = getrf_block(ctx, prec, n);
     GetrfPlan pl;
     WM_TRY((getrf_plan<T, TPR>(ctx, n, batch, &pl)));
-    const uint64_t ntags = (uint64_t)pl.nchunks * n;
+    const uint64_t ntags = (uint64_t)n;
     WM_TRY(prepare_xbuf(ctx, pl.xbytes, ntags, s));
     if (!ctx->use_graphs || ctx->prof.on) {
         WM_TRY((getrf_enqueue<T, TPR>(ctx, pl, n, A, lda, strideA, ipiv, strideP, batch, d_info, d_bidx, s, gk, ks)));
@@ -1270,6 +1284,42 @@ int getrf_batched(wm_ctx *ctx, int prec, int n, void *A, int lda, int64_t stride
 // blocked factorization on a device matrix so tests can check each kernel
 // against numpy.  A is n x n (lda = n), complex128 for prec 0, complex64 else.
 // C -= L U with C = A[K.., K..], L = A[K.., 0..K), U = A[0..K, K..] (no factorization).
+// Caps the panel workgroups taken as co-resident (0 restores the occupancy
+// query's count), so small factorizations split their panel launches as large
+// ones do; drops captured factorizations, whose launch split it changes.
+extern "C" int wmx_set_panel_resident(wm_ctx *ctx, int cap)
+{
+    if (!ctx || cap < 0) return WM_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    ctx->panel_resident_cap = cap;
+    wm::drop_graphs(ctx);
+    return WM_OK;
+}
+
+// Factors the members idx[0..count) of a batch through an index table (as the
+// refinement levels do); info[m] receives member m's info word.
+extern "C" int wmx_getrf_batched_idx(wm_ctx *ctx, int prec, int n, void *A, int lda, int64_t strideA, int *ipiv,
+                                     int64_t strideP, int count, int *info, const int *idx)
+{
+    using namespace wm;
+    if (!ctx || count <= 0 || !idx || !info) return WM_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    int nm = 0;
+    for (int i = 0; i < count; ++i) nm = std::max(nm, idx[i] + 1);
+    int *d = nullptr;
+    WM_HIP_TRY(hipMalloc(&d, sizeof(int) * (size_t)(count + nm)));
+    int rc = copy_h2d(d, idx, sizeof(int) * count, ctx->stream);
+    if (rc == WM_OK) rc = getrf_batched(ctx, prec, n, A, lda, strideA, ipiv, strideP, count, d + count, ctx->stream, d);
+    if (rc == WM_OK) rc = check_device_error(ctx, ctx->stream);
+    std::vector<int> h(nm, 0);
+    if (rc == WM_OK) rc = copy_d2h(h.data(), d + count, sizeof(int) * nm, ctx->stream);
+    for (int i = 0; i < count && rc == WM_OK; ++i) info[i] = h[idx[i]];
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(d);
+    return rc;
+}
+
 extern "C" int wmx_gemm_k(wm_ctx *ctx, int prec, int n, void *A, int K, int reps)
 {
     using namespace wm;

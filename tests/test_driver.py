@@ -106,3 +106,37 @@ def test_monopole_over_ground_lines_match_oracle(geometries, oracle, tmp_path):
         zo = oracle.solve_freq(t, f, gaps=(1,), ienv=2)[0][0]
         assert abs(z - zo) <= 1e-3 * abs(zo), (f, z, zo)
         assert float(f"{zo.real:.4g}") == z.real and float(f"{zo.imag:.4g}") == z.imag
+
+
+@pytest.mark.gpu
+def test_loads_and_generators_files_match_oracle(geometries, oracle, tmp_path):
+    """-L loads file and -S generators file: the printed Zin (V/I at the first
+    generator) equals the oracle's to the 4 significant digits printed."""
+    geo = str(tmp_path / "dipol.geo")
+    geometries.dipole(geo, 512, 1.0, 0.001)
+    freqs = [150.0, 300.0]
+    _write_freq(tmp_path / "freq.txt", freqs)
+    with open(tmp_path / "loads.txt", "w") as f:
+        f.write("# basis R[ohm] L[uH] C[uF]\n100 50 0 0\n400 0 0.05 0\n400 10 0 1e-5\n")
+    with open(tmp_path / "src.txt", "w") as f:
+        f.write("256 1 0\n128 0.5 90  # second generator\n")
+    r = _run(["-L", "loads.txt", "-S", "src.txt"], str(tmp_path))
+    assert r.returncode == 0, r.stderr
+    rows = _lines(r.stdout)
+    assert [f for _, f, _ in rows] == freqs
+    t = oracle.load(geo)
+    loads = [(100, 50.0, 0.0, 0.0), (400, 0.0, 0.05, 0.0), (400, 10.0, 0.0, 1e-5)]
+    for _, f, z in rows:
+        zo = oracle.solve_freq_ex(t, f, 1, [(256, 0, 1.0, 0.0), (128, 0, 0.5, 90.0)], loads)[0][0]
+        assert float(f"{zo.real:.4g}") == z.real and float(f"{zo.imag:.4g}") == z.imag, (f, z, zo)
+
+
+def test_bad_loads_file_is_rejected(geometries, tmp_path):
+    geo = str(tmp_path / "dipol.geo")
+    geometries.dipole(geo, 64, 1.0, 0.001)
+    _write_freq(tmp_path / "freq.txt", [100.0])
+    with open(tmp_path / "loads.txt", "w") as f:
+        f.write("10 50 0\n")
+    r = _run(["-s", "32", "-L", "loads.txt"], str(tmp_path))
+    assert r.returncode == 1
+    assert "expected 4 numbers" in r.stderr

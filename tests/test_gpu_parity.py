@@ -256,6 +256,56 @@ def test_getrf_graph_replays_match_direct(wm, ctx, prec):
         ctx.set_graphs(True)
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32", "bf16x3"])
+@pytest.mark.parametrize("indexed", [False, True])
+def test_getrf_split_panel_launches_match(wm, ctx, prec, indexed):
+    """Members of a batch step through the panels together, each panel in as
+    many launches as keep the clusters co-resident at its height.  With the
+    co-resident count capped (tall panels in 3 launches, short ones in 1) the
+    factors and pivots equal those of one launch per panel, bit for bit, also
+    when members are picked through an index table."""
+    n, b = 576, 5
+    rng = np.random.RandomState(23)
+    dt = np.complex128 if prec == "f64" else np.complex64
+    A = (rng.randn(b, n, n) + 1j * rng.randn(b, n, n)).astype(dt)
+    colmajor = np.ascontiguousarray(A.transpose(0, 2, 1))
+    dev = wm.DeviceArray(ctx, colmajor)
+    piv = wm.DeviceArray(ctx, np.zeros((b, n), np.int32))
+    info = (C.c_int * b)()
+    idx = (C.c_int * 3)(4, 1, 3)
+    L = wm.lib()
+    L.wmx_set_panel_resident.argtypes = [C.c_void_p, C.c_int]
+    L.wmx_getrf_batched_idx.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int64, C.c_void_p,
+                                        C.c_int64, C.c_int, C.c_void_p, C.c_void_p]
+    p = wm.PREC_NAMES[prec]
+
+    def factor():
+        dev.upload(colmajor)
+        piv.upload(np.zeros((b, n), np.int32))
+        if indexed:
+            rc = L.wmx_getrf_batched_idx(ctx.h, p, n, dev.ptr, n, n * n, piv.ptr, n, len(idx), info, idx)
+        else:
+            rc = L.wm_getrf_batched(ctx.h, p, n, dev.ptr, n, n * n, piv.ptr, n, b, info)
+        assert rc == 0
+        ctx.sync()
+        return dev.download(), piv.download()
+
+    ref_lu, ref_piv = factor()
+    # complex128 panels hold 128 rows per workgroup, complex64 256: the first
+    # panels need 5 (3) workgroups per member, the last 1
+    cap = 12 if prec == "f64" else 6
+    assert L.wmx_set_panel_resident(ctx.h, cap) == 0
+    try:
+        lu, pv = factor()
+    finally:
+        assert L.wmx_set_panel_resident(ctx.h, 0) == 0
+    assert np.array_equal(pv, ref_piv)
+    assert np.array_equal(lu, ref_lu)
+    if indexed:  # members outside the table are untouched
+        for m in (0, 2):
+            assert np.array_equal(lu[m], colmajor[m])
+
+
 def test_getrf_f64_zero_pivot_info(wm, ctx):
     A = np.eye(128, dtype=complex)[None].repeat(2, axis=0)
     A[1, :, 70] = 0
@@ -498,3 +548,62 @@ def test_large_cage_multi_rhs(wm, geometries, ctx, geo_dir):
         print(prec, [(s["prec"], s["iters"]) for s in st], rel, relI)
         assert rel <= 1e-10 and relI <= 1e-9, (prec, rel, relI)
         assert all(s["prec"] != wm.PREC_F64 for s in st)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_sweep_with_generators_and_loads_matches_oracle(wm, geometries, oracle, ctx, geo_dir, prec):
+    """wm_solve_sweep_ex: three generators over two right-hand sides (two in
+    one), loads with and without a capacitor (two on one basis, summed).  FP64
+    currents and V/I at each generator within 1e-10 of the oracle; FP32
+    factors with FP64 refinement within 1e-10 of FP64."""
+    p = os.path.join(geo_dir, "gl1024.geo")
+    geometries.planar_grid(p, 1024, seed=7)
+    t = oracle.load(p)
+    g = wm.Geometry.from_topology(ctx, wm.configwire(p, cwd=geo_dir))
+    freqs = [5.0, 40.0, 90.0]
+    sources = [(100, 0, 1.0, 0.0), (600, 0, 0.5, 30.0), (901, 1, 2.0, -60.0)]
+    loads = [(250, 50.0, 0.0, 0.0), (250, 0.0, 0.1, 0.0), (777, 10.0, 0.2, 1e-4), (901, 5.0, 0.0, 0.0)]
+    z64, I64, _ = wm.solve_sweep_ex(g, freqs, 2, sources, loads, "f64")
+    if prec == "f64":
+        for i, f in enumerate(freqs):
+            zo, Xo = oracle.solve_freq_ex(t, f, 2, sources, loads)
+            e = np.abs(I64[i].T - Xo).max() / np.abs(Xo).max()
+            ez = np.abs(z64[i] - zo) / np.abs(zo)
+            print(f, e, ez.max())
+            assert e <= 1e-10 and ez.max() <= 1e-10, (f, e, ez)
+    else:
+        zm, Im, st = wm.solve_sweep_ex(g, freqs, 2, sources, loads, prec, size_policy=False)
+        ez = np.abs(zm - z64) / np.abs(z64)
+        print([s["prec"] for s in st], ez.max())
+        assert ez.max() <= 1e-10
+    g.close()
+
+
+def test_sweep_loads_reproduce_reference_load_block(wm, geometries, ctx, geo_dir):
+    """The reference's hard-wired loads (iload == 1: 1 + j 2 pi f 4.5 on bases
+    592..2581, 30 on 2613, 1e12 on 2582; "wersja referencyjna/Wire_MoM.cu":346-363)
+    as a load list give the currents of the CalculateByFreq drop-in with
+    iload = 1, which adds them to its host copy of Z."""
+    p = _dipole(geometries, geo_dir, 2700, 25.0, 0.0005)
+    t = wm.configwire(p, cwd=geo_dir)
+    g = wm.Geometry.from_topology(ctx, t)
+    npls, ngap = g.npls, 1350
+    loads = [(b, 1.0, 4.5, 0.0) for b in range(592, 2582)] + [(2613, 30.0, 0.0, 0.0), (2582, 1e12, 0.0, 0.0)]
+    f = 55.0
+    zs, Is, _ = wm.solve_sweep_ex(g, [f], 1, [(ngap, 0, 1.0, 0.0)], loads, "f64")
+    g.close()
+    L = wm.lib()
+    cz = np.zeros(npls * npls, np.complex128)
+    cu = np.zeros(npls, np.complex128)
+    ci = np.zeros(npls, np.complex128)
+    fp = C.POINTER(C.c_float)
+    ip = C.POINTER(C.c_int)
+    L.CalculateByFreq.restype = None
+    L.CalculateByFreq(0, npls, 1, C.c_float(f), t.x.ctypes.data_as(fp), t.y.ctypes.data_as(fp),
+                      t.z.ctypes.data_as(fp), t.rad.ctypes.data_as(fp), t.ipc.ctypes.data_as(ip),
+                      t.ips.ctypes.data_as(ip), t.ise.ctypes.data_as(ip), t.nsmax, 1, C.c_float(1.0), C.c_float(1.0),
+                      C.c_float(1.0), C.c_float(1.0), 1, C.c_float(0.0), ngap, C.c_float(1.0),
+                      cz.ctypes.data_as(C.c_void_p), cu.ctypes.data_as(C.c_void_p), ci.ctypes.data_as(C.c_void_p), 0)
+    e = np.abs(Is[0, 0] - ci).max() / np.abs(ci).max()
+    print(npls, e, zs[0, 0], cu[ngap - 1] / ci[ngap - 1])
+    assert e <= 1e-12

@@ -261,3 +261,56 @@ def test_pec_monopole_is_half_dipole(oracle, geometries, geo_dir, nseg):
         zm, _, _ = oracle.solve_freq(tm, f, gaps=(gm,), ienv=2)
         zd, _, _ = oracle.solve_freq(td, f, gaps=(gd,))
         assert abs(zm[0] - zd[0] / 2) <= 2e-7 * abs(zd[0]), (nseg, f, zm, zd)
+
+
+def test_load_impedance_units_match_reference_block(oracle):
+    """The reference's channel load is Z_MAKE(1, 2*pi*freq*4.5) with freq the
+    float32 MHz value ("wersja referencyjna/Wire_MoM.cu":349): 4.5 uH in the
+    load's units, bit for bit; a capacitor adds -1/(2 pi f C)."""
+    for f in (0.01, 1.0, 37.5, 150.0):
+        ff = float(np.float32(f))
+        assert oracle.load_impedance(f, 1.0, 4.5, 0.0) == complex(1.0, 2.0 * np.pi * ff * 4.5)
+        z = oracle.load_impedance(f, 0.0, 0.0, 2e-6)
+        assert abs(z.imag + 1.0 / (2 * np.pi * ff * 1e6 * 2e-12)) <= 1e-12 * abs(z.imag)
+
+
+def test_series_load_at_feed_adds_to_zin(oracle, geometries, geo_dir):
+    """A series load Z_L at the fed basis: (Z + Z_L e e^T) I = V e gives
+    V/I_g = 1/(Z^-1)_gg + Z_L = Zin + Z_L exactly (circuit theory)."""
+    t = oracle.load(_dipole(geometries, geo_dir, 121, 1.0, 0.001))
+    g = 61
+    for f, load in ((150.0, (g, 50.0, 0.0, 0.0)), (90.0, (g, 10.0, 0.05, 0.0)), (200.0, (g, 0.0, 0.0, 1e-5))):
+        z0 = oracle.solve_freq_ex(t, f, 1, [(g, 0, 1.0, 0.0)])[0][0]
+        z1 = oracle.solve_freq_ex(t, f, 1, [(g, 0, 1.0, 0.0)], [load])[0][0]
+        zl = oracle.load_impedance(f, *load[1:])
+        assert abs(z1 - (z0 + zl)) <= 1e-10 * abs(z1), (f, z1, z0, zl)
+
+
+def test_load_elsewhere_is_sherman_morrison(oracle, geometries, geo_dir):
+    """A load Z_L on basis l != feed g: with x_g = Z^-1 e_g and x_l = Z^-1 e_l
+    (two generators, two right-hand sides), the loaded current is
+    I' = x_g - x_l Z_L x_g[l] / (1 + Z_L x_l[l]) (Sherman-Morrison)."""
+    p = os.path.join(geo_dir, "sm_grid.geo")
+    geometries.planar_grid(p, 1024, seed=11)
+    t = oracle.load(p)
+    g, l, f = 200, 731, 60.0
+    _, X = oracle.solve_freq_ex(t, f, 2, [(g, 0, 1.0, 0.0), (l, 1, 1.0, 0.0)])
+    # two loads on basis l are summed: 75 ohm and 0.02 uH
+    _, Xl = oracle.solve_freq_ex(t, f, 1, [(g, 0, 1.0, 0.0)], [(l, 75.0, 0.0, 0.0), (l, 0.0, 0.02, 0.0)])
+    xg, xl = X[:, 0], X[:, 1]
+    zl = oracle.load_impedance(f, 75.0, 0.0, 0.0) + oracle.load_impedance(f, 0.0, 0.02, 0.0)
+    want = xg - xl * zl * xg[l - 1] / (1 + zl * xl[l - 1])
+    assert np.abs(Xl[:, 0] - want).max() <= 1e-10 * np.abs(want).max()
+
+
+def test_generators_superpose(oracle, geometries, geo_dir):
+    """Two generators in one right-hand side give the sum of their separate
+    solutions (linearity); each generator's V/I uses its own voltage."""
+    t = oracle.load(_dipole(geometries, geo_dir, 121, 1.0, 0.001))
+    f = 120.0
+    z, X = oracle.solve_freq_ex(t, f, 1, [(30, 0, 1.0, 0.0), (90, 0, 2.0, 45.0)])
+    _, Y = oracle.solve_freq_ex(t, f, 2, [(30, 0, 1.0, 0.0), (90, 1, 2.0, 45.0)])
+    s = Y[:, 0] + Y[:, 1]
+    assert np.abs(X[:, 0] - s).max() <= 1e-12 * np.abs(s).max()
+    v2 = 2.0 * np.exp(1j * np.pi * float(np.float32(45.0)) / 180.0)
+    assert abs(z[1] - v2 / X[89, 0]) <= 1e-12 * abs(z[1])

@@ -459,6 +459,7 @@ __global__ __launch_bounds__(256) void k_swap_trsm(typename CT<T>::V *A, int lda
         for (int e = 0; e < EPT; ++e) {
             const int idx = tid + 256 * e;
             const int t = idx / CB, cc = idx % CB, col = c0 + cc;
+            val[e] = V{0, 0};
             if (t < tcnt && col < cend) {
                 val[e] = Ab[s_src[t] + (size_t)col * lda];
                 if (trsm && t < NB) s_top[t][cc] = val[e];
@@ -575,9 +576,13 @@ __device__ inline void tile_coords(int t, int TI, int TJ, int &ti, int &tj)
 // MFMA cycles per update.  The error bound per entry is that of the
 // four-product form with |a||b| replaced by (|Ar| + |Ai|)(|Br| + |Bi|), at
 // most twice as large.
-constexpr int GZ_BM = 64, GZ_BN = 64, GZ_KC = 16;
+// K is staged through LDS in chunks of 8: the smaller chunk's prefetch registers
+// keep the kernel at 152 VGPRs, three waves per SIMD, which hides the LDS and
+// global latency better than two waves with chunks of 16 (83 % against 78 % of
+// the FP64 matrix-core peak issued at K = 512, N = 8192).
+constexpr int GZ_BM = 64, GZ_BN = 64, GZ_KC = 8;
 
-__global__ __launch_bounds__(256, 2) void k_gemm_f64(double2 *Abase, int lda, int64_t strideA, GemmArgs g,
+__global__ __launch_bounds__(256, 3) void k_gemm_f64(double2 *Abase, int lda, int64_t strideA, GemmArgs g,
                                                      const int *bidx, int TI, int TJ)
 {
     const int t = xcd_tile(blockIdx.x, TI * TJ);
@@ -594,26 +599,27 @@ __global__ __launch_bounds__(256, 2) void k_gemm_f64(double2 *Abase, int lda, in
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wi = (wave & 1) * 32, wj = (wave >> 1) * 32;
 
+    constexpr int EA = GZ_KC / 4, BJS = 256 / GZ_KC, EB = GZ_BN / BJS;
     __shared__ double2 sA[GZ_KC][GZ_BM + 1];
     __shared__ double2 sB[GZ_KC][GZ_BN + 1];
-    double2 pa[4], pb[4];
-    // chunk elements moved by this thread: L rows ai, k = ak + 4e; U k = bk, columns bj + 16e
+    double2 pa[EA], pb[EB];
+    // chunk elements moved by this thread: L rows ai, k = ak + 4e; U k = bk, columns bj + BJS e
     const int ai = tid % GZ_BM, ak = tid / GZ_BM;
     const int bk = tid % GZ_KC, bj = tid / GZ_KC;
     const bool arow = i0 + ai < mrows;
     auto fetch = [&](int kc) {
         const double2 *pA = A + (i0 + ai) + (size_t)(kc + ak) * lda;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) pa[e] = arow ? pA[(size_t)(4 * e) * lda] : double2{0.0, 0.0};
+        for (int e = 0; e < EA; ++e) pa[e] = arow ? pA[(size_t)(4 * e) * lda] : double2{0.0, 0.0};
         const double2 *pB = B + (kc + bk) + (size_t)(j0 + bj) * lda;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) pb[e] = (j0 + bj + 16 * e < ncols) ? pB[(size_t)(16 * e) * lda] : double2{0.0, 0.0};
+        for (int e = 0; e < EB; ++e) pb[e] = (j0 + bj + BJS * e < ncols) ? pB[(size_t)(BJS * e) * lda] : double2{0.0, 0.0};
     };
     auto stash = [&]() {
 #pragma unroll
-        for (int e = 0; e < 4; ++e) sA[ak + 4 * e][ai] = pa[e];
+        for (int e = 0; e < EA; ++e) sA[ak + 4 * e][ai] = pa[e];
 #pragma unroll
-        for (int e = 0; e < 4; ++e) sB[bk][bj + 16 * e] = pb[e];
+        for (int e = 0; e < EB; ++e) sB[bk][bj + BJS * e] = pb[e];
     };
     v4d P[2][2], Q[2][2], S[2][2];
 #pragma unroll
@@ -1118,12 +1124,14 @@ int getrf_impl(wm_ctx *ctx, const GetrfPlan &pl, int n, typename CT<T>::V *A, in
     auto gemm = [&](const Grp &G, int r0, int c0, int mrows, int ncols, int kk, int K) {
         if (mrows <= 0 || ncols <= 0) return;
         const GemmArgs g{r0, c0, mrows, ncols, kk, K};
-        // C read and written, L and U read once; algorithmic flops of a complex
-        // multiply-add (8 real).  The kernels issue 6 (three real products) for
-        // complex128 and complex64, and 4 x 3 bf16 products for split-BF16x3.
+        // C read and written, L and U read once; flops of the kernel's algorithm:
+        // three real products per complex multiply-add (6 flops) for complex128
+        // and complex64, four FP32-equivalent real products (8 flops, each taken
+        // as three bf16 products) for split-BF16x3
+        const double fpc = gk == 2 ? 8.0 : 6.0;
         prof_begin(ctx, kGemmClass, G.st);
         launch_gemm(gk, G.Ab, lda, strideA, g, G.m, G.bidx, G.st);
-        prof_end(ctx, kGemmClass, G.st, G.m * 8.0 * mrows * ncols * K,
+        prof_end(ctx, kGemmClass, G.st, G.m * fpc * mrows * ncols * K,
                  G.m * (2.0 * mrows * ncols + (double)(mrows + ncols) * K) * esz);
     };
 

@@ -278,6 +278,12 @@ void configwire(int *npnt, int nnod, float *xnod, float *ynod, float *znod, floa
                 int nw, int *iwn, float *wrad, int *nsw, int *ise, int *isp, int *ips, int *ipc, int *iwe,
                 float *rad, int *nplsw, int *nseg, int nsmax, int nwmax, int ienv);
 
+/* Replaces tablice, "wersja referencyjna/Configuration.cpp":524-556 and
+ * Configuration.h:21: writes the geometry tables (points, basis peak nodes,
+ * segment nodes, basis segments) to tablice.txt in the working directory. */
+void tablice(int npnt, int nseg, int nsmax, float *x, float *y, float *z, int nplsw, int *ise, int *isp,
+             int *ips, int *ipc);
+
 /* Replaces zmatrix, "wersja referencyjna/Zmatrix.cpp":27-52 and Zmatrix.h:7-9:
  * fills cz (host, npls x npls, column-major) for wavenumber ka.  The fill
  * runs on the GPU.  ienv 1 (free space) or 2 (PEC ground z = 0 with the

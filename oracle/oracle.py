@@ -67,6 +67,7 @@ def _declare(L):
     L.wmo_cdiv.argtypes = [C.c_double] * 4 + [dp, dp]
     L.wmo_num_threads.restype = C.c_int
     L.wmo_set_threads.argtypes = [C.c_int]
+    L.wmo_tablice.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, fp, fp, fp, C.c_int, ip, ip, ip]
 
 
 def _ptr(a, t):
@@ -167,6 +168,16 @@ def configwire(g: GeoInput, ienv: int = 1, nsmax: int = NSMAX, nwmax: int = NWMA
     if rc != 0:
         raise ValueError(f"configwire failed with code {rc}")
     return Topology(nsmax, npnt.value, nseg.value, npls.value, x, y, z, rad, ise, isp, ips, ipc, iwe)
+
+
+def tablice(t: Topology, path: str) -> None:
+    """tablice ("wersja referencyjna/Configuration.cpp":524-556): geometry tables to `path`."""
+    L = lib()
+    rc = L.wmo_tablice(path.encode(), t.npnt, t.nseg, t.nsmax, _ptr(t.x, C.c_float), _ptr(t.y, C.c_float),
+                       _ptr(t.z, C.c_float), t.npls, _ptr(t.ise, C.c_int), _ptr(t.ips, C.c_int),
+                       _ptr(t.ipc, C.c_int))
+    if rc != 0:
+        raise OSError(f"cannot write {path}")
 
 
 def load(path: str, **kw) -> Topology:

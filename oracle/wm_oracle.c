@@ -822,3 +822,24 @@ int wmo_configwire_path(int nnod, const float *xnod, const float *ynod, const fl
     if (f) fclose(f);
     return rc;
 }
+
+/* tablice ("wersja referencyjna/Configuration.cpp":524-556): the geometry
+ * tables as text, floats as a default-formatted stream prints them (%g,
+ * 6 significant digits).  Writes to `path`; returns 0, or -5 if it cannot. */
+int wmo_tablice(const char *path, int npnt, int nseg, int nsmax, const float *x, const float *y, const float *z,
+                int nplsw, const int *ise, const int *ips, const int *ipc)
+{
+    FILE *f = fopen(path, "wb");
+    if (!f) return -5;
+    const char *rule = " ------------------------------------------------";
+    fprintf(f, "%s\n Coordinates of points: \n Number of nodes: %d\n", rule, npnt);
+    for (int i = 0; i < npnt; ++i) fprintf(f, " Point: %d\t%g\t%g\t%g\n", i, (double)x[i], (double)y[i], (double)z[i]);
+    fprintf(f, "%s\n ---== WIRES ==---\n ipc: number of node for i-th basis function for wires\n", rule);
+    for (int i = 0; i < nplsw; ++i) fprintf(f, " Basis function: %d\tNode: %d\n", i, ipc[i]);
+    fprintf(f, " Number of 1 and 2 nodes of i-th segment\n");
+    for (int i = 0; i < nseg; ++i) fprintf(f, " segment: %d\tnodes: %d\t%d\n", i, ise[i], ise[i + nsmax]);
+    fprintf(f, " Number of 1 and 2 segment of i-th basis function: \n");
+    for (int i = 0; i < nplsw; ++i) fprintf(f, " Basis function: %d\tsegments: %d\t%d\n", i, ips[i], ips[i + nsmax]);
+    fclose(f);
+    return 0;
+}

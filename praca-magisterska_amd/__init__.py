@@ -36,7 +36,7 @@ EXPORTS = [
     "wm_dev_free", "wm_memcpy_h2d", "wm_memcpy_d2h", "wm_memcpy_d2d", "wm_geom_create", "wm_geom_create_env", "wm_geom_load", "wm_geom_load_env", "wm_geom_destroy",
     "wm_geom_npls", "wm_geom_ld", "wm_zfill", "wm_getrf_batched", "wm_getrs_batched", "wm_set_getrf_block", "wm_get_getrf_block", "wm_tune_getrf", "wm_set_graphs", "wm_solve_opts_default",
     "wm_solve_sweep", "wm_solve_sweep_ex", "wm_server_opts_default", "wm_server_create", "wm_server_submit", "wm_server_step",
-    "wm_server_result", "wm_server_status", "wm_server_done", "wm_server_destroy", "geoinp", "config", "configwire", "zmatrix",
+    "wm_server_result", "wm_server_status", "wm_server_done", "wm_server_destroy", "geoinp", "config", "configwire", "tablice", "zmatrix",
     "zmatrix_gpu", "zmatrix_gpu_rank", "LU_cuBLAS", "CalculateByFreq",
 ]
 

@@ -24,6 +24,7 @@
 //   -P PREC   factor precision of the sweep: f64 (drop-in), f32, bf16x3   default f64
 //   -r N      process number printed in the lines                     default 0
 //   -t        write "GPU time.dat" (three lines of ms, as zmatrix_gpu does)
+//   -T        write tablice.txt (the geometry tables; the reference's call after config is commented out)
 //
 // Without the reference's hard-wired loads the whole frequency list is one
 // batched sweep on the GPU (wm_solve_sweep, or wm_solve_sweep_ex with -L/-S);
@@ -51,14 +52,14 @@ struct Opts {
     std::string geo = "dipol.geo", freq = "freq.txt", loads, sources;
     int ienv = 1, ngap = 100, iload = 2, rank = 0, prec = WM_PREC_F64;
     float ampl = 1.f, phas = 0.f;
-    bool timefile = false;
+    bool timefile = false, tables = false;
 };
 
 [[noreturn]] void usage(const char *argv0)
 {
     std::fprintf(stderr,
                  "usage: %s [-g geo] [-f freq.txt] [-e 1|2] [-s source] [-a volts] [-p degrees] [-l 1|2]\n"
-                 "          [-L loads.txt] [-S sources.txt] [-P f64|f32|bf16x3] [-r rank] [-t]\n",
+                 "          [-L loads.txt] [-S sources.txt] [-P f64|f32|bf16x3] [-r rank] [-t] [-T]\n",
                  argv0);
     std::exit(2);
 }
@@ -81,6 +82,7 @@ Opts parse(int argc, char **argv)
         else if (a == "-l") o.iload = std::atoi(val());
         else if (a == "-r") o.rank = std::atoi(val());
         else if (a == "-t") o.timefile = true;
+        else if (a == "-T") o.tables = true;
         else if (a == "-L") o.loads = val();
         else if (a == "-S") o.sources = val();
         else if (a == "-P") {
@@ -186,6 +188,8 @@ int main(int argc, char **argv)
            wrad.data(), nsw.data(), ise.data(), isp.data(), ips.data(), ipc.data(), iwe.data(), rad.data(), &npls,
            &nplsw, &nseg, kNsmax, kNwmax, o.ienv);
     if (npls <= 0) return fail("segmentation (configwire)");
+    if (o.tables) tablice(npnt, nseg, kNsmax, x.data(), y.data(), z.data(), nplsw, ise.data(), isp.data(), ips.data(),
+                          ipc.data());
     std::cout << std::endl
               << " ##################################################################" << std::endl
               << " +-----------------------------------------------------------------+" << std::endl

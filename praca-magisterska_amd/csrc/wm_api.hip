@@ -881,6 +881,19 @@ void geoinp(int nnmax, int nwmax, int ntmax, int *nnod, float *xnod, float *ynod
     }
 }
 
+void tablice(int npnt, int nseg, int nsmax, float *x, float *y, float *z, int nplsw, int *ise, int *isp, int *ips,
+             int *ipc)
+{
+    (void)isp;
+    const std::string text = tablice_text(npnt, nseg, nsmax, x, y, z, nplsw, ise, ips, ipc);
+    if (FILE *f = std::fopen("tablice.txt", "wb")) {
+        std::fwrite(text.data(), 1, text.size(), f);
+        std::fclose(f);
+    } else {
+        std::fprintf(stderr, "tablice: cannot write tablice.txt\n");
+    }
+}
+
 void configwire(int *npnt, int nnod, float *xnod, float *ynod, float *znod, float *x, float *y, float *z, int nw,
                 int *iwn, float *wrad, int *nsw, int *ise, int *isp, int *ips, int *ipc, int *iwe, float *rad,
                 int *nplsw, int *nseg, int nsmax, int nwmax, int ienv)

@@ -28,5 +28,9 @@ constexpr int kNnmax = 3000, kNtmax = 5000, kNwmax = 10000, kNpwmax = 20480, kNs
 
 int geo_read(const char *path, GeoInput &g);
 int configwire_build(const GeoInput &g, int ienv, int nsmax, int nwmax, Topology &t, std::string *wiretext);
+// Text of the reference's tablice.txt geometry dump ("wersja referencyjna/Configuration.cpp":524-556):
+// points, basis peak nodes, segment nodes and basis segments, floats in default stream format.
+std::string tablice_text(int npnt, int nseg, int nsmax, const float *x, const float *y, const float *z, int nplsw,
+                         const int *ise, const int *ips, const int *ipc);
 
 }  // namespace wm

@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -203,6 +204,23 @@ int configwire_build(const GeoInput &g, int ienv, int nsmax, int nwmax, Topology
     shift1(X, t.x); shift1(Y, t.y); shift1(Z, t.z); shift1(RAD, t.rad); shift1(IPC, t.ipc);
     shift2(ISE, t.ise); shift2(ISP, t.isp); shift2(IPS, t.ips); shift2(IWE, t.iwe);
     return WM_OK;
+}
+
+std::string tablice_text(int npnt, int nseg, int nsmax, const float *x, const float *y, const float *z, int nplsw,
+                         const int *ise, const int *ips, const int *ipc)
+{
+    std::ostringstream o;
+    const char *rule = " ------------------------------------------------";
+    o << rule << "\n Coordinates of points: \n Number of nodes: " << npnt << "\n";
+    for (int i = 0; i < npnt; ++i) o << " Point: " << i << "\t" << x[i] << "\t" << y[i] << "\t" << z[i] << "\n";
+    o << rule << "\n ---== WIRES ==---\n ipc: number of node for i-th basis function for wires\n";
+    for (int i = 0; i < nplsw; ++i) o << " Basis function: " << i << "\tNode: " << ipc[i] << "\n";
+    o << " Number of 1 and 2 nodes of i-th segment\n";
+    for (int i = 0; i < nseg; ++i) o << " segment: " << i << "\tnodes: " << ise[i] << "\t" << ise[i + nsmax] << "\n";
+    o << " Number of 1 and 2 segment of i-th basis function: \n";
+    for (int i = 0; i < nplsw; ++i)
+        o << " Basis function: " << i << "\tsegments: " << ips[i] << "\t" << ips[i + nsmax] << "\n";
+    return o.str();
 }
 
 }  // namespace wm

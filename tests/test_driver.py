@@ -140,3 +140,23 @@ def test_bad_loads_file_is_rejected(geometries, tmp_path):
     r = _run(["-s", "32", "-L", "loads.txt"], str(tmp_path))
     assert r.returncode == 1
     assert "expected 4 numbers" in r.stderr
+
+
+@pytest.mark.parametrize("ienv", [1, 2])
+def test_tablice_matches_oracle(geometries, oracle, tmp_path, ienv):
+    """-T writes tablice.txt ("wersja referencyjna/Configuration.cpp":524-556)
+    byte for byte as the oracle's restatement (host only: the run stops at the
+    source-zone check before any GPU call)."""
+    geo = str(tmp_path / "t.geo")
+    if ienv == 1:
+        geometries.planar_grid(geo, 1024, seed=5)
+    else:
+        geometries.monopole(geo, 60, 1.0, 0.001)
+    r = _run(["-g", geo, "-e", str(ienv), "-s", "0", "-T"], str(tmp_path))
+    assert r.returncode == 1
+    want = str(tmp_path / "oracle_tablice.txt")
+    oracle.tablice(oracle.load(geo, ienv=ienv), want)
+    with open(tmp_path / "tablice.txt", "rb") as a, open(want, "rb") as b:
+        got, exp = a.read(), b.read()
+    assert got == exp
+    assert got.count(b" Point: ") > 100 if ienv == 1 else got.count(b" Point: ") > 10

@@ -223,6 +223,24 @@ int wm_solve_sweep_ex(wm_ctx *ctx, const wm_geom *g, const wm_solve_opts *opts, 
                       const float *freq_mhz, int nrhs, int nsrc, const wm_source *src, int nload,
                       const wm_load *loads, wm_c128 *I, wm_c128 *zin, wm_freq_stats *st);
 
+/* ---- time domain ------------------------------------------------------------------ */
+/* Real responses from a sweep, the time analysis the reference prompts for
+ * but leaves out ("spectrum weight", "wersja referencyjna/Wire_MoM.cu":232-248,
+ * 390): for outputs m with transfer H[k][m] at increasing frequencies
+ * freq_mhz[k] (MHz, any spacing, e.g. the piecewise-uniform freq1.txt) and
+ * source spectrum S[k] (NULL: 1),
+ *   r[m][j] = f_0 Re X[0][m] + 2 sum_k w_k Re(X[k][m] exp(j 2 pi f_k t_j)),  X = H S,
+ * t_j = t0 + j dt [s]: the trapezoid rule for 2 Re int_0^inf X e^{j 2 pi f t} df
+ * (f in Hz) over [0, f_0, ..., f_{K-1}] with the DC value taken as Re X(f_0)
+ * (w_k = (f_{k+1} - f_{k-1}) / 2, f_{-1} = 0, f_K = f_{K-1}).  A sweep's currents for a
+ * unit delta gap give H in A/V; divided by the current at the feed they give
+ * the current transfer of a current injection.  Runs on the GPU. */
+int wm_transient(wm_ctx *ctx, int nfreq, const float *freq_mhz, int nout, const wm_c128 *H, const wm_c128 *S,
+                 int nt, double t0, double dt, double *r);
+/* Spectrum of the double-exponential source i0 (e^{-alpha t} - e^{-beta t}),
+ * t >= 0 (alpha, beta in 1/s): S = i0 (1/(alpha + j w) - 1/(beta + j w)), w = 2 pi f. */
+int wm_spectrum_dexp(int nfreq, const float *freq_mhz, double i0, double alpha, double beta, wm_c128 *S);
+
 /* ---- serving: continuous batching ------------------------------------------------ */
 typedef struct wm_server wm_server;
 

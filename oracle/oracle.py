@@ -322,3 +322,28 @@ def solve_freq(t: Topology, freq_mhz: float, gaps=(100,), ampl: float = 1.0, pha
     gaps = np.atleast_1d(gaps)
     zin = np.array([cdiv(B[int(g) - 1, c], X[int(g) - 1, c]) for c, g in enumerate(gaps)])
     return zin, X, Z
+
+
+def trapezoid_weights(freq_mhz) -> np.ndarray:
+    """Weights (Hz) of the trapezoid rule over [0, f_0, ..., f_K-1] at f_0..f_K-1 (the node at 0 aside)."""
+    f = np.asarray(freq_mhz, np.float32).astype(np.float64) * 1e6
+    d = np.diff(np.concatenate([[0.0], f])) / 2.0
+    w = d.copy()
+    w[:-1] += d[1:]
+    return w
+
+
+def transient(freq_mhz, H, S, t) -> np.ndarray:
+    """r[m, j] = f_0 Re X[0, m] + 2 sum_k w_k Re(X[k, m] e^{j 2 pi f_k t_j}), X = H S: the trapezoid rule
+    for the inverse transform of a real response with its DC value taken as Re X(f_0) (the reference
+    leaves this step out, "wersja referencyjna/Wire_MoM.cu":232-248, 390)."""
+    f = np.asarray(freq_mhz, np.float32).astype(np.float64) * 1e6
+    X = (1.0 if S is None else np.asarray(S))[:, None] * np.asarray(H).reshape(len(f), -1)
+    C = trapezoid_weights(freq_mhz)[:, None] * X
+    E = np.exp(2j * np.pi * np.outer(f, np.asarray(t, np.float64)))
+    return f[0] * X[0].real[:, None] + 2.0 * (C.T[:, :, None] * E[None, :, :]).real.sum(axis=1)
+
+
+def spectrum_dexp(freq_mhz, i0, alpha, beta) -> np.ndarray:
+    w = 2 * np.pi * np.asarray(freq_mhz, np.float32).astype(np.float64) * 1e6
+    return i0 * (1.0 / (alpha + 1j * w) - 1.0 / (beta + 1j * w))

@@ -35,7 +35,7 @@ EXPORTS = [
     "wm_prof_get", "wm_dev_alloc",
     "wm_dev_free", "wm_memcpy_h2d", "wm_memcpy_d2h", "wm_memcpy_d2d", "wm_geom_create", "wm_geom_create_env", "wm_geom_load", "wm_geom_load_env", "wm_geom_destroy",
     "wm_geom_npls", "wm_geom_ld", "wm_zfill", "wm_getrf_batched", "wm_getrs_batched", "wm_set_getrf_block", "wm_get_getrf_block", "wm_tune_getrf", "wm_set_graphs", "wm_solve_opts_default",
-    "wm_solve_sweep", "wm_solve_sweep_ex", "wm_server_opts_default", "wm_server_create", "wm_server_submit", "wm_server_step",
+    "wm_solve_sweep", "wm_solve_sweep_ex", "wm_transient", "wm_spectrum_dexp", "wm_server_opts_default", "wm_server_create", "wm_server_submit", "wm_server_step",
     "wm_server_result", "wm_server_status", "wm_server_done", "wm_server_destroy", "geoinp", "config", "configwire", "tablice", "zmatrix",
     "zmatrix_gpu", "zmatrix_gpu_rank", "LU_cuBLAS", "CalculateByFreq",
 ]
@@ -130,6 +130,8 @@ def lib() -> C.CDLL:
                                      _p(FreqStats)]
         L.wm_solve_sweep_ex.argtypes = [vp, vp, _p(SolveOpts), C.c_int, fp, C.c_int, C.c_int, _p(Source), C.c_int,
                                         _p(Load), vp, vp, _p(FreqStats)]
+        L.wm_transient.argtypes = [vp, C.c_int, fp, C.c_int, vp, vp, C.c_int, C.c_double, C.c_double, vp]
+        L.wm_spectrum_dexp.argtypes = [C.c_int, fp, C.c_double, C.c_double, C.c_double, vp]
         L.wm_server_opts_default.argtypes = [_p(ServerOpts)]
         L.wm_server_opts_default.restype = None
         L.wm_server_create.argtypes = [vp, _p(ServerOpts), _p(vp)]
@@ -439,6 +441,26 @@ def solve_sweep_ex(geom: Geometry, freqs_mhz, nrhs: int, sources, loads=(), prec
                                  src, len(loads), lds, I.ctypes.data, zin.ctypes.data, st)
     _check(rc)
     return zin, I, [_stats(s) for s in st]
+
+
+def spectrum_dexp(freqs_mhz, i0: float, alpha: float, beta: float) -> np.ndarray:
+    """Spectrum of i0 (e^{-alpha t} - e^{-beta t}) at the given frequencies (wm_spectrum_dexp)."""
+    f = np.ascontiguousarray(np.atleast_1d(freqs_mhz), np.float32)
+    S = np.zeros(len(f), np.complex128)
+    _check(lib().wm_spectrum_dexp(len(f), _ptr(f, C.c_float), float(i0), float(alpha), float(beta), S.ctypes.data))
+    return S
+
+
+def transient(ctx: Context, freqs_mhz, H, S=None, t0: float = 0.0, dt: float = 1e-7, nt: int = 1024) -> np.ndarray:
+    """Real time responses r[m, j] at t0 + j dt from transfers H[k, m] (frequency-major) and spectrum S[k]
+    (wm_transient)."""
+    f = np.ascontiguousarray(np.atleast_1d(freqs_mhz), np.float32)
+    H = np.ascontiguousarray(np.asarray(H, np.complex128).reshape(len(f), -1))
+    Sv = None if S is None else np.ascontiguousarray(np.asarray(S, np.complex128))
+    r = np.zeros((H.shape[1], nt), np.float64)
+    _check(lib().wm_transient(ctx.h, len(f), _ptr(f, C.c_float), H.shape[1], H.ctypes.data,
+                              None if Sv is None else Sv.ctypes.data, int(nt), float(t0), float(dt), r.ctypes.data))
+    return r
 
 
 def getrf_batched(ctx: Context, A: np.ndarray, prec="f64"):

@@ -607,3 +607,41 @@ def test_sweep_loads_reproduce_reference_load_block(wm, geometries, ctx, geo_dir
     e = np.abs(Is[0, 0] - ci).max() / np.abs(ci).max()
     print(npls, e, zs[0, 0], cu[ngap - 1] / ci[ngap - 1])
     assert e <= 1e-12
+
+
+def test_transient_matches_oracle(wm, ctx, oracle):
+    """wm_transient against the oracle's transform on random transfers over a
+    non-uniform grid (the piecewise-uniform steps of the reference's freq1.txt)."""
+    f = np.concatenate([np.arange(1, 10) * 1e-3, np.arange(1, 10) * 1e-2, np.arange(1, 101) * 1e-1]).astype(np.float32)
+    rng = np.random.RandomState(3)
+    H = rng.randn(len(f), 5) + 1j * rng.randn(len(f), 5)
+    S = wm.spectrum_dexp(f, 30e3, 1.4e4, 6e6)
+    assert np.abs(S - oracle.spectrum_dexp(f, 30e3, 1.4e4, 6e6)).max() <= 1e-12 * np.abs(S).max()
+    t0, dt, nt = 1e-7, 2e-8, 700
+    r = wm.transient(ctx, f, H, S, t0, dt, nt)
+    ro = oracle.transient(f, H, S, t0 + dt * np.arange(nt))
+    err = np.abs(r - ro).max() / np.abs(ro).max()
+    print(err)
+    assert err <= 1e-11
+
+
+def test_current_injection_transient_from_sweep(wm, geometries, oracle, ctx, geo_dir):
+    """Currents of an FP64 sweep divided by the feed current (a current
+    injection): at the feed the transfer is 1 and the transient is the source
+    waveform's transform; at another basis it equals the oracle's transform of
+    the same transfer."""
+    p = _dipole(geometries, geo_dir, 256, 1.0, 0.001)
+    g = wm.Geometry.load(ctx, p)
+    f = (np.arange(1, 129) * 2.0).astype(np.float32)  # 2 - 256 MHz
+    gap = 128
+    _, I, _ = wm.solve_sweep(g, f, [gap], "f64")
+    g.close()
+    H = I[:, 0, :] / I[:, 0, gap - 1][:, None]
+    S = wm.spectrum_dexp(f, 1.0, 2e7, 2e8)
+    t0, dt, nt = 0.0, 1e-10, 256
+    r = wm.transient(ctx, f, H[:, [gap - 1, 40]], S, t0, dt, nt)
+    t = t0 + dt * np.arange(nt)
+    r_src = oracle.transient(f, np.ones((len(f), 1)), S, t)[0]
+    assert np.abs(r[0] - r_src).max() <= 1e-12 * np.abs(r_src).max()
+    r40 = oracle.transient(f, H[:, 40], S, t)[0]
+    assert np.abs(r[1] - r40).max() <= 1e-11 * np.abs(r40).max()

@@ -314,3 +314,41 @@ def test_generators_superpose(oracle, geometries, geo_dir):
     assert np.abs(X[:, 0] - s).max() <= 1e-12 * np.abs(s).max()
     v2 = 2.0 * np.exp(1j * np.pi * float(np.float32(45.0)) / 180.0)
     assert abs(z[1] - v2 / X[89, 0]) <= 1e-12 * abs(z[1])
+
+
+def _uniform_grid(df_mhz, fmax_mhz):
+    return (np.arange(1, int(round(fmax_mhz / df_mhz)) + 1) * df_mhz).astype(np.float32)
+
+
+def test_transient_reproduces_double_exponential(oracle):
+    """The transform of the double-exponential spectrum (H = 1) returns
+    i0 (e^{-a t} - e^{-b t}) to the truncation error of the grid (closed form)."""
+    a, b, i0 = 1e5, 1e6, 10.0
+    f = _uniform_grid(0.002, 50.0)
+    t = np.linspace(0.2e-6, 60e-6, 120)
+    r = oracle.transient(f, np.ones((len(f), 1)), oracle.spectrum_dexp(f, i0, a, b), t)[0]
+    want = i0 * (np.exp(-a * t) - np.exp(-b * t))
+    assert np.abs(r - want).max() <= 2e-3 * np.abs(want).max(), np.abs(r - want).max()
+
+
+def test_transient_of_low_pass_is_convolution(oracle):
+    """A first-order low-pass transfer 1/(1 + j 2 pi f tau) applied to the
+    double exponential gives the convolution with e^{-t/tau}/tau in closed form."""
+    a, b, tau = 1e5, 1e6, 2e-6
+    f = _uniform_grid(0.002, 50.0)
+    t = np.linspace(0.5e-6, 60e-6, 100)
+    H = 1.0 / (1.0 + 2j * np.pi * f.astype(np.float64) * 1e6 * tau)
+    r = oracle.transient(f, H[:, None], oracle.spectrum_dexp(f, 1.0, a, b), t)[0]
+    c = 1.0 / tau
+    want = (np.exp(-a * t) - np.exp(-c * t)) * c / (c - a) - (np.exp(-b * t) - np.exp(-c * t)) * c / (c - b)
+    assert np.abs(r - want).max() <= 2e-3 * np.abs(want).max(), np.abs(r - want).max()
+
+
+def test_trapezoid_weights_integrate_linear_functions_exactly(oracle):
+    """Over [0, f_K-1] the weights plus f_0/2 at the DC node integrate linear functions exactly."""
+    f = np.array([0.001, 0.002, 0.005, 0.01, 0.05, 0.1, 1.0, 2.5], np.float32)
+    w = oracle.trapezoid_weights(f)
+    fh = f.astype(np.float64) * 1e6
+    g = 3.0 + 2e-6 * fh
+    exact = 3.0 * fh[-1] + 1e-6 * fh[-1] ** 2
+    assert abs((w * g).sum() + 0.5 * fh[0] * 3.0 - exact) <= 1e-12 * exact

@@ -47,7 +47,7 @@ CLASSES = {
 }
 # HBM bytes / algorithmic bytes of k_gemm_f64, from FETCH_SIZE + WRITE_SIZE passes
 # calibrated on a plain copy (profiles/gemm_f64_traffic_c3_n8192.md)
-GEMM_F64_HBM_RATIO = 2.03
+GEMM_F64_HBM_RATIO = 1.88
 REF_BEST_PUBLISHED = 0.148  # freq-solves/s at N=8192, reference's best variant (GPU fill + MKL LU, 6 procs)
 
 

@@ -645,3 +645,28 @@ def test_current_injection_transient_from_sweep(wm, geometries, oracle, ctx, geo
     assert np.abs(r[0] - r_src).max() <= 1e-12 * np.abs(r_src).max()
     r40 = oracle.transient(f, H[:, 40], S, t)[0]
     assert np.abs(r[1] - r40).max() <= 1e-11 * np.abs(r40).max()
+
+
+def test_planar_grid_sweep_c2(wm, geometries, oracle, ctx, geo_dir):
+    """The 10 x 10 m planar grid at N = 4096 (seed 2) over 12 frequencies
+    1 - 100 MHz, fed at the basis nearest the centre: FP64 Zin within 1e-10 of
+    the oracle at one frequency, and FP32 / split-BF16x3 factors with FP64
+    refinement within 1e-10 of the FP64 sweep at all twelve."""
+    p = os.path.join(geo_dir, "c2.geo")
+    geometries.planar_grid(p, 4096, seed=2)
+    t = oracle.load(p)
+    gap = geometries.nearest_basis(t, (5.0, 5.0, 0.0))
+    g = wm.Geometry.load(ctx, p)
+    assert g.npls == 4096
+    f = geometries.freqs_c2()
+    z64, _, _ = wm.solve_sweep(g, f, [gap], "f64")
+    zo = oracle.solve_freq(t, float(f[4]), gaps=(gap,))[0][0]
+    e = abs(z64[4, 0] - zo) / abs(zo)
+    print("c2 oracle", f[4], z64[4, 0], zo, e)
+    assert e <= 1e-10
+    for prec in ("f32", "bf16x3"):
+        z, _, st = wm.solve_sweep(g, f, [gap], prec)
+        rel = (np.abs(z - z64) / np.abs(z64)).max()
+        print(prec, [(s["prec"], s["iters"]) for s in st], rel)
+        assert rel <= 1e-10
+    g.close()

@@ -444,8 +444,12 @@ __global__ __launch_bounds__(256) void k_swap_trsm(typename CT<T>::V *A, int lda
     const int cend = tile < a.ntl ? k : trsm ? a.c_tr1 : n;
 
     // gather the old contents of the touched rows, keep the new top block
-    // (rows k..k+NB-1) of solved columns in LDS, then scatter
-    __shared__ V s_top[NB][CB];
+    // (rows k..k+NB-1) of solved columns in LDS, then scatter.  Consecutive
+    // lanes take consecutive row slots of one column: the NB rows k..k+NB-1,
+    // source or target of most moves, are then one contiguous run per column
+    // (coalesced in column-major storage); s_top is kept column by column,
+    // padded so the solve's reads across columns spread over the LDS banks.
+    __shared__ V s_top[CB][NB + 1];
     if (a.do_swap) {
         __shared__ int s_rows[2 * NB], s_src[2 * NB];
         __shared__ int s_cnt;
@@ -458,24 +462,24 @@ __global__ __launch_bounds__(256) void k_swap_trsm(typename CT<T>::V *A, int lda
 #pragma unroll
         for (int e = 0; e < EPT; ++e) {
             const int idx = tid + 256 * e;
-            const int t = idx / CB, cc = idx % CB, col = c0 + cc;
+            const int t = idx % (2 * NB), cc = idx / (2 * NB), col = c0 + cc;
             val[e] = V{0, 0};
             if (t < tcnt && col < cend) {
                 val[e] = Ab[s_src[t] + (size_t)col * lda];
-                if (trsm && t < NB) s_top[t][cc] = val[e];
+                if (trsm && t < NB) s_top[cc][t] = val[e];
             }
         }
         __syncthreads();
 #pragma unroll
         for (int e = 0; e < EPT; ++e) {
             const int idx = tid + 256 * e;
-            const int t = idx / CB, col = c0 + idx % CB;
+            const int t = idx % (2 * NB), col = c0 + idx / (2 * NB);
             if (t < tcnt && col < cend) Ab[s_rows[t] + (size_t)col * lda] = val[e];
         }
     } else if (trsm) {
         for (int idx = tid; idx < NB * CB; idx += 256) {
-            const int t = idx / CB, cc = idx % CB, col = c0 + cc;
-            if (col < cend) s_top[t][cc] = Ab[(k + t) + (size_t)col * lda];
+            const int t = idx % NB, cc = idx / NB, col = c0 + cc;
+            if (col < cend) s_top[cc][t] = Ab[(k + t) + (size_t)col * lda];
         }
     }
     if (!trsm) return;
@@ -493,7 +497,7 @@ __global__ __launch_bounds__(256) void k_swap_trsm(typename CT<T>::V *A, int lda
     T xr[RPT], xi[RPT];
 #pragma unroll
     for (int q = 0; q < RPT; ++q) {
-        const V v = cok ? s_top[rg + q * RG][c] : V{0, 0};
+        const V v = cok ? s_top[c][rg + q * RG] : V{0, 0};
         xr[q] = v.x;
         xi[q] = v.y;
     }

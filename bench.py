@@ -136,6 +136,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-bf16x3", action="store_true")
+    ap.add_argument("--no-band", action="store_true", help="skip the 2.5 - 10 MHz band where every frequency refines")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch factorizations directly instead of replaying captured HIP graphs "
                          "(for rocprofv3, whose tracing of graph capture faults on this ROCm)")
@@ -185,8 +186,11 @@ def main():
     ctx.prof_enable(False)
     value = a.steps * len(freqs) / dt
 
-    # plain FP64 GPU path on the same sweep (the drop-in precision and the accuracy reference)
+    # plain FP64 GPU path on the same sweep (the drop-in precision and the accuracy reference),
+    # after one untimed sweep that captures its factorization graphs
     if a.prec != "f64":
+        sweep("f64")
+        ctx.sync()
         t0 = time.perf_counter()
         z64, _, _ = sweep("f64")
         dt64 = time.perf_counter() - t0
@@ -210,6 +214,28 @@ def main():
                  "factor_prec_counts": {v: int(sum(s["prec"] == k for s in stb)) for k, v in names.items()},
                  "fell_back": int(sum(s["fell_back"] == 1 for s in stb)),
                  "ir_iters_mean": float(np.mean([s["iters"] for s in stb]))}
+
+    # A band where every frequency refines (2.5 - 10 MHz on the same structure, 32 log-spaced
+    # frequencies): the speedup of complex64 factors with FP64 refinement over FP64 factors that
+    # the conditioning allows, beside the full band's (below about 1.4 MHz refinement cannot finish)
+    band = None
+    if not a.no_band:
+        fb = np.logspace(np.log10(2.5), 1.0, 32).astype(np.float32)
+        runs = {}
+        for prec in ("f64", "f32", "bf16x3"):
+            wm.solve_sweep(g, fb, [gap], prec, batch=a.batch)
+            ctx.sync()
+            t0 = time.perf_counter()
+            zb_, _, stb_ = wm.solve_sweep(g, fb, [gap], prec, batch=a.batch)
+            runs[prec] = (zb_, stb_, time.perf_counter() - t0)
+        z64b, _, dt64b = runs["f64"]
+        band = {"freqs_mhz": "32 log-spaced, 2.5 - 10 MHz", "fp64_solves_per_s": len(fb) / dt64b}
+        for prec in ("f32", "bf16x3"):
+            zb_, stb_, dtb_ = runs[prec]
+            band[prec] = {"solves_per_s": len(fb) / dtb_, "speedup_vs_fp64": dt64b / dtb_,
+                          "zin_rel_err_vs_fp64_max": float((np.abs(zb_[:, 0] - z64b[:, 0]) / np.abs(z64b[:, 0])).max()),
+                          "routed_f64": int(sum(x["prec"] == 0 for x in stb_)),
+                          "ir_iters_mean": float(np.mean([x["iters"] for x in stb_]))}
 
     # roofline of the dominant kernel class (largest event time)
     peak_mfma = PEAK_TFLOPS.get(a.prec, 157.3)
@@ -282,6 +308,7 @@ def main():
         "kernel_algo": {k: {"launches": v[1], "flops": v[2], "bytes": v[3]} for k, v in prof.items() if v[1]},
         "roofline": roof,
         "bf16x3_sweep": other,
+        "refinable_band": band,
         "serving": serving,
         "cpu_baseline": cpu,
         "reference_best_published_solves_per_s": REF_BEST_PUBLISHED,

@@ -182,7 +182,7 @@ typedef struct {
  * In mixed precision the frequencies are solved highest first, in batches one
  * factorization launch wide; the refinement contraction rates already seen,
  * scaled by (f_obs/f)^3, predict the rate at each lower frequency, and each is
- * factored in the cheapest precision predicted to reach tol within 3/4 of
+ * factored in the cheapest precision predicted to reach tol within
  * max_iter: the requested one, else FP32 (for BF16X3), else FP64 directly
  * (fell_back = 2).  A batch whose size makes FP64 factors cheaper than
  * complex64 factors plus refinement (small N, where both factorizations are

@@ -273,9 +273,11 @@ bool join_f64(wm_ctx *c, int prec, int n, int r)
 void route_items(const wm_solve_opts &o, WorkItem *items, int count)
 {
     // rate that reaches tol from a first correction of relative size 1 within
-    // three quarters of max_iter (beyond that a complex64 factorization plus the
-    // iterations costs about as much as FP64 factors)
-    const double rmax = o.max_iter > 1 ? std::pow(o.tol, 1.0 / (0.75 * o.max_iter)) : 0.0;
+    // max_iter: the iterations of a batch share their triangular-solve launches
+    // (about 3 ms per iteration for a batch at N = 8192), so even slow
+    // contraction costs less than FP64 factors in a batch of their own; an item
+    // whose measured contraction falls short stops early and is refactored
+    const double rmax = o.max_iter > 1 ? std::pow(o.tol, 1.0 / o.max_iter) : 0.0;
     for (int b = 0; b < count; ++b) {
         int p = o.prec;
         if (p == WM_PREC_BF16X3) {

@@ -179,14 +179,20 @@ typedef struct {
  *   I   : complex128 [nfreq][nrhs][npls]   current coefficients
  *   zin : complex128 [nfreq][nrhs]          V/I at each feed
  *   st  : wm_freq_stats [nfreq]
- * In mixed precision the frequencies are solved highest first, in batches one
- * factorization launch wide; the refinement contraction rates already seen,
- * scaled by (f_obs/f)^3, predict the rate at each lower frequency, and each is
- * factored in the cheapest precision predicted to reach tol within
+ * An FP64 sweep factors as many frequencies per batch as the HBM budget
+ * allows (members of a batch go through the panels together).  In mixed
+ * precision the frequencies are solved highest first, in batches at most one
+ * full-height panel launch wide; the refinement contraction rates already
+ * seen, scaled by (f_obs/f)^3, predict the rate at each lower frequency, and
+ * each is factored in the cheapest precision predicted to reach tol within
  * max_iter: the requested one, else FP32 (for BF16X3), else FP64 directly
- * (fell_back = 2).  A batch whose size makes FP64 factors cheaper than
- * complex64 factors plus refinement (small N, where both factorizations are
- * bound by panel latency) is factored in FP64 directly as well.  A split-BF16x3 attempt that does not converge is retried
+ * (fell_back = 2).  Batches are cut where the routed precision changes; the
+ * run of frequencies routed to FP64 goes as one batch, and a short run of
+ * complex64 frequencies just above it joins that batch when its own panel
+ * launches and refinement are predicted to cost more.  A batch whose size
+ * makes FP64 factors cheaper than complex64 factors plus refinement (small N,
+ * where both factorizations are bound by panel latency) is factored in FP64
+ * directly as well.  A split-BF16x3 attempt that does not converge is retried
  * with FP32 factors, an FP32 one with FP64 factors (fell_back = 1).
  * Returns the first nonzero status. */
 int wm_solve_sweep(wm_ctx *ctx, const wm_geom *g, const wm_solve_opts *opts, int nfreq,

@@ -220,14 +220,33 @@ typedef struct {
     float ampl, phas;
 } wm_source;
 
-/* wm_solve_sweep with general generators and lumped loads: nrhs right-hand
- * sides built from nsrc generators, nload loads (may be 0 / NULL).
+/* Incident plane wave in right-hand side `rhs` (0-based): the reference's
+ * excitation 2, prompted for but not built ("wersja referencyjna/Wire_MoM.cu":
+ * 171-181).  The wave arrives from direction (theta, phi) [degrees], unit
+ * vector rh = (sin t cos p, sin t sin p, cos t):
+ *   E_inc(r) = (e_theta th + e_phi ph) exp(+j k rh.r)  [V/m],
+ * th = (cos t cos p, cos t sin p, -sin t), ph = (-sin p, cos p, 0).  Basis m
+ * receives sum_h l_h . E_inc(r_h) over its two half-segment midpoints r_h with
+ * half vectors l_h, the points and vectors at which the fill tests the field
+ * (the same convention as a delta gap: a field along the basis direction acts
+ * as a positive gap voltage).  Over the PEC ground add the reflected wave as a
+ * second plane wave. */
+typedef struct {
+    int rhs;
+    double theta_deg, phi_deg;
+    wm_c128 e_theta, e_phi;
+} wm_plane_wave;
+
+/* wm_solve_sweep with general excitations and lumped loads: nrhs right-hand
+ * sides built from nsrc generators and npw plane waves, nload loads (counts may
+ * be 0 with NULL arrays; at least one generator or plane wave).
  *   I   : complex128 [nfreq][nrhs][npls]
  *   zin : complex128 [nfreq][nsrc]   V_s / I at generator s (its active impedance)
  * Routing, batching and statistics as wm_solve_sweep. */
 int wm_solve_sweep_ex(wm_ctx *ctx, const wm_geom *g, const wm_solve_opts *opts, int nfreq,
-                      const float *freq_mhz, int nrhs, int nsrc, const wm_source *src, int nload,
-                      const wm_load *loads, wm_c128 *I, wm_c128 *zin, wm_freq_stats *st);
+                      const float *freq_mhz, int nrhs, int nsrc, const wm_source *src, int npw,
+                      const wm_plane_wave *pw, int nload, const wm_load *loads, wm_c128 *I, wm_c128 *zin,
+                      wm_freq_stats *st);
 
 /* ---- time domain ------------------------------------------------------------------ */
 /* Real responses from a sweep, the time analysis the reference prompts for

@@ -68,6 +68,8 @@ def _declare(L):
     L.wmo_num_threads.restype = C.c_int
     L.wmo_set_threads.argtypes = [C.c_int]
     L.wmo_tablice.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, fp, fp, fp, C.c_int, ip, ip, ip]
+    L.wmo_plane_wave_rhs.argtypes = [fp, fp, fp, ip, ip, ip, C.c_int, C.c_int] + [C.c_double] * 7 + [dp]
+    L.wmo_plane_wave_rhs.restype = None
 
 
 def _ptr(a, t):
@@ -280,8 +282,20 @@ def load_impedance(freq_mhz: float, r_ohm: float, l_uh: float, c_uf: float) -> c
     return complex(r_ohm, x)
 
 
-def solve_freq_ex(t: Topology, freq_mhz: float, nrhs: int, sources, loads=(), ienv: int = 1):
-    """One frequency with generators (basis, rhs, ampl, phas) and series loads
+def plane_wave_rhs(t: Topology, freq_mhz: float, theta_deg: float, phi_deg: float, e_theta, e_phi) -> np.ndarray:
+    """Tested incident plane-wave field per basis (wmo_plane_wave_rhs)."""
+    b = np.zeros(t.npls, np.complex128)
+    et, ep = complex(e_theta), complex(e_phi)
+    lib().wmo_plane_wave_rhs(_ptr(t.x, C.c_float), _ptr(t.y, C.c_float), _ptr(t.z, C.c_float), _ptr(t.ipc, C.c_int),
+                             _ptr(t.ips, C.c_int), _ptr(t.ise, C.c_int), t.npls, t.nsmax, wavenumber(freq_mhz),
+                             float(theta_deg), float(phi_deg), et.real, et.imag, ep.real, ep.imag,
+                             b.ctypes.data_as(C.POINTER(C.c_double)))
+    return b
+
+
+def solve_freq_ex(t: Topology, freq_mhz: float, nrhs: int, sources, loads=(), ienv: int = 1, plane_waves=()):
+    """One frequency with generators (basis, rhs, ampl, phas), plane waves (rhs, theta,
+    phi, e_theta, e_phi) and series loads
     (basis, r_ohm, l_uh, c_uf) added to Z's diagonal before the LU, as the
     reference's load block does ("wersja referencyjna/Wire_MoM.cu":344-363).
     Loads on one basis are summed first.  Returns (zin[nsrc] = V/I at each
@@ -299,6 +313,8 @@ def solve_freq_ex(t: Topology, freq_mhz: float, nrhs: int, sources, loads=(), ie
         v = excitation(1, [1], a, p)[0, 0]
         B[int(b) - 1, int(r)] += v
         vals.append(v)
+    for r, th, ph, et, ep in plane_waves:
+        B[:, int(r)] += plane_wave_rhs(t, freq_mhz, th, ph, et, ep)
     LU, ipiv, info = zgetrf(Z)
     if info:
         raise np.linalg.LinAlgError(f"zgetrf info={info}")

@@ -843,3 +843,40 @@ int wmo_tablice(const char *path, int npnt, int nseg, int nsmax, const float *x,
     fclose(f);
     return 0;
 }
+
+/* Right-hand side of an incident plane wave (the reference prompts for this
+ * excitation, "wersja referencyjna/Wire_MoM.cu":171-181, but does not build
+ * it): b[m] = sum over the two test points r_h of basis m (the points and half
+ * vectors l_h at which wim tests the field, Zmatrix.cpp:68-84, 153-165) of
+ * l_h . E exp(j ka rh . r_h), E = e_theta th + e_phi ph, the wave arriving
+ * from (theta, phi) degrees.  b: npls complex values (re, im interleaved). */
+void wmo_plane_wave_rhs(const float *x, const float *y, const float *z, const int *ipc, const int *ips,
+                        const int *ise, int npls, int nsmax, double ka, double theta_deg, double phi_deg,
+                        double eth_re, double eth_im, double eph_re, double eph_im, double *b)
+{
+    const double d2r = acos(-1.0) / 180.0;
+    const double t = theta_deg * d2r, p = phi_deg * d2r;
+    const double rh[3] = {sin(t) * cos(p), sin(t) * sin(p), cos(t)};
+    const double th[3] = {cos(t) * cos(p), cos(t) * sin(p), -sin(t)}, ph[3] = {-sin(p), cos(p), 0.0};
+    double er[3], ei[3];
+    for (int d = 0; d < 3; ++d) {
+        er[d] = eth_re * th[d] + eph_re * ph[d];
+        ei[d] = eth_im * th[d] + eph_im * ph[d];
+    }
+    for (int m = 0; m < npls; ++m) {
+        double vr = 0.0, vi = 0.0;
+        for (int half = 1; half <= 2; ++half) {
+            double rc[3], lm[3];
+            int ml, mr;
+            wmo_obs_point(x, y, z, ipc, ips, ise, nsmax, m, half, rc, lm, &ml, &mr);
+            const double ph_ = ka * (rh[0] * rc[0] + rh[1] * rc[1] + rh[2] * rc[2]);
+            const double c = cos(ph_), s = sin(ph_);
+            const double dr = lm[0] * er[0] + lm[1] * er[1] + lm[2] * er[2];
+            const double di = lm[0] * ei[0] + lm[1] * ei[1] + lm[2] * ei[2];
+            vr += dr * c - di * s;
+            vi += dr * s + di * c;
+        }
+        b[2 * m] = vr;
+        b[2 * m + 1] = vi;
+    }
+}

@@ -75,6 +75,16 @@ class Source(C.Structure):
     _fields_ = [("basis", C.c_int), ("rhs", C.c_int), ("ampl", C.c_float), ("phas", C.c_float)]
 
 
+class C128(C.Structure):
+    _fields_ = [("re", C.c_double), ("im", C.c_double)]
+
+
+class PlaneWave(C.Structure):
+    """wm_plane_wave: arrives from (theta, phi) degrees, field e_theta th + e_phi ph, in right-hand side rhs."""
+    _fields_ = [("rhs", C.c_int), ("theta_deg", C.c_double), ("phi_deg", C.c_double), ("e_theta", C128),
+                ("e_phi", C128)]
+
+
 class ServerOpts(C.Structure):
     _fields_ = [("hbm_budget", C.c_size_t), ("max_inflight", C.c_int), ("prec", C.c_int), ("streams", C.c_int),
                 ("size_policy", C.c_int)]
@@ -129,7 +139,7 @@ def lib() -> C.CDLL:
         L.wm_solve_sweep.argtypes = [vp, vp, _p(SolveOpts), C.c_int, fp, C.c_int, ip, C.c_float, C.c_float, vp, vp,
                                      _p(FreqStats)]
         L.wm_solve_sweep_ex.argtypes = [vp, vp, _p(SolveOpts), C.c_int, fp, C.c_int, C.c_int, _p(Source), C.c_int,
-                                        _p(Load), vp, vp, _p(FreqStats)]
+                                        _p(PlaneWave), C.c_int, _p(Load), vp, vp, _p(FreqStats)]
         L.wm_transient.argtypes = [vp, C.c_int, fp, C.c_int, vp, vp, C.c_int, C.c_double, C.c_double, vp]
         L.wm_spectrum_dexp.argtypes = [C.c_int, fp, C.c_double, C.c_double, C.c_double, vp]
         L.wm_server_opts_default.argtypes = [_p(ServerOpts)]
@@ -420,15 +430,20 @@ def solve_sweep(geom: Geometry, freqs_mhz, gaps, prec="f64", ampl: float = 1.0, 
 
 
 def solve_sweep_ex(geom: Geometry, freqs_mhz, nrhs: int, sources, loads=(), prec="f64", batch: int = 0,
-                   max_iter: int = 20, tol: float = 1e-12, fallback: bool = True, size_policy: bool = True):
-    """Sweep with general generators and lumped loads (wm_solve_sweep_ex).
+                   max_iter: int = 20, tol: float = 1e-12, fallback: bool = True, size_policy: bool = True,
+                   plane_waves=()):
+    """Sweep with general generators, plane waves and lumped loads (wm_solve_sweep_ex).
 
-    sources: (basis, rhs, ampl, phas) per generator; loads: (basis, r_ohm, l_uh, c_uf) per load.
+    sources: (basis, rhs, ampl, phas) per generator; loads: (basis, r_ohm, l_uh, c_uf) per load;
+    plane_waves: (rhs, theta_deg, phi_deg, e_theta, e_phi) per wave (complex field components).
     Returns (zin[nf, nsrc] = V/I at each generator, I[nf, nrhs, N], stats list)."""
     f = np.ascontiguousarray(np.atleast_1d(freqs_mhz), np.float32)
     nf, n = len(f), geom.npls
-    src = (Source * len(sources))(*[Source(int(b), int(r), float(a), float(p)) for b, r, a, p in sources])
+    src = (Source * max(1, len(sources)))(*[Source(int(b), int(r), float(a), float(p)) for b, r, a, p in sources])
     lds = (Load * max(1, len(loads)))(*[Load(int(b), float(r), float(l), float(c)) for b, r, l, c in loads])
+    pws = (PlaneWave * max(1, len(plane_waves)))(
+        *[PlaneWave(int(r), float(t), float(p), C128(complex(et).real, complex(et).imag),
+                    C128(complex(ep).real, complex(ep).imag)) for r, t, p, et, ep in plane_waves])
     o = SolveOpts()
     lib().wm_solve_opts_default(C.byref(o))
     o.prec = PREC_NAMES[prec] if isinstance(prec, str) else int(prec)
@@ -438,7 +453,8 @@ def solve_sweep_ex(geom: Geometry, freqs_mhz, nrhs: int, sources, loads=(), prec
     zin = np.zeros((nf, len(sources)), np.complex128)
     st = (FreqStats * nf)()
     rc = lib().wm_solve_sweep_ex(geom.ctx.h, geom.h, C.byref(o), nf, _ptr(f, C.c_float), int(nrhs), len(sources),
-                                 src, len(loads), lds, I.ctypes.data, zin.ctypes.data, st)
+                                 src, len(plane_waves), pws, len(loads), lds, I.ctypes.data,
+                                 zin.ctypes.data if len(sources) else None, st)
     _check(rc)
     return zin, I, [_stats(s) for s in st]
 

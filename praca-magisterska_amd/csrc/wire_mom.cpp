@@ -289,8 +289,8 @@ int main(int argc, char **argv)
                                ? wm_solve_sweep(ctx, g, &so, nfreq, freqs.data(), 1, &o.ngap, ampl, o.phas, nullptr,
                                                 zall.data(), st.data())
                                : wm_solve_sweep_ex(ctx, g, &so, nfreq, freqs.data(), 1, (int)src.size(), src.data(),
-                                                   (int)lds.size(), lds.empty() ? nullptr : lds.data(), nullptr,
-                                                   zall.data(), st.data());
+                                                   0, nullptr, (int)lds.size(), lds.empty() ? nullptr : lds.data(),
+                                                   nullptr, zall.data(), st.data());
             if (rc != WM_OK) return fail("solving the sweep");
             t_call_ms = 1e3 * t.seconds();
         }

@@ -820,10 +820,11 @@ int wm_solve_sweep(wm_ctx *c, const wm_geom *g, const wm_solve_opts *opts, int n
 }
 
 int wm_solve_sweep_ex(wm_ctx *c, const wm_geom *g, const wm_solve_opts *opts, int nfreq, const float *freq_mhz,
-                      int nrhs, int nsrc, const wm_source *src, int nload, const wm_load *loads, wm_c128 *I,
-                      wm_c128 *zin, wm_freq_stats *st)
+                      int nrhs, int nsrc, const wm_source *src, int npw, const wm_plane_wave *pw, int nload,
+                      const wm_load *loads, wm_c128 *I, wm_c128 *zin, wm_freq_stats *st)
 {
-    if (!c || !g || nfreq <= 0 || !freq_mhz || nrhs <= 0 || nsrc <= 0 || !src || nload < 0 || (nload > 0 && !loads)) {
+    if (!c || !g || nfreq <= 0 || !freq_mhz || nrhs <= 0 || nsrc < 0 || npw < 0 || nload < 0 || nsrc + npw == 0 ||
+        (nsrc > 0 && !src) || (npw > 0 && !pw) || (nload > 0 && !loads)) {
         set_error("wm_solve_sweep_ex: bad arguments");
         return WM_ERR_ARG;
     }
@@ -841,10 +842,12 @@ int wm_solve_sweep_ex(wm_ctx *c, const wm_geom *g, const wm_solve_opts *opts, in
         w.nrhs = nrhs;
         w.src = src;
         w.nsrc = nsrc;
+        w.pw = pw;
+        w.npw = npw;
         w.loads = loads;
         w.nload = nload;
         w.I = I ? I + (size_t)f * nrhs * N : nullptr;
-        w.zin = zin ? zin + (size_t)f * nsrc : nullptr;
+        w.zin = zin && nsrc > 0 ? zin + (size_t)f * nsrc : nullptr;
         w.st = st ? st + f : nullptr;
     }
     return run_sweep(c, g, o, items, nrhs);

@@ -68,6 +68,57 @@ __global__ void k_add_diag(double2 *Z, int64_t strideZ, int ld, float2 *Z32, con
     if (Z32) Z32[o] = make_float2((float)z.x, (float)z.y);
 }
 
+// Incident plane wave: B[m] += sum_h l_h . E_inc(r_h) over the two test points
+// of basis m (obs: rc1, l1, rc2, l2 as 12 arrays of ld doubles), E_inc(r) =
+// E exp(j k rh.r) with E = e_theta th + e_phi ph (complex vector, 6 doubles).
+struct PlaneWaveArgs {
+    double rh[3];
+    double ev[6];  // re x, re y, re z, im x, im y, im z
+    double ka;
+};
+
+__global__ void k_plane_wave(double2 *B, const double *__restrict__ obs, int ld, int npls, PlaneWaveArgs a)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= npls) return;
+    double vr = 0.0, vi = 0.0;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const double rx = obs[(size_t)(6 * h + 0) * ld + m], ry = obs[(size_t)(6 * h + 1) * ld + m],
+                     rz = obs[(size_t)(6 * h + 2) * ld + m];
+        const double lx = obs[(size_t)(6 * h + 3) * ld + m], ly = obs[(size_t)(6 * h + 4) * ld + m],
+                     lz = obs[(size_t)(6 * h + 5) * ld + m];
+        double sn, cs;
+        sincos(a.ka * (a.rh[0] * rx + a.rh[1] * ry + a.rh[2] * rz), &sn, &cs);
+        const double dr = lx * a.ev[0] + ly * a.ev[1] + lz * a.ev[2];
+        const double di = lx * a.ev[3] + ly * a.ev[4] + lz * a.ev[5];
+        vr += dr * cs - di * sn;
+        vi += dr * sn + di * cs;
+    }
+    double2 b = B[m];
+    b.x += vr;
+    b.y += vi;
+    B[m] = b;
+}
+
+PlaneWaveArgs plane_wave_args(const wm_plane_wave &p, double ka)
+{
+    const double d2r = kPi / 180.0;
+    const double t = p.theta_deg * d2r, f = p.phi_deg * d2r;
+    const double st = std::sin(t), ct = std::cos(t), sf = std::sin(f), cf = std::cos(f);
+    const double th[3] = {ct * cf, ct * sf, -st}, ph[3] = {-sf, cf, 0.0};
+    PlaneWaveArgs a;
+    a.rh[0] = st * cf;
+    a.rh[1] = st * sf;
+    a.rh[2] = ct;
+    for (int d = 0; d < 3; ++d) {
+        a.ev[d] = p.e_theta.re * th[d] + p.e_phi.re * ph[d];
+        a.ev[3 + d] = p.e_theta.im * th[d] + p.e_phi.im * ph[d];
+    }
+    a.ka = ka;
+    return a;
+}
+
 double2 source_value(float ampl, float phas)  // ampl*exp(j*pi*phas/180), "wersja referencyjna/Wire_MoM.cu":307-309
 {
     const double arg = kPi * (double)phas / 180.;
@@ -304,11 +355,16 @@ int solve_items(wm_ctx *c, const wm_solve_opts &o, WorkItem *items, int count, h
             set_error("solve_items: items of different sizes in one batch");
             return WM_ERR_ARG;
         }
-        if (w.src) {
+        if (!w.gaps) {  // generators and plane waves
             for (int q = 0; q < w.nsrc; ++q)
                 if (w.src[q].basis < 1 || w.src[q].basis > np || w.src[q].rhs < 0 || w.src[q].rhs >= w.nrhs) {
                     set_error("generator %d: basis %d outside 1..%d or right-hand side %d outside 0..%d", q,
                               w.src[q].basis, np, w.src[q].rhs, w.nrhs - 1);
+                    return WM_ERR_ARG;
+                }
+            for (int q = 0; q < w.npw; ++q)
+                if (w.pw[q].rhs < 0 || w.pw[q].rhs >= w.nrhs) {
+                    set_error("plane wave %d: right-hand side %d outside 0..%d", q, w.pw[q].rhs, w.nrhs - 1);
                     return WM_ERR_ARG;
                 }
         } else {
@@ -344,7 +400,7 @@ int solve_items(wm_ctx *c, const wm_solve_opts &o, WorkItem *items, int count, h
     int ne = 1, nl = 0;
     for (int b = 0; b < count; ++b) {
         const WorkItem &w = items[b];
-        if (w.src) {
+        if (!w.gaps) {
             for (int q = 0; q < w.nsrc; ++q)
                 ports[b].push_back(Port{w.src[q].rhs, w.src[q].basis - 1, source_value(w.src[q].ampl, w.src[q].phas)});
         } else {
@@ -410,6 +466,13 @@ int solve_items(wm_ctx *c, const wm_solve_opts &o, WorkItem *items, int count, h
     hipLaunchKernelGGL(k_zero2, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, s, B, nv);
     hipLaunchKernelGGL(k_set_rhs, dim3((unsigned)((cne + 255) / 256)), dim3(256), 0, s, B, sV, n, ne, gcol, gidx, gval,
                        count);
+    for (int b = 0; b < count; ++b)
+        for (int q = 0; q < items[b].npw; ++q) {
+            const WorkItem &w = items[b];
+            const PlaneWaveArgs pa = plane_wave_args(w.pw[q], wavenumber_mhz(w.freq));
+            hipLaunchKernelGGL(k_plane_wave, dim3((unsigned)((w.g->npls + 255) / 256)), dim3(256), 0, s,
+                               B + (size_t)b * sV + (size_t)w.pw[q].rhs * n, w.g->d_obs, n, w.g->npls, pa);
+        }
 
     // fill, one launch group per run of items sharing a geometry
     for (int b0 = 0; b0 < count;) {

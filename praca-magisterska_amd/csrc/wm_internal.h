@@ -194,6 +194,8 @@ struct WorkItem {
     int nsrc = 0;
     const wm_load *loads = nullptr;  // lumped series loads on Z's diagonal
     int nload = 0;
+    const wm_plane_wave *pw = nullptr;  // incident plane waves added to the right-hand sides
+    int npw = 0;
     wm_freq_stats *st = nullptr;
     int fprec = -1;            // mixed precision: factor precision chosen by routing (-1: the batch's);
                                // out: the precision of the factors that produced the result

@@ -670,3 +670,31 @@ def test_planar_grid_sweep_c2(wm, geometries, oracle, ctx, geo_dir):
         print(prec, [(s["prec"], s["iters"]) for s in st], rel)
         assert rel <= 1e-10
     g.close()
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_sweep_with_plane_waves_matches_oracle(wm, geometries, oracle, ctx, geo_dir, prec):
+    """Plane-wave excitation (an oblique wave in one right-hand side; a generator
+    plus a second wave in the other) on the planar grid: FP64 currents within
+    1e-10 of the oracle; FP32 factors with FP64 refinement within 1e-10 of FP64."""
+    p = os.path.join(geo_dir, "pw1024.geo")
+    geometries.planar_grid(p, 1024, seed=7)
+    t = oracle.load(p)
+    g = wm.Geometry.from_topology(ctx, wm.configwire(p, cwd=geo_dir))
+    freqs = [10.0, 55.0]
+    waves = [(0, 35.0, 20.0, 1.0 + 0.2j, -0.4j), (1, 120.0, 250.0, 0.0, 2.0)]
+    sources = [(300, 1, 1.0, 0.0)]
+    loads = [(640, 25.0, 0.0, 0.0)]
+    z64, I64, _ = wm.solve_sweep_ex(g, freqs, 2, sources, loads, "f64", plane_waves=waves)
+    if prec == "f64":
+        for i, f in enumerate(freqs):
+            _, Xo = oracle.solve_freq_ex(t, f, 2, sources, loads, plane_waves=waves)
+            e = np.abs(I64[i].T - Xo).max() / np.abs(Xo).max()
+            print(f, e)
+            assert e <= 1e-10, (f, e)
+    else:
+        _, Im, st = wm.solve_sweep_ex(g, freqs, 2, sources, loads, prec, size_policy=False, plane_waves=waves)
+        e = (np.abs(Im - I64).max(axis=2) / np.abs(I64).max(axis=2)).max()
+        print([s["prec"] for s in st], e)
+        assert e <= 1e-9
+    g.close()

@@ -352,3 +352,40 @@ def test_trapezoid_weights_integrate_linear_functions_exactly(oracle):
     g = 3.0 + 2e-6 * fh
     exact = 3.0 * fh[-1] + 1e-6 * fh[-1] ** 2
     assert abs((w * g).sum() + 0.5 * fh[0] * 3.0 - exact) <= 1e-12 * exact
+
+
+def _segment_steps(t):
+    """Per basis, the distance between its two far nodes (its support) projected on each axis."""
+    n = t.npls
+    f1 = t.ise[t.ips[:n] - 1] - 1
+    f2 = t.ise[t.ips[t.nsmax:t.nsmax + n] - 1 + t.nsmax] - 1
+    return (t.x[f2].astype(np.float64) - t.x[f1], t.y[f2].astype(np.float64) - t.y[f1],
+            t.z[f2].astype(np.float64) - t.z[f1])
+
+
+def test_plane_wave_broadside_is_field_times_support(oracle, geometries, geo_dir):
+    """A wave from theta = 90 deg, phi = 0 with e_theta = -E0 has E = E0 z on the
+    z axis and constant phase along a z-directed wire: each basis receives E0
+    times half its support; the crossed polarization (e_phi) gives zero."""
+    t = oracle.load(_dipole(geometries, geo_dir, 121, 1.0, 0.001))
+    E0 = 2.5 - 1.0j
+    b = oracle.plane_wave_rhs(t, 150.0, 90.0, 0.0, -E0, 0.0)
+    _, _, dz = _segment_steps(t)
+    want = E0 * dz[: t.npls] / 2
+    assert np.abs(b - want).max() <= 1e-12 * abs(E0)
+    assert np.abs(oracle.plane_wave_rhs(t, 150.0, 90.0, 0.0, 0.0, E0)).max() <= 1e-15
+
+
+def test_plane_wave_phase_and_orientation(oracle, geometries, geo_dir):
+    """An x-directed wire at y = 0.3 under a wave from theta = 90, phi = 90 (from +y)
+    with e_phi = -E0 (E = E0 x): each basis receives E0 times half its support
+    times exp(j k 0.3), the phase of the wave at the wire's line."""
+    p = os.path.join(geo_dir, "xwire.geo")
+    geometries.write_geo(p, [(-0.5, 0.3, 0.0), (0.5, 0.3, 0.0)], [(1, 2, 0.001, 61)], "x wire")
+    t = oracle.load(p)
+    f, E0 = 80.0, 1.0 + 0.5j
+    b = oracle.plane_wave_rhs(t, f, 90.0, 90.0, 0.0, -E0)
+    dx, _, _ = _segment_steps(t)
+    y0 = float(np.float32(0.3))
+    want = E0 * dx[: t.npls] / 2 * np.exp(1j * oracle.wavenumber(f) * y0)
+    assert np.abs(b - want).max() <= 1e-12 * abs(E0)

@@ -6,24 +6,31 @@
 // |re|+|im| in the column (izamax), 1-based ipiv, info = first exactly-zero
 // pivot, rows interchanged across the whole matrix.
 //
-// Blocked right-looking algorithm with panel width NB = 64.  Per panel k:
+// Blocked right-looking algorithm with panel width NB = 64, in outer blocks
+// of kSub panels.  Per panel k:
 //   1. k_panel      : GEPP of the n-k x NB panel.  A cluster of P workgroups
 //                     per matrix holds the panel in registers (one row per
 //                     thread, or two threads per row for complex128); per
-//                     column the cluster agrees on the pivot through one
-//                     bounded cross-workgroup barrier, swaps and applies the
-//                     rank-1 update locally.  It also emits the composite row
-//                     permutation of the panel.
-//   2. k_swap_trsm  : applies that permutation to every column outside the
-//                     panel and computes U12 = L11^{-1} A12 (forward
-//                     substitution, L11 from LDS).
-//   3. k_gemm_*     : A22 -= L21 U12 on the matrix cores: complex128 with
+//                     column the workgroups exchange their pivot candidates
+//                     through tagged packets, all reduce them to the same
+//                     pivot, swap and apply the rank-1 update locally.  It
+//                     also emits the composite row permutation of the panel.
+//   2. k_swap_trsm  : applies that permutation to the columns outside the
+//                     panel and computes U12 = L11^{-1} A12 for the outer
+//                     block's remaining columns (forward substitution, L11
+//                     from LDS).
+//   3. k_gemm_*     : A -= L U on the matrix cores, with three real products
+//                     per complex product: complex128 with
 //                     v_mfma_f64_16x16x4f64, complex64 with
-//                     v_mfma_f32_32x32x2f32, four real products per complex
-//                     product.  The MFMA is issued on the transposed problem
-//                     (C^T = U12^T L21^T) so each lane's accumulator column is
-//                     a run of consecutive rows of C: the C read-modify-write is
-//                     coalesced in column-major storage.
+//                     v_mfma_f32_32x32x2f32, or with split-bf16 operands and
+//                     v_mfma_f32_32x32x16_bf16.  Inside an outer block a
+//                     short product brings the later panels' rows up to
+//                     date; one product with K = 64 kSub updates the
+//                     trailing matrix per outer block.  The MFMA is issued on
+//                     the transposed problem (C^T = U^T L^T) so each lane's
+//                     accumulator column is a run of consecutive rows of C:
+//                     the C read-modify-write is coalesced in column-major
+//                     storage.
 #include <algorithm>
 
 #include "wm_cplx.h"
@@ -533,15 +540,15 @@ __global__ __launch_bounds__(256) void k_swap_trsm(typename CT<T>::V *A, int lda
 // MFMA rows are columns of C and MFMA columns are rows of C, so each
 // accumulator register of a lane maps to a run of consecutive rows of one
 // column of C and the read-modify-write of C is coalesced.  Complex products
-// take four real MFMAs (re*re - im*im, re*im + im*re).  K is walked in
-// chunks of 16 staged through LDS; the next chunk is fetched from global
-// memory into registers while the current one is multiplied.
+// take three real products (below).  K is walked in chunks staged through
+// LDS (8 for complex128, 16 for complex64); the next chunk is fetched from
+// global memory into registers while the current one is multiplied.
 //
 // Tiles are dealt to the XCDs in contiguous runs (workgroup ids go to the 8
 // XCDs round-robin) and ordered in groups of row tiles (tile_coords), so the
 // tiles an XCD runs at once share their L and U blocks in its L2.
 
-// C = A[r0.., c0..] (mrows x ncols) -= A[r0.., kk..kk+K) * A[kk..kk+K, c0..); K a multiple of 16.
+// C = A[r0.., c0..] (mrows x ncols) -= A[r0.., kk..kk+K) * A[kk..kk+K, c0..); K a multiple of 64.
 struct GemmArgs {
     int r0, c0, mrows, ncols, kk, K;
 };
@@ -1335,7 +1342,7 @@ extern "C" int wmx_getrf_batched_idx(wm_ctx *ctx, int prec, int n, void *A, int 
 extern "C" int wmx_gemm_k(wm_ctx *ctx, int prec, int n, void *A, int K, int reps)
 {
     using namespace wm;
-    if (K <= 0 || K >= n || reps <= 0) return WM_ERR_ARG;
+    if (K <= 0 || K >= n || K % NB || reps <= 0) return WM_ERR_ARG;
     const GemmArgs g{K, K, n - K, n - K, 0, K};
     for (int r = 0; r < reps; ++r)
         launch_gemm(prec == WM_PREC_F64 ? 8 : prec == WM_PREC_BF16X3 ? 2 : 4, A, n, 0, g, 1, nullptr, ctx->stream);

@@ -83,18 +83,21 @@ __device__ inline double bcast(double v, int lane)  // value of `lane` to the wh
 __device__ inline int dl_idx(int i, int j) { return i * (i - 1) / 2 + j; }
 __device__ inline int du_idx(int i, int j) { return j * (j + 1) / 2 + i; }
 
-// Triangular solve, one launch.  Workgroup g owns row block bi (lower: g,
-// upper: nblk-1-g); lower solves L y = P b, upper solves U x = y in place.
-// Waits only on blocks of workgroups with smaller ids (dispatched earlier).
+// Triangular solve, one launch.  Workgroup (member, g) owns row block bi of
+// that launch member (lower: g, upper: nblk-1-g); lower solves L y = P b,
+// upper solves U x = y in place.  The grid is (members, blocks), so workgroup
+// ids run over the members first: the resident workgroups hold the leading
+// blocks of every member and all members' chains advance together.  Waits
+// only on blocks of workgroups with smaller ids (dispatched earlier).
 template <typename T, int NR, bool UPPER>
 __global__ __launch_bounds__(256) void k_trsv(TrsvArgs a, int r_off)
 {
     using V = typename CT<T>::V;
-    const int bl = blockIdx.y, tid = threadIdx.x, row = tid & 63;
+    const int bl = blockIdx.x, tid = threadIdx.x, row = tid & 63;
     const int b = a.bidx ? a.bidx[bl] : bl;
     const int part = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int nblk = a.n / TB;
-    const int bi = UPPER ? nblk - 1 - (int)blockIdx.x : (int)blockIdx.x, r0 = bi * TB;
+    const int bi = UPPER ? nblk - 1 - (int)blockIdx.y : (int)blockIdx.y, r0 = bi * TB;
     const V *A = (const V *)a.A + (size_t)b * a.strideA;
     double2 *Y = a.Y + (size_t)b * a.strideB + (size_t)r_off * a.ldb;
     int *flags = a.flags + (size_t)bl * nblk;
@@ -289,7 +292,7 @@ __global__ void k_resid_final(const double2 *P, int nsplit, int n, int nrhs, con
 template <typename T, int NR>
 void launch_trsv(const TrsvArgs &a, int batch, int r_off, bool upper, hipStream_t s)
 {
-    const dim3 grid(a.n / TB, batch);
+    const dim3 grid(batch, a.n / TB);
     if (upper) hipLaunchKernelGGL((k_trsv<T, NR, true>), grid, dim3(256), 0, s, a, r_off);
     else hipLaunchKernelGGL((k_trsv<T, NR, false>), grid, dim3(256), 0, s, a, r_off);
 }

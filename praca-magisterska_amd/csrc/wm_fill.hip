@@ -146,6 +146,10 @@ __global__ __launch_bounds__(256) void k_fill(const double *__restrict__ obs, co
                                               FreqArgs fa, double2 *__restrict__ Z, int64_t strideZ,
                                               float2 *__restrict__ Z32, int64_t strideZ32)
 {
+    // Every rounding is spelled out (fma where fused), so an entry does not
+    // depend on how many frequencies share the launch (F): the compiler may
+    // not contract products differently per instantiation.
+#pragma clang fp contract(off)
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int m = blockIdx.x * 64 + lane;
@@ -189,8 +193,8 @@ __global__ __launch_bounds__(256) void k_fill(const double *__restrict__ obs, co
 #pragma unroll
                     for (int qp = 0; qp < 2; ++qp) {
                         const double s = qp == 0 ? half + off : half - off;
-                        const double dx = bx - s * dcx, dy = by - s * dcy, dz = bz - s * dcz;
-                        const double R = sqrt(an * an + dx * dx + dy * dy + dz * dz);
+                        const double dx = fma(-s, dcx, bx), dy = fma(-s, dcy, by), dz = fma(-s, dcz, bz);
+                        const double R = sqrt(fma(dz, dz, fma(dy, dy, fma(dx, dx, an * an))));
                         const double rR = 1.0 / R;
                         const double w = (g == 0 ? s : dl - s) * rdl;
 #pragma unroll
@@ -198,11 +202,11 @@ __global__ __launch_bounds__(256) void k_fill(const double *__restrict__ obs, co
                             double sn, cn;
                             sincos(fa.ka[f] * R, &sn, &cn);
                             const double gr = cn * rR, gi = -sn * rR;
-                            V[f].x += w * gr; V[f].y += w * gi;
-                            S[f].x += gr;     S[f].y += gi;
+                            V[f].x = fma(w, gr, V[f].x); V[f].y = fma(w, gi, V[f].y);
+                            S[f].x += gr;                S[f].y += gi;
                         }
                     }
-                    double t = dcx * lm[h][0] + dcy * lm[h][1] + dcz * lm[h][2];
+                    double t = fma(dcz, lm[h][2], fma(dcy, lm[h][1], dcx * lm[h][0]));
                     if (pass) t *= im.sgnenv;
 #pragma unroll
                     for (int f = 0; f < F; ++f) {
@@ -212,22 +216,18 @@ __global__ __launch_bounds__(256) void k_fill(const double *__restrict__ obs, co
                             v = self[((size_t)f * ld + n) * 4 + 2 * g];
                             sc = self[((size_t)f * ld + n) * 4 + 2 * g + 1];
                         }
-                        cv[f].x += v.x * t; cv[f].y += v.y * t;
-                        if (pass) {
-                            const double wq = im.sgnenv * rdl;
-                            if (g == 0) { cs[f].x += sc.x * wq; cs[f].y += sc.y * wq; }
-                            else        { cs[f].x -= sc.x * wq; cs[f].y -= sc.y * wq; }
-                        } else {
-                            if (g == 0) { cs[f].x += sc.x * rdl; cs[f].y += sc.y * rdl; }
-                            else        { cs[f].x -= sc.x * rdl; cs[f].y -= sc.y * rdl; }
-                        }
+                        cv[f].x = fma(v.x, t, cv[f].x); cv[f].y = fma(v.y, t, cv[f].y);
+                        const double wq = pass ? im.sgnenv * rdl : rdl;
+                        const double ws = g == 0 ? wq : -wq;
+                        cs[f].x = fma(sc.x, ws, cs[f].x); cs[f].y = fma(sc.y, ws, cs[f].y);
                     }
                 }
             }
 #pragma unroll
             for (int f = 0; f < F; ++f) {
-                if (h == 0) { acc[f].x += cv[f].x * fa.ka2[f] - cs[f].x; acc[f].y += cv[f].y * fa.ka2[f] - cs[f].y; }
-                else        { acc[f].x += cv[f].x * fa.ka2[f] + cs[f].x; acc[f].y += cv[f].y * fa.ka2[f] + cs[f].y; }
+                const double sx = h == 0 ? -cs[f].x : cs[f].x, sy = h == 0 ? -cs[f].y : cs[f].y;
+                acc[f].x += fma(cv[f].x, fa.ka2[f], sx);
+                acc[f].y += fma(cv[f].y, fa.ka2[f], sy);
             }
         }
         const bool inside = (m < npls) && (n < npls);

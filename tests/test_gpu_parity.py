@@ -422,7 +422,9 @@ def test_dropin_zmatrix_and_lu(wm, geometries, oracle, ctx, geo_dir):
 
 
 def test_server_result_same_alone_and_in_mix(wm, geometries, ctx, geo_dir):
-    """A request gives bit-identical results solved alone and batched with others."""
+    """A request gives bit-identical results solved alone and batched with others,
+    including another request on the same structure whose frequency joins its
+    fill launch (two matrices per launch alone, three in the mix)."""
     p1 = os.path.join(geo_dir, "s1.geo")
     p2 = os.path.join(geo_dir, "s2.geo")
     geometries.planar_grid(p1, 512, seed=21)
@@ -433,9 +435,10 @@ def test_server_result_same_alone_and_in_mix(wm, geometries, ctx, geo_dir):
     while s.step() == 0:
         pass
     alone = s.result(rid)
-    ids = [s.submit(g2, [5.0, 9.0, 11.0], [3]), s.submit(g1, [3.0, 40.0], [7, 100]), s.submit(g2, [1.0], [44, 45, 46])]
+    ids = [s.submit(g2, [5.0, 9.0, 11.0], [3]), s.submit(g1, [3.0, 40.0], [7, 100]), s.submit(g2, [1.0], [44, 45, 46]),
+           s.submit(g1, [5.0], [7])]
     got = {}
-    while len(got) < 3:
+    while len(got) < len(ids):
         s.step()
         for i in ids:
             if i not in got:
@@ -548,6 +551,26 @@ def test_large_cage_multi_rhs(wm, geometries, ctx, geo_dir):
         print(prec, [(s["prec"], s["iters"]) for s in st], rel, relI)
         assert rel <= 1e-10 and relI <= 1e-9, (prec, rel, relI)
         assert all(s["prec"] != wm.PREC_F64 for s in st)
+
+
+def test_large_cage_full_band(wm, geometries, ctx, geo_dir):
+    """The large case over its whole band: the N = 14336 cage at 16 frequencies,
+    0.1 - 50 MHz linearly spaced, with 8 feeds.  The mixed sweeps route the
+    ill-conditioned low end to FP64 factors and refine the rest; every Zin of
+    every feed is within 1e-10 of the FP64 sweep's, element by element."""
+    p = os.path.join(geo_dir, "cage.geo")
+    geometries.cage(p, 14336, seed=4)
+    g = wm.Geometry.load(ctx, p)
+    t = wm.configwire(p, cwd=geo_dir)
+    gaps = list(np.random.RandomState(4).choice(t.npls, 8, replace=False) + 1)
+    f = np.linspace(0.1, 50.0, 16).astype(np.float32)
+    z64, _, _ = wm.solve_sweep(g, f, gaps, "f64")
+    for prec in ("f32", "bf16x3"):
+        z, _, st = wm.solve_sweep(g, f, gaps, prec)
+        rel = np.abs(z - z64) / np.abs(z64)
+        print(prec, [(float(fr), s["prec"], s["iters"], s["fell_back"]) for fr, s in zip(f, st)], rel.max())
+        assert z.shape == (16, 8) and (rel <= 1e-10).all(), (prec, rel.max(axis=1))
+        assert st[-1]["prec"] != wm.PREC_F64  # the top of the band refines
 
 
 @pytest.mark.parametrize("prec", ["f64", "f32"])

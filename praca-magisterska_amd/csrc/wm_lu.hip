@@ -219,6 +219,7 @@ __global__ __launch_bounds__(256, 2) void k_panel(PanelArgs<T> a)
     __shared__ V s_l[TPR > 1 ? RW : 1];  // multipliers for the other chunks
     __shared__ double s_gv;
     __shared__ int s_gr;
+    __shared__ int s_piv[NB];  // pivot row of each column (workgroup 0)
 
     const uint32_t tag0 = *a.tagp + a.toff;
     // this row's value in the column of the next step (column 0 is in chunk 0)
@@ -320,6 +321,7 @@ __global__ __launch_bounds__(256, 2) void k_panel(PanelArgs<T> a)
         const V p = s_u[j];
         const double2 rinv = (gv != 0.0) ? crecip((double)p.x, (double)p.y) : make_double2(0.0, 0.0);
         if (w == 0 && tid == 0) {
+            s_piv[j] = gr;
             a.ipiv[(size_t)b * a.strideP + pr] = gr + 1;
             if (gv == 0.0 && a.info[b] == 0) a.info[b] = pr + 1;
         }
@@ -395,26 +397,49 @@ __global__ __launch_bounds__(256, 2) void k_panel(PanelArgs<T> a)
 
     // composite permutation of the panel's interchanges, for the other columns:
     // after applying them in order, row rows[t] holds the old content of src[t].
-    if (w == 0 && tid == 0) {
+    // Slots t < NB are rows k + t; the rows from below the panel that take
+    // part get slots NB, NB+1, ... in order of first use.  One wave of
+    // workgroup 0 keeps slot t and slot NB + t in lane t (NB is the wave
+    // width), so each interchange is a ballot and two lane reads.
+    __syncthreads();  // s_piv complete
+    if (w == 0 && wave == 0) {
+        static_assert(NB == 64, "one slot pair per lane");
+        const int rlo = k + lane;
+        int slo = k + lane, rhi = -1, shi = -1;
+        int cnt = NB;
+        for (int jx = 0; jx < NB; ++jx) {
+            const int pp = s_piv[jx];
+            if (pp == k + jx) continue;
+            int tp;
+            if (pp < k + NB) {
+                tp = pp - k;
+            } else {
+                const unsigned long long hit = __ballot(NB + lane < cnt && rhi == pp);
+                if (hit) {
+                    tp = NB + (int)__builtin_ctzll(hit);
+                } else {
+                    tp = cnt++;
+                    if (lane == tp - NB) { rhi = pp; shi = pp; }
+                }
+            }
+            const int sj = __builtin_amdgcn_readlane(slo, jx);
+            const int st = tp < NB ? __builtin_amdgcn_readlane(slo, tp) : __builtin_amdgcn_readlane(shi, tp - NB);
+            if (lane == jx) slo = st;
+            if (tp < NB) {
+                if (lane == tp) slo = sj;
+            } else if (lane == tp - NB) {
+                shi = sj;
+            }
+        }
         int *pm = X.perm(xm);
         int *rows = pm + 1, *src = pm + 1 + 2 * NB;
-        int cnt = NB;
-        for (int t = 0; t < NB; ++t) { rows[t] = k + t; src[t] = k + t; }
-        for (int jx = 0; jx < NB; ++jx) {
-            const int pp = a.ipiv[(size_t)b * a.strideP + k + jx] - 1;
-            if (pp == k + jx) continue;
-            int tp = -1;
-            if (pp < k + NB) tp = pp - k;
-            else {
-                for (int t = NB; t < cnt; ++t)
-                    if (rows[t] == pp) { tp = t; break; }
-                if (tp < 0) { tp = cnt++; rows[tp] = pp; src[tp] = pp; }
-            }
-            const int tmp = src[jx];
-            src[jx] = src[tp];
-            src[tp] = tmp;
+        rows[lane] = rlo;
+        src[lane] = slo;
+        if (NB + lane < cnt) {
+            rows[NB + lane] = rhi;
+            src[NB + lane] = shi;
         }
-        pm[0] = cnt;
+        if (lane == 0) pm[0] = cnt;
     }
 }
 

@@ -64,9 +64,8 @@ typedef float v16f __attribute__((ext_vector_type(16)));
 // every workgroup published after its reads of column j.
 //
 // Per launch member and parity:
-//   slot [pmax][4] : |candidate| as two 32-bit halves of a double, its row, unused
+//   slot [pmax][4] : |candidate| as two 32-bit halves of a double, its position, unused
 //   cand [pmax][NW]: each workgroup's candidate row (NW 32-bit words)
-//   diag [NW]      : row k+j before the interchange (from workgroup 0)
 // followed by the panel permutation: cnt, rows[2NB], src[2NB].
 template <typename V>
 struct XLayout {
@@ -75,7 +74,7 @@ struct XLayout {
     int pmax;
     __host__ __device__ XLayout(int pmax_) : pmax(pmax_)
     {
-        par_bytes = ((size_t)pmax * 4 + (size_t)pmax * NW + NW) * sizeof(uint64_t);
+        par_bytes = ((size_t)pmax * 4 + (size_t)pmax * NW) * sizeof(uint64_t);
         par_bytes = (par_bytes + 255) & ~(size_t)255;
         perm_off = 2 * par_bytes;
         mat_bytes = perm_off + (1 + 4 * NB) * sizeof(int);
@@ -83,7 +82,6 @@ struct XLayout {
     }
     __device__ uint64_t *slot(char *m, int par) const { return (uint64_t *)(m + par * par_bytes); }
     __device__ uint64_t *cand(char *m, int par) const { return slot(m, par) + (size_t)pmax * 4; }
-    __device__ uint64_t *diag(char *m, int par) const { return cand(m, par) + (size_t)pmax * NW; }
     __device__ int *perm(char *m) const { return (int *)(m + perm_off); }
 };
 
@@ -180,14 +178,23 @@ __device__ inline int wave_min_i32(int v)
 // column j take part in its argmax (a DPP reduction), and each thread carries
 // its row's value in the next pivot column (nxr, nxi), taken from the rank-1
 // update that produces it, so registers are only indexed statically.
+//
+// Interchanges move labels, not data: each thread keeps its row and carries
+// the row's current position lr (initially the row itself).  Exchanging rows
+// k+j and gr swaps the two labels; the row labelled k+j is the pivot row and
+// is final from then on, the rows labelled above k+j take the rank-1 update.
+// The arithmetic and the pivots are those of the interchanging algorithm
+// (ties go to the lowest position, as izamax), and each row is stored at its
+// final position at the end.
 template <typename T, int TPR>
-__global__ __launch_bounds__(256, 2) void k_panel(PanelArgs<T> a)
+__global__ __launch_bounds__(256, 3) void k_panel(PanelArgs<T> a)
 {
     using V = typename CT<T>::V;
     constexpr int W = NB / TPR;     // columns per thread
     constexpr int NWC = 4 / TPR;    // waves per chunk
     constexpr int RW = 64 * NWC;    // rows per workgroup
     constexpr int NW = XLayout<V>::NW;
+    constexpr int kNone = 0x7fffffff;
     static_assert(NW <= 256 && RW >= NB, "panel layout");
     const int bl = blockIdx.y, w = blockIdx.x;          // bl: member of this launch
     const int b = a.bidx ? a.bidx[bl] : bl;              // b: matrix index
@@ -213,15 +220,14 @@ __global__ __launch_bounds__(256, 2) void k_panel(PanelArgs<T> a)
     __shared__ double s_wv[NWC];
     __shared__ int s_wr[NWC];
     __shared__ V s_cand[NB];  // this workgroup's best row
-    __shared__ V s_dg[NB];    // row k+j (workgroup 0)
     __shared__ V s_u[NB];     // pivot row
-    __shared__ V s_d[NB];     // row k+j, for the workgroup holding the pivot row
     __shared__ V s_l[TPR > 1 ? RW : 1];  // multipliers for the other chunks
     __shared__ double s_gv;
-    __shared__ int s_gr;
+    __shared__ int s_gr, s_gw;
     __shared__ int s_piv[NB];  // pivot row of each column (workgroup 0)
 
     const uint32_t tag0 = *a.tagp + a.toff;
+    int lr = row;  // current position of this thread's row
     // this row's value in the column of the next step (column 0 is in chunk 0)
     T nxr = re[0], nxi = im[0];
     for (int j = 0; j < NB; ++j) {
@@ -230,52 +236,46 @@ __global__ __launch_bounds__(256, 2) void k_panel(PanelArgs<T> a)
         const int cj = j / W;
         const uint32_t tag = tag0 + (uint32_t)j + 1u;
 
-        // 1. best row of each wave of the chunk holding column j
+        // 1. best row of each wave of the chunk holding column j: largest
+        //    |re|+|im| among positions >= k+j, the lowest position on ties
         if (c == cj) {
-            const double v = (valid && row >= pr) ? fabs((double)nxr) + fabs((double)nxi) : -1.0;
+            const double v = (valid && lr >= pr) ? fabs((double)nxr) + fabs((double)nxi) : -1.0;
             const double m = wave_max_f64(v);
-            const unsigned long long bal = __ballot(v == m && v >= 0.0);
+            const int r = wave_min_i32(v == m && v >= 0.0 ? lr : kNone);
             if (lane == 0) {
                 s_wv[rb] = m;
-                s_wr[rb] = bal ? base + rb * 64 + __builtin_ctzll(bal) : 0x7fffffff;
+                s_wr[rb] = r;
             }
         }
         __syncthreads();
 
-        // 2. this workgroup's candidate; stage it and (workgroup 0) row k+j in LDS
+        // 2. this workgroup's candidate; stage its row in LDS
         double wv = s_wv[0];
         int wr = s_wr[0];
 #pragma unroll
         for (int q = 1; q < NWC; ++q)
             if (better(s_wv[q], s_wr[q], wv, wr)) { wv = s_wv[q]; wr = s_wr[q]; }
-        if (wr != 0x7fffffff && rb == (wr - base) / 64 && lane == (wr - base) % 64) {
+        if (wr != kNone && lr == wr) {
 #pragma unroll
             for (int i = 0; i < W; ++i) s_cand[c * W + i] = V{re[i], im[i]};
-        }
-        if (w == 0 && rb == 0 && lane == j) {
-#pragma unroll
-            for (int i = 0; i < W; ++i) s_dg[c * W + i] = V{re[i], im[i]};
         }
         __syncthreads();
 
         // 3. publish
         uint64_t *slot = X.slot(xm, par);
         uint64_t *cand = X.cand(xm, par);
-        uint64_t *diag = X.diag(xm, par);
-        if (tid < NW) {
-            put_pkt(cand + (size_t)w * NW + tid, tag, ((const uint32_t *)s_cand)[tid]);
-            if (w == 0) put_pkt(diag + tid, tag, ((const uint32_t *)s_dg)[tid]);
-        }
+        if (tid < NW) put_pkt(cand + (size_t)w * NW + tid, tag, ((const uint32_t *)s_cand)[tid]);
         if (tid < 3) {
             const uint64_t bits = (uint64_t)__double_as_longlong(wv);
             const uint32_t word = tid == 0 ? (uint32_t)bits : tid == 1 ? (uint32_t)(bits >> 32) : (uint32_t)wr;
             put_pkt(slot + (size_t)w * 4 + tid, tag, word);
         }
 
-        // 4. every workgroup reduces all candidates to the same pivot (first max)
+        // 4. every workgroup reduces all candidates to the same pivot (first
+        //    max) and notes the workgroup that holds it
         if (wave == 0) {
             double gv = -1.0;
-            int gr = 0x7fffffff;
+            int gr = kNone, gs = 0;
             for (int s = lane; s < a.P; s += 64) {
                 const uint64_t *ps = slot + (size_t)s * 4;
                 uint64_t w0, w1, w2;
@@ -290,34 +290,38 @@ __global__ __launch_bounds__(256, 2) void k_panel(PanelArgs<T> a)
                 }
                 const double v2 = __longlong_as_double((long long)(((w1 & 0xffffffffull) << 32) | (w0 & 0xffffffffull)));
                 const int r2 = (int)(uint32_t)w2;
-                if (better(v2, r2, gv, gr)) { gv = v2; gr = r2; }
+                if (better(v2, r2, gv, gr)) { gv = v2; gr = r2; gs = s; }
             }
             const double m = wave_max_f64(gv);
-            const int r = wave_min_i32(gv == m ? gr : 0x7fffffff);
-            if (lane == 0) { s_gv = m; s_gr = r; }
+            const int r = wave_min_i32(gv == m ? gr : kNone);
+            const unsigned long long win = __ballot(gv == m && gr == r);
+            const int ws = __builtin_amdgcn_readlane(gs, win ? (int)__builtin_ctzll(win) : 0);
+            if (lane == 0) {
+                // no usable candidate (non-finite column): a zero pivot in place
+                const bool ok = r >= pr && r < a.n;
+                s_gv = ok ? m : 0.0;
+                s_gr = ok ? r : pr;
+                s_gw = ok ? ws : 0;
+            }
         }
         __syncthreads();
         const double gv = s_gv;
         const int gr = s_gr;
-        const int gw = (gr - k) / RW;
-        const bool own_gr = gr != pr && gw == w;
+        const int gw = s_gw;
         if (tid < NW) {
             const uint64_t *pu = cand + (size_t)gw * NW + tid;
-            const uint64_t *pd = diag + tid;
-            uint64_t wu, wd = 0;
+            uint64_t wu;
             long it = 0;
             for (;;) {
                 wu = ld_pkt(pu);
-                if (own_gr) wd = ld_pkt(pd);
-                if ((uint32_t)(wu >> 32) == tag && (!own_gr || (uint32_t)(wd >> 32) == tag)) break;
+                if ((uint32_t)(wu >> 32) == tag) break;
                 if (spin_give_up(it, a.err)) break;
             }
             ((uint32_t *)s_u)[tid] = (uint32_t)wu;
-            ((uint32_t *)s_d)[tid] = (uint32_t)wd;
         }
         __syncthreads();
 
-        // 5. interchange rows k+j and gr, scale column j, rank-1 update
+        // 5. interchange positions k+j and gr, scale column j, rank-1 update
         const V p = s_u[j];
         const double2 rinv = (gv != 0.0) ? crecip((double)p.x, (double)p.y) : make_double2(0.0, 0.0);
         if (w == 0 && tid == 0) {
@@ -325,46 +329,23 @@ __global__ __launch_bounds__(256, 2) void k_panel(PanelArgs<T> a)
             a.ipiv[(size_t)b * a.strideP + pr] = gr + 1;
             if (gv == 0.0 && a.info[b] == 0) a.info[b] = pr + 1;
         }
-        if (gr != pr) {
-            if (own_gr && rb == (gr - base) / 64 && lane == (gr - base) % 64) {
-#pragma unroll
-                for (int i = 0; i < W; ++i) {
-                    const V d = s_d[c * W + i];
-                    re[i] = d.x;
-                    im[i] = d.y;
-                }
-            }
-            if (w == 0 && rb == 0 && lane == j) {
-#pragma unroll
-                for (int i = 0; i < W; ++i) {
-                    const V u = s_u[c * W + i];
-                    re[i] = u.x;
-                    im[i] = u.y;
-                }
-            }
-        }
-        const bool act = gv != 0.0 && valid && row > pr;
-        T lr = 0, li = 0;
+        if (lr == gr) lr = pr;
+        else if (lr == pr) lr = gr;
+        const bool act = gv != 0.0 && valid && lr > pr;
+        T lre = 0, lim = 0;
         if (c == cj) {
-            // column j after the interchange: the row that moved to gr brings row k+j's value
-            T cr = nxr, ci = nxi;
-            if (own_gr && row == gr) {
-                const V d = s_d[j];
-                cr = d.x;
-                ci = d.y;
-            }
             if (act) {
-                lr = (T)((double)cr * rinv.x - (double)ci * rinv.y);
-                li = (T)((double)cr * rinv.y + (double)ci * rinv.x);
+                lre = (T)((double)nxr * rinv.x - (double)nxi * rinv.y);
+                lim = (T)((double)nxr * rinv.y + (double)nxi * rinv.x);
             }
-            if (TPR > 1) s_l[rb * 64 + lane] = V{lr, li};
+            if (TPR > 1) s_l[rb * 64 + lane] = V{lre, lim};
         }
         if (TPR > 1) {
             __syncthreads();
             if (c != cj) {
                 const V l = s_l[rb * 64 + lane];
-                lr = l.x;
-                li = l.y;
+                lre = l.x;
+                lim = l.y;
             }
         }
 #pragma unroll
@@ -372,14 +353,14 @@ __global__ __launch_bounds__(256, 2) void k_panel(PanelArgs<T> a)
             const int col = c * W + i;  // uniform
             if (col == j) {             // the multiplier replaces column j
                 if (act) {
-                    re[i] = lr;
-                    im[i] = li;
+                    re[i] = lre;
+                    im[i] = lim;
                 }
             } else if (col > j) {
                 if (act) {
                     const V u = s_u[col];
-                    re[i] -= lr * u.x - li * u.y;
-                    im[i] -= lr * u.y + li * u.x;
+                    re[i] -= lre * u.x - lim * u.y;
+                    im[i] -= lre * u.y + lim * u.x;
                 }
                 if (col == j + 1) {
                     nxr = re[i];
@@ -391,9 +372,10 @@ __global__ __launch_bounds__(256, 2) void k_panel(PanelArgs<T> a)
         }
     }
 
+    // each row to its final position
 #pragma unroll
     for (int i = 0; i < W; ++i)
-        if (valid) Ab[row + (size_t)(k + c * W + i) * a.lda] = V{re[i], im[i]};
+        if (valid) Ab[lr + (size_t)(k + c * W + i) * a.lda] = V{re[i], im[i]};
 
     // composite permutation of the panel's interchanges, for the other columns:
     // after applying them in order, row rows[t] holds the old content of src[t].

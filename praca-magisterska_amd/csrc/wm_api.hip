@@ -735,6 +735,8 @@ namespace {
 // Solves the items of one geometry (one per frequency) in batches: FP64 sweeps
 // as wide as the HBM budget allows, mixed sweeps highest frequency first with
 // routing (see below).
+constexpr int kBlindBatch = 16;  // first batch of a mixed sweep (the top quarter of a 64-point sweep)
+
 int run_sweep(wm_ctx *c, const wm_geom *g, const wm_solve_opts &o, std::vector<WorkItem> &items, int nrhs)
 {
     const int nfreq = (int)items.size();
@@ -776,7 +778,11 @@ int run_sweep(wm_ctx *c, const wm_geom *g, const wm_solve_opts &o, std::vector<W
                 }
                 cnt = run;
             } else {
-                cnt = w;
+                // the first batch is factored before any refinement rate of the
+                // structure is known: at most kBlindBatch frequencies from the
+                // top, however wide a launch is, so that frequencies far below
+                // the top are not put into complex64 factors blindly
+                cnt = std::min(w, kBlindBatch);
             }
         }
         const int rc = solve_items(c, o, items.data() + f0, cnt, c->stream);

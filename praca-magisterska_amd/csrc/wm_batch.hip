@@ -284,40 +284,44 @@ double panel_launches(int n, int count, int width)
     return t;
 }
 
+// Cost constants measured on MI355X at N = 8192 (lockstep batches of 48
+// complex128 and 16 complex64 matrices): a panel launch costs 64 columns x
+// 8.2 us (complex128) or x 7.9 us (complex64) whatever the cluster size; the
+// products and interchanges of a factorization run at 76 TF/s (FP64),
+// 122 TF/s (FP32) and 180 TF/s FP32-equivalent (split-BF16x3), all counted at
+// 8 real flops per complex multiply-add; a refinement iteration costs two
+// triangular-solve latency chains per batch (n/64 blocks x 8 us each) plus
+// 24 N^2 bytes per member at about 5 TB/s; about 6 iterations.
+constexpr double kPanelColF64 = 8.2e-6, kPanelColC64 = 7.9e-6;
+constexpr double kRateF64 = 76e12, kRateF32 = 122e12, kRateB3 = 180e12;
+
+double refine_cost(double n, int count)
+{
+    return 6.0 * (2.0 * n / 64 * 8e-6 + count * 24.0 * n * n / 5e12);
+}
+
 bool prefer_f64(wm_ctx *c, int prec, int n, int count)
 {
     if (prec == WM_PREC_F64 || count <= 0) return false;
-    // Measured on MI355X: a panel launch costs about 64 columns x 7.2 us
-    // (complex128) and x 8.5 us (complex64) whatever the cluster size; trailing
-    // updates run at about 72 TF/s (FP64), 108 TF/s (FP32) and 190 TF/s
-    // FP32-equivalent (split-BF16x3), all counted at 8 real flops per complex
-    // product; a refinement iteration costs about 0.1 us per unknown
-    // (triangular-solve latency chain) plus its 24 N^2 bytes; about 6 of them.
     const double nn = (double)n, flu = 8.0 / 3.0 * nn * nn * nn;
     const int w64 = getrf_width(c, WM_PREC_F64, n), w32 = getrf_width(c, prec, n);
-    const double t64 = 64 * 7.2e-6 * panel_launches(n, count, w64) + count * flu / 72e12;
-    const double rate = prec == WM_PREC_BF16X3 ? 190e12 : 108e12;
-    const double tmix = 64 * 8.5e-6 * panel_launches(n, count, w32) +
-                        count * (flu / rate + 6.0 * (1e-7 * nn + 24.0 * nn * nn / 4e12));
+    const double t64 = 64 * kPanelColF64 * panel_launches(n, count, w64) + count * flu / kRateF64;
+    const double rate = prec == WM_PREC_BF16X3 ? kRateB3 : kRateF32;
+    const double tmix = 64 * kPanelColC64 * panel_launches(n, count, w32) + count * flu / rate + refine_cost(nn, count);
     return t64 <= tmix;
 }
 
 bool join_f64(wm_ctx *c, int prec, int n, int r)
 {
     if (prec == WM_PREC_F64 || r <= 0) return false;
-    // Measured on MI355X at N = 8192: complex128 members of a large lockstep
-    // batch cost their product (78 TF/s) plus a share of the panel launches
-    // (n x 7.2 us / (2 x launch width)); a separate complex64 batch pays its
-    // own panel launches (64 columns x 8.5 us each), its products (130 TF/s
-    // FP32, 190 TF/s split-BF16x3) and about 6 refinement iterations of two
-    // triangular-solve latency chains (n/64 blocks x 8 us) plus 24 N^2 bytes
-    // per member.
+    // complex128 members of a large lockstep batch cost their products plus a
+    // share of its panel launches (n x 8.2 us / (2 x launch width)); a separate
+    // complex64 batch pays its own panel launches, products and refinement
     const double nn = (double)n, flu = 8.0 / 3.0 * nn * nn * nn;
     const int w64 = getrf_width(c, WM_PREC_F64, n), w32 = getrf_width(c, prec, n);
-    const double t_join = r * (flu / 78e12 + nn * 7.2e-6 / (2.0 * w64));
-    const double rate = prec == WM_PREC_BF16X3 ? 190e12 : 130e12;
-    const double t_sep = 64 * 8.5e-6 * panel_launches(n, r, w32) + r * flu / rate +
-                         6.0 * (2.0 * nn / 64 * 8e-6 + r * 24.0 * nn * nn / 5e12);
+    const double t_join = r * (flu / kRateF64 + nn * kPanelColF64 / (2.0 * w64));
+    const double rate = prec == WM_PREC_BF16X3 ? kRateB3 : kRateF32;
+    const double t_sep = 64 * kPanelColC64 * panel_launches(n, r, w32) + r * flu / rate + refine_cost(nn, r);
     return t_join <= t_sep;
 }
 

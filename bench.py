@@ -147,7 +147,8 @@ def main():
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported "
-              "(frequencies would shard across GPUs with no exchange, see DESIGN.md)", file=sys.stderr)
+              "(frequencies would shard across GPUs, one process per GPU, with no exchange "
+              "beyond the geometry)", file=sys.stderr)
         sys.exit(2)
 
     wm = _load()

@@ -132,6 +132,7 @@ __global__ __launch_bounds__(256) void k_trsv(TrsvArgs a, int r_off)
     auto blk = [&](int q) { return UPPER ? nblk - 1 - q : q; };
     __shared__ int s_ready;
     int ready = 0;  // blocks q < ready are known to be solved
+    const bool lane16 = row < 16;
     double2 lcur[16], lnext[16];
     if (nb_off > 0) {
         const int c0 = blk(0) * TB + part * 16;
@@ -139,11 +140,6 @@ __global__ __launch_bounds__(256) void k_trsv(TrsvArgs a, int r_off)
         for (int c = 0; c < 16; ++c) lcur[c] = ld2<T>(A + (r0 + row) + (size_t)(c0 + c) * a.lda);
     }
     for (int q = 0; q < nb_off; ++q) {
-        if (q + 1 < nb_off) {
-            const int c1 = blk(q + 1) * TB + part * 16;
-#pragma unroll
-            for (int c = 0; c < 16; ++c) lnext[c] = ld2<T>(A + (r0 + row) + (size_t)(c1 + c) * a.lda);
-        }
         if (q >= ready) {
             // wave 0 polls up to 64 flags per round trip and records the ready run
             if (tid < 64) {
@@ -170,12 +166,26 @@ __global__ __launch_bounds__(256) void k_trsv(TrsvArgs a, int r_off)
             __syncthreads();
             ready = s_ready;
         }
+        // The part's 16 solved values of block q, one per lane, all loads
+        // issued before the next block's L prefetch: the first use waits for
+        // them alone (one memory round trip), not for the prefetch behind them.
         const int c0 = blk(q) * TB + part * 16;
+        double2 yl[NR];
+#pragma unroll
+        for (int r = 0; r < NR; ++r)
+            yl[r] = (lane16 && r < nr) ? Y[(c0 + row) + (size_t)r * a.ldb] : make_double2(0.0, 0.0);
+        __builtin_amdgcn_sched_barrier(0);
+        if (q + 1 < nb_off) {
+            const int c1 = blk(q + 1) * TB + part * 16;
+#pragma unroll
+            for (int c = 0; c < 16; ++c) lnext[c] = ld2<T>(A + (r0 + row) + (size_t)(c1 + c) * a.lda);
+        }
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int c = 0; c < 16; ++c) {
 #pragma unroll
             for (int r = 0; r < NR; ++r)
-                if (r < nr) cfms(acc[r], lcur[c], Y[(c0 + c) + (size_t)r * a.ldb]);
+                if (r < nr) cfms(acc[r], lcur[c], make_double2(bcast(yl[r].x, c), bcast(yl[r].y, c)));
         }
 #pragma unroll
         for (int c = 0; c < 16; ++c) lcur[c] = lnext[c];

@@ -201,7 +201,10 @@ __global__ __launch_bounds__(256) void k_trsv(TrsvArgs a, int r_off)
         for (int r = 0; r < NR; ++r)
 #pragma unroll
             for (int p = 0; p < 3; ++p) { acc[r].x += s_acc[p][row][r].x; acc[r].y += s_acc[p][row][r].y; }
+        // the substitution is the launch's critical path: unrolled, so each
+        // step's LDS read of the block is issued ahead of the step
         if (!UPPER) {
+#pragma unroll
             for (int j = 0; j < TB - 1; ++j) {
                 const double2 l = row > j ? s_d[dl_idx(row, j)] : make_double2(0.0, 0.0);
 #pragma unroll
@@ -211,6 +214,7 @@ __global__ __launch_bounds__(256) void k_trsv(TrsvArgs a, int r_off)
                 }
             }
         } else {
+#pragma unroll
             for (int j = TB - 1; j >= 0; --j) {
                 const double2 u = row < j ? s_d[du_idx(row, j)] : make_double2(0.0, 0.0);
                 const double2 ri = s_rd[j];

@@ -595,9 +595,11 @@ __device__ inline void tile_coords(int t, int TI, int TJ, int &ti, int &tj)
 // four-product form with |a||b| replaced by (|Ar| + |Ai|)(|Br| + |Bi|), at
 // most twice as large.
 // K is staged through LDS in chunks of 8: the smaller chunk's prefetch registers
-// keep the kernel at 152 VGPRs, three waves per SIMD, which hides the LDS and
-// global latency better than two waves with chunks of 16 (83 % against 78 % of
-// the FP64 matrix-core peak issued at K = 512, N = 8192).
+// keep the kernel at three waves per SIMD, which hides the LDS and global
+// latency better than two waves with chunks of 16 (83 % against 78 % of the
+// FP64 matrix-core peak issued at K = 512, N = 8192).  Two chunks are in flight
+// in registers (the next to stage and the one after): 168 VGPRs, four dwords of
+// tile bookkeeping spilled outside the loop, 1 % faster than one chunk.
 constexpr int GZ_BM = 64, GZ_BN = 64, GZ_KC = 8;
 
 __global__ __launch_bounds__(256, 3) void k_gemm_f64(double2 *Abase, int lda, int64_t strideA, GemmArgs g,
@@ -620,12 +622,13 @@ __global__ __launch_bounds__(256, 3) void k_gemm_f64(double2 *Abase, int lda, in
     constexpr int EA = GZ_KC / 4, BJS = 256 / GZ_KC, EB = GZ_BN / BJS;
     __shared__ double2 sA[GZ_KC][GZ_BM + 1];
     __shared__ double2 sB[GZ_KC][GZ_BN + 1];
-    double2 pa[EA], pb[EB];
+    // two chunks in flight in registers: the one to stage next and the one after
+    double2 pa0[EA], pb0[EB], pa1[EA], pb1[EB];
     // chunk elements moved by this thread: L rows ai, k = ak + 4e; U k = bk, columns bj + BJS e
     const int ai = tid % GZ_BM, ak = tid / GZ_BM;
     const int bk = tid % GZ_KC, bj = tid / GZ_KC;
     const bool arow = i0 + ai < mrows;
-    auto fetch = [&](int kc) {
+    auto fetch = [&](int kc, double2 *pa, double2 *pb) {
         const double2 *pA = A + (i0 + ai) + (size_t)(kc + ak) * lda;
 #pragma unroll
         for (int e = 0; e < EA; ++e) pa[e] = arow ? pA[(size_t)(4 * e) * lda] : double2{0.0, 0.0};
@@ -633,7 +636,7 @@ __global__ __launch_bounds__(256, 3) void k_gemm_f64(double2 *Abase, int lda, in
 #pragma unroll
         for (int e = 0; e < EB; ++e) pb[e] = (j0 + bj + BJS * e < ncols) ? pB[(size_t)(BJS * e) * lda] : double2{0.0, 0.0};
     };
-    auto stash = [&]() {
+    auto stash = [&](const double2 *pa, const double2 *pb) {
 #pragma unroll
         for (int e = 0; e < EA; ++e) sA[ak + 4 * e][ai] = pa[e];
 #pragma unroll
@@ -644,12 +647,7 @@ __global__ __launch_bounds__(256, 3) void k_gemm_f64(double2 *Abase, int lda, in
     for (int x = 0; x < 2; ++x)
 #pragma unroll
         for (int y = 0; y < 2; ++y) { P[x][y] = v4d{0, 0, 0, 0}; Q[x][y] = P[x][y]; S[x][y] = P[x][y]; }
-
-    fetch(0);
-    stash();
-    __syncthreads();
-    for (int kc = 0; kc < K; kc += GZ_KC) {
-        if (kc + GZ_KC < K) fetch(kc + GZ_KC);
+    auto compute = [&]() {
 #pragma unroll
         for (int ks = 0; ks < GZ_KC; ks += 4) {
             const int kk = ks + lane / 16;
@@ -675,9 +673,25 @@ __global__ __launch_bounds__(256, 3) void k_gemm_f64(double2 *Abase, int lda, in
 #pragma unroll
                 for (int ti = 0; ti < 2; ++ti) S[tj][ti] = __builtin_amdgcn_mfma_f64_16x16x4f64(bs[tj], as[ti], S[tj][ti], 0, 0, 0);
         }
-        if (kc + GZ_KC < K) {
+    };
+
+    // K is a multiple of 2 chunks: the loop takes two chunks per trip, chunk
+    // kc from LDS while chunk kc + GZ_KC waits in pa1/pb1 and kc + 2 GZ_KC loads
+    fetch(0, pa0, pb0);
+    stash(pa0, pb0);
+    fetch(GZ_KC, pa1, pb1);
+    __syncthreads();
+    for (int kc = 0; kc < K; kc += 2 * GZ_KC) {
+        if (kc + 2 * GZ_KC < K) fetch(kc + 2 * GZ_KC, pa0, pb0);
+        compute();
+        __syncthreads();
+        stash(pa1, pb1);
+        __syncthreads();
+        if (kc + 3 * GZ_KC < K) fetch(kc + 3 * GZ_KC, pa1, pb1);
+        compute();
+        if (kc + 2 * GZ_KC < K) {
             __syncthreads();
-            stash();
+            stash(pa0, pb0);
             __syncthreads();
         }
     }

@@ -249,27 +249,28 @@ __global__ __launch_bounds__(256, 3) void k_panel(PanelArgs<T> a)
         }
         __syncthreads();
 
-        // 2. this workgroup's candidate; stage its row in LDS
+        // 2. this workgroup's candidate: its slot is published at once, so the
+        //    other workgroups' reductions start while the row is staged in LDS
         double wv = s_wv[0];
         int wr = s_wr[0];
 #pragma unroll
         for (int q = 1; q < NWC; ++q)
             if (better(s_wv[q], s_wr[q], wv, wr)) { wv = s_wv[q]; wr = s_wr[q]; }
+        uint64_t *slot = X.slot(xm, par);
+        uint64_t *cand = X.cand(xm, par);
+        if (tid < 3) {
+            const uint64_t bits = (uint64_t)__double_as_longlong(wv);
+            const uint32_t word = tid == 0 ? (uint32_t)bits : tid == 1 ? (uint32_t)(bits >> 32) : (uint32_t)wr;
+            put_pkt(slot + (size_t)w * 4 + tid, tag, word);
+        }
         if (wr != kNone && lr == wr) {
 #pragma unroll
             for (int i = 0; i < W; ++i) s_cand[c * W + i] = V{re[i], im[i]};
         }
         __syncthreads();
 
-        // 3. publish
-        uint64_t *slot = X.slot(xm, par);
-        uint64_t *cand = X.cand(xm, par);
+        // 3. publish the candidate row
         if (tid < NW) put_pkt(cand + (size_t)w * NW + tid, tag, ((const uint32_t *)s_cand)[tid]);
-        if (tid < 3) {
-            const uint64_t bits = (uint64_t)__double_as_longlong(wv);
-            const uint32_t word = tid == 0 ? (uint32_t)bits : tid == 1 ? (uint32_t)(bits >> 32) : (uint32_t)wr;
-            put_pkt(slot + (size_t)w * 4 + tid, tag, word);
-        }
 
         // 4. every workgroup reduces all candidates to the same pivot (first
         //    max) and notes the workgroup that holds it

@@ -195,7 +195,7 @@ __global__ __launch_bounds__(256, 3) void k_panel(PanelArgs<T> a)
     constexpr int RW = 64 * NWC;    // rows per workgroup
     constexpr int NW = XLayout<V>::NW;
     constexpr int kNone = 0x7fffffff;
-    static_assert(NW <= 256 && RW >= NB, "panel layout");
+    static_assert(NW <= 256 && NW % 64 == 0 && RW >= NB, "panel layout");
     const int bl = blockIdx.y, w = blockIdx.x;          // bl: member of this launch
     const int b = a.bidx ? a.bidx[bl] : bl;              // b: matrix index
     const int tid = threadIdx.x, lane = tid & 63;
@@ -223,7 +223,8 @@ __global__ __launch_bounds__(256, 3) void k_panel(PanelArgs<T> a)
     __shared__ V s_u[NB];     // pivot row
     __shared__ V s_l[TPR > 1 ? RW : 1];  // multipliers for the other chunks
     __shared__ double s_gv;
-    __shared__ int s_gr, s_gw;
+    __shared__ int s_gr;
+    __shared__ double2 s_rinv;  // reciprocal of the pivot
     __shared__ int s_piv[NB];  // pivot row of each column (workgroup 0)
 
     const uint32_t tag0 = *a.tagp + a.toff;
@@ -272,8 +273,9 @@ __global__ __launch_bounds__(256, 3) void k_panel(PanelArgs<T> a)
         // 3. publish the candidate row
         if (tid < NW) put_pkt(cand + (size_t)w * NW + tid, tag, ((const uint32_t *)s_cand)[tid]);
 
-        // 4. every workgroup reduces all candidates to the same pivot (first
-        //    max) and notes the workgroup that holds it
+        // 4. wave 0 reduces all candidates to the same pivot (first max), then
+        //    fetches the pivot row from the workgroup holding it and takes the
+        //    pivot's reciprocal, so one barrier hands everything to the others
         if (wave == 0) {
             double gv = -1.0;
             int gr = kNone, gs = 0;
@@ -297,34 +299,37 @@ __global__ __launch_bounds__(256, 3) void k_panel(PanelArgs<T> a)
             const int r = wave_min_i32(gv == m ? gr : kNone);
             const unsigned long long win = __ballot(gv == m && gr == r);
             const int ws = __builtin_amdgcn_readlane(gs, win ? (int)__builtin_ctzll(win) : 0);
+            // no usable candidate (non-finite column): a zero pivot in place
+            const bool ok = r >= pr && r < a.n;
+            constexpr int PL = NW / 64;  // pivot-row words per lane, loaded together
+            const uint64_t *pu = cand + (size_t)(ok ? ws : 0) * NW + lane;
+            uint64_t wu[PL];
+            long it = 0;
+            for (;;) {
+                bool got = true;
+#pragma unroll
+                for (int e = 0; e < PL; ++e) {
+                    wu[e] = ld_pkt(pu + 64 * e);
+                    got = got && (uint32_t)(wu[e] >> 32) == tag;
+                }
+                if (got) break;
+                if (spin_give_up(it, a.err)) break;
+            }
+#pragma unroll
+            for (int e = 0; e < PL; ++e) ((uint32_t *)s_u)[lane + 64 * e] = (uint32_t)wu[e];
             if (lane == 0) {
-                // no usable candidate (non-finite column): a zero pivot in place
-                const bool ok = r >= pr && r < a.n;
                 s_gv = ok ? m : 0.0;
                 s_gr = ok ? r : pr;
-                s_gw = ok ? ws : 0;
+                const V p = s_u[j];  // written by this wave above
+                s_rinv = ok && m != 0.0 ? crecip((double)p.x, (double)p.y) : make_double2(0.0, 0.0);
             }
         }
         __syncthreads();
         const double gv = s_gv;
         const int gr = s_gr;
-        const int gw = s_gw;
-        if (tid < NW) {
-            const uint64_t *pu = cand + (size_t)gw * NW + tid;
-            uint64_t wu;
-            long it = 0;
-            for (;;) {
-                wu = ld_pkt(pu);
-                if ((uint32_t)(wu >> 32) == tag) break;
-                if (spin_give_up(it, a.err)) break;
-            }
-            ((uint32_t *)s_u)[tid] = (uint32_t)wu;
-        }
-        __syncthreads();
 
         // 5. interchange positions k+j and gr, scale column j, rank-1 update
-        const V p = s_u[j];
-        const double2 rinv = (gv != 0.0) ? crecip((double)p.x, (double)p.y) : make_double2(0.0, 0.0);
+        const double2 rinv = s_rinv;
         if (w == 0 && tid == 0) {
             s_piv[j] = gr;
             a.ipiv[(size_t)b * a.strideP + pr] = gr + 1;

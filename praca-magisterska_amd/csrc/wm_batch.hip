@@ -286,13 +286,13 @@ double panel_launches(int n, int count, int width)
 
 // Cost constants measured on MI355X at N = 8192 (lockstep batches of 48
 // complex128 and 16 complex64 matrices): a panel launch costs 64 columns x
-// 8.2 us (complex128) or x 7.9 us (complex64) whatever the cluster size; the
+// 7.4 us (complex128) or x 7.3 us (complex64) whatever the cluster size; the
 // products and interchanges of a factorization run at 76 TF/s (FP64),
 // 122 TF/s (FP32) and 180 TF/s FP32-equivalent (split-BF16x3), all counted at
 // 8 real flops per complex multiply-add; a refinement iteration costs two
 // triangular-solve latency chains per batch (n/64 blocks x 8 us each) plus
 // 24 N^2 bytes per member at about 5 TB/s; about 6 iterations.
-constexpr double kPanelColF64 = 8.2e-6, kPanelColC64 = 7.9e-6;
+constexpr double kPanelColF64 = 7.4e-6, kPanelColC64 = 7.3e-6;
 constexpr double kRateF64 = 76e12, kRateF32 = 122e12, kRateB3 = 180e12;
 
 double refine_cost(double n, int count)
@@ -315,7 +315,7 @@ bool join_f64(wm_ctx *c, int prec, int n, int r)
 {
     if (prec == WM_PREC_F64 || r <= 0) return false;
     // complex128 members of a large lockstep batch cost their products plus a
-    // share of its panel launches (n x 8.2 us / (2 x launch width)); a separate
+    // share of its panel launches (n x 7.4 us / (2 x launch width)); a separate
     // complex64 batch pays its own panel launches, products and refinement
     const double nn = (double)n, flu = 8.0 / 3.0 * nn * nn * nn;
     const int w64 = getrf_width(c, WM_PREC_F64, n), w32 = getrf_width(c, prec, n);

@@ -16,53 +16,34 @@ template <> struct CT<float> { using V = float2; };
 // leaves once any wave has recorded a timeout in the error word.
 constexpr long kSpinLimit = 1L << 24;
 
-// Arrive at and wait on a counter shared by `members` workgroups.  `target`
-// is the counter value once everyone has arrived at this barrier instance.
-// Thread 0 of the block does the atomics; the block is fenced on both sides.
-__device__ inline void cluster_barrier(int *cnt, int target, int *err)
+// Tagged 8-byte packets {32-bit tag, 32-bit payload} for data handed between
+// workgroups of one grid: a reader polls a packet until its tag matches, so
+// neither a counter nor a release/acquire fence (an L2 write-back and
+// invalidate on gfx950) is needed.  8-byte loads and stores are single-copy
+// atomic; agent-scope relaxed atomics reach the device coherence point.
+__device__ inline void put_pkt(uint64_t *p, uint32_t tag, uint32_t v)
 {
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __hip_atomic_fetch_add(cnt, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        long it = 0;
-        while (__hip_atomic_load(cnt, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
-            if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
-            if (++it > kSpinLimit) {
-                __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                break;
-            }
-            __builtin_amdgcn_s_sleep(2);
-        }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    }
-    __syncthreads();
+    __hip_atomic_store(p, ((uint64_t)tag << 32) | v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Wait until *flag >= want (flag written by an earlier-dispatched workgroup
-// of the same grid).  Bounded like cluster_barrier.
-__device__ inline void wait_flag(const int *flag, int want, int *err)
+__device__ inline uint64_t ld_pkt(const uint64_t *p)
 {
-    if (threadIdx.x == 0) {
-        long it = 0;
-        while (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < want) {
-            if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
-            if (++it > kSpinLimit) {
-                __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                break;
-            }
-            __builtin_amdgcn_s_sleep(1);
-        }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    }
-    __syncthreads();
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Publish *flag = value after this block's prior global stores.
-__device__ inline void set_flag(int *flag, int value)
+// Bounded-wait bookkeeping for a polling lane: true if the lane must stop
+// (the error word is set, or this lane ran out of spins and set it).
+__device__ inline bool spin_give_up(long &it, int *err)
 {
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) __hip_atomic_store(flag, value, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if ((++it & 255) == 0) {
+        if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return true;
+        if (it > kSpinLimit) {
+            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return true;
+        }
+    }
+    __builtin_amdgcn_s_sleep(1);
+    return false;
 }
 
 // Smith (1962) reciprocal of a complex number.

@@ -83,11 +83,11 @@ struct wm_ctx {
     wm::DevBuf ws_fill;     // per-frequency self-term integrals
     wm::DevBuf ws_lu;       // panel exchange buffers
     wm::DevBuf ws_solve;    // triangular-solve permutation and staging
-    wm::DevBuf ws_flags;    // triangular-solve completion flags (epoch tags only)
+    wm::DevBuf ws_flags;    // triangular solves: solved values as packets (epoch tags only)
     wm::DevBuf ws_perm;     // row permutations for getrs on raw pivots
     wm::DevBuf ws_resid;    // residual partial sums
     wm::DevBuf ws_sweep;    // matrices and vectors of a sweep batch
-    int solve_epoch = 0;    // monotonic tag of triangular-solve flags
+    int solve_epoch = 0;    // monotonic tag of triangular-solve packets
     uint64_t panel_tag = 0; // packet tags used since the exchange buffer was last cleared
     uint32_t *d_tag = nullptr;  // device tag base, bumped by every factorization launch chunk
     bool use_graphs = true;     // replay captured factorizations (wm_set_graphs)

@@ -85,31 +85,6 @@ struct XLayout {
     __device__ int *perm(char *m) const { return (int *)(m + perm_off); }
 };
 
-__device__ inline void put_pkt(uint64_t *p, uint32_t tag, uint32_t v)
-{
-    __hip_atomic_store(p, ((uint64_t)tag << 32) | v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ inline uint64_t ld_pkt(const uint64_t *p)
-{
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Bounded-wait bookkeeping for a polling lane: true if the lane must stop
-// (the error word is set, or this lane ran out of spins and set it).
-__device__ inline bool spin_give_up(long &it, int *err)
-{
-    if ((++it & 255) == 0) {
-        if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return true;
-        if (it > kSpinLimit) {
-            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            return true;
-        }
-    }
-    __builtin_amdgcn_s_sleep(1);
-    return false;
-}
-
 template <typename T>
 struct PanelArgs {
     typename CT<T>::V *A;

@@ -6,12 +6,12 @@
 //
 // Each triangular solve is one launch.  Workgroup i owns row block i (64
 // rows).  It subtracts the contributions of the blocks already solved, as
-// each one becomes ready (a flag per block, set after its values are
-// stored), and then solves its diagonal block inside one wave with lane
-// broadcasts.  A workgroup only ever waits on blocks owned by workgroups
-// with a smaller id in the same grid, which are dispatched before it, so the
-// chain always makes progress; every wait is bounded.  Arithmetic is FP64;
-// complex64 factors are widened on load.
+// each one becomes ready (its values arrive as tagged packets, written by the
+// block's workgroup after its solve), and then solves its diagonal block
+// inside one wave with lane broadcasts.  A workgroup only ever waits on
+// blocks owned by workgroups with a smaller id in the same grid, which are
+// dispatched before it, so the chain always makes progress; every wait is
+// bounded.  Arithmetic is FP64; complex64 factors are widened on load.
 #include "wm_cplx.h"
 #include "wm_internal.h"
 
@@ -64,8 +64,8 @@ struct TrsvArgs {
     int ldb;
     int64_t strideB;
     double2 *Y;           // output (lower: y, upper: x in place of y)
-    int *flags;           // nblk per launch member
-    int epoch;
+    uint64_t *pkt;        // solved values as tagged packets: per member [block][rhs][row][4 words]
+    int epoch;            // packet tag of this launch
     int *err;
     const int *bidx;      // launch member -> matrix index (null: identity)
 };
@@ -78,6 +78,13 @@ __device__ inline double bcast(double v, int lane)  // value of `lane` to the wh
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
+// The double whose low and high words lanes `l` and `l + 1` hold, to the whole wave.
+__device__ inline double word_pair(uint32_t w, int l)
+{
+    const unsigned lo = __builtin_amdgcn_readlane(w, l), hi = __builtin_amdgcn_readlane(w, l + 1);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // Packed index of element (i, j) of the 64 x 64 diagonal block: strictly
 // lower part for L (i > j), upper part with the diagonal for U (i <= j).
 __device__ inline int dl_idx(int i, int j) { return i * (i - 1) / 2 + j; }
@@ -85,10 +92,11 @@ __device__ inline int du_idx(int i, int j) { return j * (j + 1) / 2 + i; }
 
 // Triangular solve, one launch.  Workgroup (member, g) owns row block bi of
 // that launch member (lower: g, upper: nblk-1-g); lower solves L y = P b,
-// upper solves U x = y in place.  The grid is (members, blocks), so workgroup
-// ids run over the members first: the resident workgroups hold the leading
-// blocks of every member and all members' chains advance together.  Waits
-// only on blocks of workgroups with smaller ids (dispatched earlier).
+// upper solves U x = y in place, and also publishes each solved value as four
+// tagged packets for the later blocks.  The grid is (members, blocks), so
+// workgroup ids run over the members first: the resident workgroups hold the
+// leading blocks of every member and all members' chains advance together.
+// Waits only on blocks of workgroups with smaller ids (dispatched earlier).
 template <typename T, int NR, bool UPPER>
 __global__ __launch_bounds__(256) void k_trsv(TrsvArgs a, int r_off)
 {
@@ -100,7 +108,8 @@ __global__ __launch_bounds__(256) void k_trsv(TrsvArgs a, int r_off)
     const int bi = UPPER ? nblk - 1 - (int)blockIdx.y : (int)blockIdx.y, r0 = bi * TB;
     const V *A = (const V *)a.A + (size_t)b * a.strideA;
     double2 *Y = a.Y + (size_t)b * a.strideB + (size_t)r_off * a.ldb;
-    int *flags = a.flags + (size_t)bl * nblk;
+    uint64_t *pk = a.pkt + (size_t)bl * nblk * NR * 256;
+    const uint32_t tag = (uint32_t)a.epoch;
     const int nr = a.nrhs - r_off;
 
     // diagonal block to LDS (independent of other blocks, so before any wait)
@@ -130,9 +139,6 @@ __global__ __launch_bounds__(256) void k_trsv(TrsvArgs a, int r_off)
     // off-diagonal blocks in dependency order, next block prefetched
     const int nb_off = UPPER ? nblk - 1 - bi : bi;
     auto blk = [&](int q) { return UPPER ? nblk - 1 - q : q; };
-    __shared__ int s_ready;
-    int ready = 0;  // blocks q < ready are known to be solved
-    const bool lane16 = row < 16;
     double2 lcur[16], lnext[16];
     if (nb_off > 0) {
         const int c0 = blk(0) * TB + part * 16;
@@ -140,40 +146,27 @@ __global__ __launch_bounds__(256) void k_trsv(TrsvArgs a, int r_off)
         for (int c = 0; c < 16; ++c) lcur[c] = ld2<T>(A + (r0 + row) + (size_t)(c0 + c) * a.lda);
     }
     for (int q = 0; q < nb_off; ++q) {
-        if (q >= ready) {
-            // wave 0 polls up to 64 flags per round trip and records the ready run
-            if (tid < 64) {
-                long it = 0;
-                int run = 0;
-                for (;;) {
-                    const int qq = q + tid;
-                    int f = a.epoch;
-                    if (qq < nb_off) f = __hip_atomic_load(flags + blk(qq), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    const unsigned long long m = __ballot(f >= a.epoch);
-                    run = (~m == 0ull) ? 64 : (int)__builtin_ctzll(~m);  // leading ready lanes
-                    if (run > 0) break;
-                    if (__hip_atomic_load(a.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) { run = nb_off; break; }
-                    if (++it > kSpinLimit) {
-                        if (tid == 0) __hip_atomic_store(a.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        run = nb_off;
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(1);
-                }
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-                if (tid == 0) s_ready = q + run;
-            }
-            __syncthreads();
-            ready = s_ready;
-        }
-        // The part's 16 solved values of block q, one per lane, all loads
-        // issued before the next block's L prefetch: the first use waits for
-        // them alone (one memory round trip), not for the prefetch behind them.
-        const int c0 = blk(q) * TB + part * 16;
-        double2 yl[NR];
+        // The part's 16 solved values of block q: 64 packets per right-hand
+        // side, one per lane, polled until the block's workgroup has tagged
+        // them (a value and its readiness in one memory round trip), and all
+        // loaded before the next block's L prefetch.
+        const uint64_t *pq = pk + (size_t)blk(q) * NR * 256 + part * 64 + row;
+        uint32_t yw[NR];
+        long it = 0;
+        for (;;) {
+            bool got = true;
 #pragma unroll
-        for (int r = 0; r < NR; ++r)
-            yl[r] = (lane16 && r < nr) ? Y[(c0 + row) + (size_t)r * a.ldb] : make_double2(0.0, 0.0);
+            for (int r = 0; r < NR; ++r) {
+                yw[r] = 0u;
+                if (r < nr) {
+                    const uint64_t v = ld_pkt(pq + r * 256);
+                    yw[r] = (uint32_t)v;
+                    got = got && (uint32_t)(v >> 32) == tag;
+                }
+            }
+            if (__all(got)) break;
+            if (spin_give_up(it, a.err)) break;
+        }
         __builtin_amdgcn_sched_barrier(0);
         if (q + 1 < nb_off) {
             const int c1 = blk(q + 1) * TB + part * 16;
@@ -185,7 +178,7 @@ __global__ __launch_bounds__(256) void k_trsv(TrsvArgs a, int r_off)
         for (int c = 0; c < 16; ++c) {
 #pragma unroll
             for (int r = 0; r < NR; ++r)
-                if (r < nr) cfms(acc[r], lcur[c], make_double2(bcast(yl[r].x, c), bcast(yl[r].y, c)));
+                if (r < nr) cfms(acc[r], lcur[c], make_double2(word_pair(yw[r], 4 * c), word_pair(yw[r], 4 * c + 2)));
         }
 #pragma unroll
         for (int c = 0; c < 16; ++c) lcur[c] = lnext[c];
@@ -231,9 +224,16 @@ __global__ __launch_bounds__(256) void k_trsv(TrsvArgs a, int r_off)
         }
 #pragma unroll
         for (int r = 0; r < NR; ++r)
-            if (r < nr) Y[(r0 + row) + (size_t)r * a.ldb] = acc[r];
+            if (r < nr) {
+                Y[(r0 + row) + (size_t)r * a.ldb] = acc[r];
+                uint64_t *pw = pk + ((size_t)bi * NR + r) * 256 + row * 4;
+                const uint64_t xb = (uint64_t)__double_as_longlong(acc[r].x), yb = (uint64_t)__double_as_longlong(acc[r].y);
+                put_pkt(pw, tag, (uint32_t)xb);
+                put_pkt(pw + 1, tag, (uint32_t)(xb >> 32));
+                put_pkt(pw + 2, tag, (uint32_t)yb);
+                put_pkt(pw + 3, tag, (uint32_t)(yb >> 32));
+            }
     }
-    set_flag(flags + bi, a.epoch);
 }
 
 // Partial products P[split][row][r] = sum_{j in split} Z[row][j] X[j][r].
@@ -334,9 +334,9 @@ __global__ void k_stage(const double2 *B, int ldb, int64_t strideB, int n, int n
     for (int r = 0; r < nrhs; ++r) S[(size_t)bl * strideB + (size_t)r * ldb + i] = B[(size_t)b * strideB + (size_t)r * ldb + i];
 }
 
-__global__ void k_zero_flags(int *p, int n)
+__global__ void k_zero_flags(int *p, int64_t n)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
 }
 
@@ -354,8 +354,8 @@ int perm_batched(wm_ctx *ctx, int n, const int *ipiv, int64_t strideP, int *perm
     return WM_OK;
 }
 
-// Staging of B per launch member in ctx->ws_solve; completion flags in
-// ctx->ws_flags, 2 x batch x n/64 epoch tags.
+// Staging of B per launch member in ctx->ws_solve; the solved values'
+// packets in ctx->ws_flags, per triangle and member nblk x NR x 256 packets.
 int getrs_perm_batched(wm_ctx *ctx, int prec, int n, int nrhs, const void *A, int lda, int64_t strideA,
                        const int *perm, int64_t strideQ, void *B, int ldb, int64_t strideB, int batch, hipStream_t s,
                        const int *d_bidx)
@@ -367,20 +367,22 @@ int getrs_perm_batched(wm_ctx *ctx, int prec, int n, int nrhs, const void *A, in
     const int nblk = n / TB;
     WM_TRY(ctx->ws_solve.ensure((size_t)batch * strideB * sizeof(double2)));
     double2 *Ystage = (double2 *)ctx->ws_solve.ptr;
-    // Completion flags live in their own buffer and only ever hold epoch tags
-    // of this context, so a wait can never be satisfied by stale data.  The
+    // The packets live in their own buffer and only ever carry epoch tags of
+    // this context, so a wait can never be satisfied by stale data.  The
     // buffer is zeroed whenever it is (re)allocated.
-    const size_t flag_bytes = (size_t)2 * batch * nblk * sizeof(int);
-    if (ctx->ws_flags.bytes < flag_bytes || !ctx->ws_flags.ptr) {
-        WM_TRY(ctx->ws_flags.ensure(flag_bytes));
-        const int nf = (int)(ctx->ws_flags.bytes / sizeof(int));
-        hipLaunchKernelGGL(k_zero_flags, dim3((nf + 255) / 256), dim3(256), 0, s, (int *)ctx->ws_flags.ptr, nf);
+    const int nr_max = rhs_chunk(nrhs);
+    const size_t tri_pkts = (size_t)batch * nblk * nr_max * 256;  // packets per triangle
+    const size_t pkt_bytes = 2 * tri_pkts * sizeof(uint64_t);
+    if (ctx->ws_flags.bytes < pkt_bytes || !ctx->ws_flags.ptr) {
+        WM_TRY(ctx->ws_flags.ensure(pkt_bytes));
+        const int64_t nf = (int64_t)(ctx->ws_flags.bytes / sizeof(int));
+        hipLaunchKernelGGL(k_zero_flags, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, s, (int *)ctx->ws_flags.ptr, nf);
         ctx->solve_epoch = 0;
     }
-    int *flags = (int *)ctx->ws_flags.ptr;
-    if (ctx->solve_epoch > (1 << 30)) {  // keep the tags positive: restart from a zeroed buffer
-        const int nf = (int)(ctx->ws_flags.bytes / sizeof(int));
-        hipLaunchKernelGGL(k_zero_flags, dim3((nf + 255) / 256), dim3(256), 0, s, flags, nf);
+    uint64_t *pkt = (uint64_t *)ctx->ws_flags.ptr;
+    if (ctx->solve_epoch > (1 << 30)) {  // keep the tags unique: restart from a zeroed buffer
+        const int64_t nf = (int64_t)(ctx->ws_flags.bytes / sizeof(int));
+        hipLaunchKernelGGL(k_zero_flags, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, s, (int *)pkt, nf);
         ctx->solve_epoch = 0;
     }
     // the lower solve reads B (staged per member) and writes y into B; the upper solves in place
@@ -391,7 +393,7 @@ int getrs_perm_batched(wm_ctx *ctx, int prec, int n, int nrhs, const void *A, in
         a.A = A; a.lda = lda; a.strideA = strideA; a.n = n; a.nrhs = nrhs;
         a.perm = perm; a.strideQ = strideQ; a.ldb = ldb; a.strideB = strideB; a.err = ctx->d_err;
         a.bidx = d_bidx;
-        a.Bin = Ystage; a.Y = (double2 *)B; a.flags = flags;
+        a.Bin = Ystage; a.Y = (double2 *)B; a.pkt = pkt;
         a.epoch = ++ctx->solve_epoch;
         const int nr = rhs_chunk(nrhs - r0);
         const double esz = prec == WM_PREC_F64 ? 16.0 : 8.0;
@@ -400,7 +402,7 @@ int getrs_perm_batched(wm_ctx *ctx, int prec, int n, int nrhs, const void *A, in
         if (prec == WM_PREC_F64) trsv_dispatch<double>(a, batch, r0, false, s);
         else trsv_dispatch<float>(a, batch, r0, false, s);
         prof_end(ctx, K_TRSV, s, fl, by);
-        a.Bin = nullptr; a.flags = flags + (size_t)batch * nblk;
+        a.Bin = nullptr; a.pkt = pkt + tri_pkts;
         prof_begin(ctx, K_TRSV, s);
         if (prec == WM_PREC_F64) trsv_dispatch<double>(a, batch, r0, true, s);
         else trsv_dispatch<float>(a, batch, r0, true, s);

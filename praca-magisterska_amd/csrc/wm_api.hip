@@ -258,7 +258,7 @@ int wm_shutdown(wm_ctx *c)
     c->ws_fill.release();
     c->ws_lu.release();
     c->ws_solve.release();
-    c->ws_flags.release();
+    c->ws_pkt.release();
     c->ws_perm.release();
     c->ws_resid.release();
     c->ws_sweep.release();

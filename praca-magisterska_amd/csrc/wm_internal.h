@@ -83,7 +83,7 @@ struct wm_ctx {
     wm::DevBuf ws_fill;     // per-frequency self-term integrals
     wm::DevBuf ws_lu;       // panel exchange buffers
     wm::DevBuf ws_solve;    // triangular-solve permutation and staging
-    wm::DevBuf ws_flags;    // triangular solves: solved values as packets (epoch tags only)
+    wm::DevBuf ws_pkt;    // triangular solves: solved values as packets (epoch tags only)
     wm::DevBuf ws_perm;     // row permutations for getrs on raw pivots
     wm::DevBuf ws_resid;    // residual partial sums
     wm::DevBuf ws_sweep;    // matrices and vectors of a sweep batch

@@ -334,7 +334,7 @@ __global__ void k_stage(const double2 *B, int ldb, int64_t strideB, int n, int n
     for (int r = 0; r < nrhs; ++r) S[(size_t)bl * strideB + (size_t)r * ldb + i] = B[(size_t)b * strideB + (size_t)r * ldb + i];
 }
 
-__global__ void k_zero_flags(int *p, int64_t n)
+__global__ void k_zero_words(int *p, int64_t n)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
@@ -355,7 +355,7 @@ int perm_batched(wm_ctx *ctx, int n, const int *ipiv, int64_t strideP, int *perm
 }
 
 // Staging of B per launch member in ctx->ws_solve; the solved values'
-// packets in ctx->ws_flags, per triangle and member nblk x NR x 256 packets.
+// packets in ctx->ws_pkt, per triangle and member nblk x NR x 256 packets.
 int getrs_perm_batched(wm_ctx *ctx, int prec, int n, int nrhs, const void *A, int lda, int64_t strideA,
                        const int *perm, int64_t strideQ, void *B, int ldb, int64_t strideB, int batch, hipStream_t s,
                        const int *d_bidx)
@@ -373,16 +373,16 @@ int getrs_perm_batched(wm_ctx *ctx, int prec, int n, int nrhs, const void *A, in
     const int nr_max = rhs_chunk(nrhs);
     const size_t tri_pkts = (size_t)batch * nblk * nr_max * 256;  // packets per triangle
     const size_t pkt_bytes = 2 * tri_pkts * sizeof(uint64_t);
-    if (ctx->ws_flags.bytes < pkt_bytes || !ctx->ws_flags.ptr) {
-        WM_TRY(ctx->ws_flags.ensure(pkt_bytes));
-        const int64_t nf = (int64_t)(ctx->ws_flags.bytes / sizeof(int));
-        hipLaunchKernelGGL(k_zero_flags, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, s, (int *)ctx->ws_flags.ptr, nf);
+    if (ctx->ws_pkt.bytes < pkt_bytes || !ctx->ws_pkt.ptr) {
+        WM_TRY(ctx->ws_pkt.ensure(pkt_bytes));
+        const int64_t nf = (int64_t)(ctx->ws_pkt.bytes / sizeof(int));
+        hipLaunchKernelGGL(k_zero_words, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, s, (int *)ctx->ws_pkt.ptr, nf);
         ctx->solve_epoch = 0;
     }
-    uint64_t *pkt = (uint64_t *)ctx->ws_flags.ptr;
+    uint64_t *pkt = (uint64_t *)ctx->ws_pkt.ptr;
     if (ctx->solve_epoch > (1 << 30)) {  // keep the tags unique: restart from a zeroed buffer
-        const int64_t nf = (int64_t)(ctx->ws_flags.bytes / sizeof(int));
-        hipLaunchKernelGGL(k_zero_flags, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, s, (int *)pkt, nf);
+        const int64_t nf = (int64_t)(ctx->ws_pkt.bytes / sizeof(int));
+        hipLaunchKernelGGL(k_zero_words, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, s, (int *)pkt, nf);
         ctx->solve_epoch = 0;
     }
     // the lower solve reads B (staged per member) and writes y into B; the upper solves in place

@@ -338,7 +338,12 @@ __global__ __launch_bounds__(256, 3) void k_panel(PanelArgs<T> a)
                     im[i] = lim;
                 }
             } else if (col > j) {
-                if (act) {
+                // complex64 factors: rows that take no update have a zero
+                // multiplier, so the update needs no per-row branch (a
+                // non-finite pivot row then spreads to finished rows, and the
+                // refinement of such factors fails over to FP64 factors);
+                // complex128 factors keep LAPACK's untouched finished rows
+                if (sizeof(T) == 4 || act) {
                     const V u = s_u[col];
                     re[i] -= lre * u.x - lim * u.y;
                     im[i] -= lre * u.y + lim * u.x;

@@ -157,6 +157,18 @@ typedef struct {
     double hbm_fraction;   /* share of free HBM the sweep may use, default 0.8 */
     int fallback_f64;      /* refactor in FP64 if refinement fails, default 1 */
     int size_policy;       /* factor a batch in FP64 where that is cheaper at its size, default 1 */
+    int cond_est;          /* estimate the 1-norm condition number of every Z (wm_freq_stats.cond1),
+                              default 0: Hager-Higham's estimator (LAPACK zlacon's algorithm)
+                              with the final factors, about ten extra triangular solves per batch */
+    double recycle;        /* recycled sweep, default 0 (off).  A value r > 1 factors (in FP64) only
+                              "anchor" frequencies, chosen so that every frequency f of the sweep has
+                              an anchor within a factor r(f) of it -- r while the structure is
+                              electrically small (k D <= 1, D its diameter), 1 + (r - 1) / (k D)
+                              above -- and solves each other frequency by FP64 GMRES iterative
+                              refinement preconditioned with its anchor's factors, until the
+                              correction is below tol or stops shrinking at the FP64 floor; a frequency
+                              whose refinement does not converge is factored itself.  prec is
+                              ignored. */
 } wm_solve_opts;
 
 void wm_solve_opts_default(wm_solve_opts *o);
@@ -166,11 +178,22 @@ typedef struct {
     int iters;             /* refinement iterations (0 for FP64) */
     int info;              /* LU info */
     int fell_back;         /* FP64 factors produced the result: 1 after refinement failed,
-                              2 routed there directly (predicted not to refine) */
+                              2 routed there directly (predicted not to refine); -1: refinement
+                              failed with fallback_f64 off (the result is its last iterate) */
     int prec;              /* WM_PREC_* of the factors that produced the result */
     double rate;           /* last refinement contraction ||dx_k|| / ||dx_k-1|| (-1: none) */
     double dx_rel;         /* last ||dx||_inf / ||x||_inf: the refined result's relative
                               distance from the FP64 solution is about dx_rel * rate / (1 - rate) */
+    double cond1;          /* with cond_est: estimate of ||Z||_1 ||Z^-1||_1 (a lower bound, in
+                              practice within a factor 3), else 0.  Two FP64 solutions of Z x = b
+                              with backward errors e1, e2 differ by up to about cond1 * (e1 + e2)
+                              relative: where cond1 * 1e-16 approaches 1e-10, agreement to 1e-10
+                              needs identical arithmetic */
+    double anorm1;         /* with cond_est: ||Z||_1 (largest column sum of moduli), else 0 */
+    double anchor_mhz;     /* recycled sweep (wm_solve_opts.recycle): the frequency whose FP64 factors
+                              produced the result (this one's own for anchors); else 0 */
+    double berr;           /* recycled sweep, GMRES-refined frequencies: normwise backward error
+                              ||b - Z x||_inf / (||Z||_inf ||x||_inf + ||b||_inf) of the result; else 0 */
 } wm_freq_stats;
 
 /* Sweep over nfreq frequencies (MHz, float32 like the reference's freq.txt)

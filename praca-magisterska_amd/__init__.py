@@ -23,6 +23,8 @@ LIB_PATH = os.path.join(HERE, "libwiremom.so")
 
 PREC_F64, PREC_F32, PREC_BF16X3 = 0, 1, 2
 PREC_NAMES = {"f64": PREC_F64, "f32": PREC_F32, "bf16x3": PREC_BF16X3}
+# status codes (WM_ERR_*)
+ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_SPIN, ERR_NOCONV, ERR_IO, ERR_GEOMETRY, ERR_STATE = -1, -2, -3, -4, -5, -6, -7, -8
 # capacities of the reference driver ("wersja referencyjna/Wire_MoM.cu":43-44)
 NNMAX, NTMAX, NWMAX, NPWMAX = 3000, 5000, 10000, 20480
 NSMAX = NWMAX + NPWMAX
@@ -49,7 +51,8 @@ class WireMoMError(RuntimeError):
 
 class SolveOpts(C.Structure):
     _fields_ = [("prec", C.c_int), ("max_iter", C.c_int), ("tol", C.c_double), ("batch", C.c_int),
-                ("hbm_fraction", C.c_double), ("fallback_f64", C.c_int), ("size_policy", C.c_int)]
+                ("hbm_fraction", C.c_double), ("fallback_f64", C.c_int), ("size_policy", C.c_int),
+                ("cond_est", C.c_int), ("recycle", C.c_double)]
 
 
 def _prec(p) -> int:
@@ -57,12 +60,14 @@ def _prec(p) -> int:
 
 
 def _stats(s) -> dict:
-    return dict(iters=s.iters, info=s.info, fell_back=s.fell_back, prec=s.prec, rate=s.rate, dx_rel=s.dx_rel)
+    return dict(iters=s.iters, info=s.info, fell_back=s.fell_back, prec=s.prec, rate=s.rate, dx_rel=s.dx_rel,
+                cond1=s.cond1, anorm1=s.anorm1, anchor_mhz=s.anchor_mhz, berr=s.berr)
 
 
 class FreqStats(C.Structure):
     _fields_ = [("iters", C.c_int), ("info", C.c_int), ("fell_back", C.c_int), ("prec", C.c_int),
-                ("rate", C.c_double), ("dx_rel", C.c_double)]
+                ("rate", C.c_double), ("dx_rel", C.c_double), ("cond1", C.c_double), ("anorm1", C.c_double),
+                ("anchor_mhz", C.c_double), ("berr", C.c_double)]
 
 
 class Load(C.Structure):
@@ -408,9 +413,15 @@ class Geometry:
 
 def solve_sweep(geom: Geometry, freqs_mhz, gaps, prec="f64", ampl: float = 1.0, phas: float = 0.0,
                 batch: int = 0, max_iter: int = 20, tol: float = 1e-12, fallback: bool = True,
-                size_policy: bool = True):
+                size_policy: bool = True, cond_est: bool = False, return_status: bool = False,
+                recycle: float = 0.0):
     """Whole solves (wm_solve_sweep).  Returns (zin[nf, nrhs], I[nf, nrhs, N], stats list).
-    size_policy=False keeps the requested mixed precision at sizes where FP64 factors are cheaper."""
+    size_policy=False keeps the requested mixed precision at sizes where FP64 factors are cheaper;
+    cond_est=True fills stats[f]["cond1"] with an estimate of the 1-norm condition number of Z.
+    return_status=True appends the call's status to the result instead of raising on
+    ERR_NOCONV (refinement that did not converge with fallback=False).
+    recycle=r > 1 factors only anchor frequencies (every frequency within a factor r of one) and
+    solves the others by FP64 GMRES refinement with the nearest anchor's factors."""
     f = np.ascontiguousarray(np.atleast_1d(freqs_mhz), np.float32)
     g = np.ascontiguousarray(np.atleast_1d(gaps), np.int32)
     nf, nr, n = len(f), len(g), geom.npls
@@ -419,14 +430,17 @@ def solve_sweep(geom: Geometry, freqs_mhz, gaps, prec="f64", ampl: float = 1.0, 
     o.prec = PREC_NAMES[prec] if isinstance(prec, str) else int(prec)
     o.batch, o.max_iter, o.tol, o.fallback_f64 = batch, max_iter, tol, int(fallback)
     o.size_policy = int(size_policy)
+    o.cond_est = int(cond_est)
+    o.recycle = float(recycle)
     I = np.zeros((nf, nr, n), np.complex128)
     zin = np.zeros((nf, nr), np.complex128)
     st = (FreqStats * nf)()
     rc = lib().wm_solve_sweep(geom.ctx.h, geom.h, C.byref(o), nf, _ptr(f, C.c_float), nr, _ptr(g, C.c_int),
                               float(ampl), float(phas), I.ctypes.data, zin.ctypes.data, st)
-    _check(rc)
+    if not (return_status and rc == ERR_NOCONV):
+        _check(rc)
     stats = [_stats(s) for s in st]
-    return zin, I, stats
+    return (zin, I, stats, rc) if return_status else (zin, I, stats)
 
 
 def solve_sweep_ex(geom: Geometry, freqs_mhz, nrhs: int, sources, loads=(), prec="f64", batch: int = 0,

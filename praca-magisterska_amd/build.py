@@ -24,7 +24,8 @@ BUILD = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("WM_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["wm_fill.hip", "wm_lu.hip", "wm_solve.hip", "wm_batch.hip", "wm_api.hip", "wm_server.hip", "wm_transient.hip", "wm_geometry.cpp"]
+SOURCES = ["wm_fill.hip", "wm_lu.hip", "wm_solve.hip", "wm_batch.hip", "wm_gmres.hip", "wm_api.hip", "wm_server.hip",
+           "wm_transient.hip", "wm_geometry.cpp"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-Wno-unknown-pragmas",
           "-I", os.path.join(ROOT, "include"), "-I", CSRC]

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "wm_front.h"
+#include "wm_gmres.h"
 #include "wm_internal.h"
 
 namespace wm {
@@ -482,10 +483,26 @@ int wm_geom_create_env(wm_ctx *c, const float *x, const float *y, const float *z
         }
         for (int d = 0; d < 12; ++d) obs[(size_t)d * ld + p] = 1.0e3;
     }
+    // diameter of the structure (with its image over the PEC ground): the
+    // bounding-box diagonal of every source half
+    double lo[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, hi[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+    for (int p = 0; p < npls; ++p)
+        for (int h = 0; h < (pec ? 4 : 2); ++h) {
+            const double *r8 = (h < 2 ? &src[(size_t)p * 16] : &img[(size_t)p * 16]) + 8 * (h & 1);
+            for (int d = 0; d < 3; ++d)
+                for (int e = 0; e < 2; ++e) {
+                    const double v = r8[d] + (e ? r8[6] * r8[3 + d] : 0.0);
+                    lo[d] = std::min(lo[d], v);
+                    hi[d] = std::max(hi[d], v);
+                }
+        }
+    double diam = 0.0;
+    for (int d = 0; d < 3 && npls > 0; ++d) diam += (hi[d] - lo[d]) * (hi[d] - lo[d]);
     auto *g = new wm_geom();
     g->ctx = c;
     g->npls = npls;
     g->ld = ld;
+    g->diam = std::sqrt(diam);
     g->h_src = src;
     g->ienv = ienv;
     g->sgnenv = (double)sgnenv;
@@ -724,6 +741,8 @@ void wm_solve_opts_default(wm_solve_opts *o)
     o->hbm_fraction = 0.8;
     o->fallback_f64 = 1;
     o->size_policy = 1;
+    o->cond_est = 0;
+    o->recycle = 0.0;
 }
 
 }  // extern "C"
@@ -737,8 +756,56 @@ namespace {
 // routing (see below).
 constexpr int kBlindBatch = 16;  // first batch of a mixed sweep (the top quarter of a 64-point sweep)
 
+// Recycled sweep: frequencies highest first in batches as wide as the HBM
+// budget allows; in each batch the fewest anchors such that every frequency
+// lies within a factor o.recycle of its anchor (greedy interval cover in
+// log f: the anchor of the highest uncovered frequency is the lowest one
+// within the factor, and it covers every frequency down to a factor below).
+int run_sweep_recycled(wm_ctx *c, const wm_geom *g, const wm_solve_opts &o, std::vector<WorkItem> &items, int nrhs)
+{
+    const int nfreq = (int)items.size();
+    std::stable_sort(items.begin(), items.end(), [](const WorkItem &a, const WorkItem &b) { return a.freq > b.freq; });
+    int batch = o.batch;
+    if (batch <= 0) {
+        size_t freeb = 0, total = 0;
+        if (hipMemGetInfo(&freeb, &total) != hipSuccess) freeb = c->hbm_bytes / 2;
+        const double frac = o.hbm_fraction > 0 && o.hbm_fraction <= 1 ? o.hbm_fraction : 0.8;
+        const size_t per = item_bytes(g->ld, nrhs, WM_PREC_F64) + gmres_bytes(g->ld, nrhs, kGmresRestart);
+        batch = (int)std::max<size_t>(1, (size_t)(frac * (double)(freeb + c->ws_sweep.bytes)) / per);
+        batch = std::min(batch, 128);
+    }
+    // covering factor at f: o.recycle while the structure is electrically small
+    // (k D <= 1, D its diameter), shrinking as 1 + (r - 1) / (k D) above, where
+    // the resonances between an anchor and its neighbours lengthen GMRES; it
+    // does not grow with f, so a frequency between an anchor and the highest
+    // frequency it covers is covered as well
+    const double r = o.recycle;
+    auto radius = [&](float f) {
+        const double kd = wavenumber_mhz(f) * g->diam;
+        return kd <= 1.0 ? r : 1.0 + (r - 1.0) / kd;
+    };
+    int status = WM_OK;
+    for (int f0 = 0; f0 < nfreq;) {
+        const int cnt = std::min(batch, nfreq - f0);
+        WorkItem *it = items.data() + f0;
+        for (int i = 0; i < cnt;) {
+            int a = i;
+            while (a + 1 < cnt && (double)it[i].freq <= radius(it[i].freq) * (double)it[a + 1].freq) ++a;
+            int k = i;
+            while (k < cnt && (double)it[a].freq <= radius(it[a].freq) * (double)it[k].freq) it[k++].anchor = a;
+            i = k;
+        }
+        const int rc = solve_items(c, o, it, cnt, c->stream);
+        if (rc < 0) return rc;
+        if (rc != WM_OK && status == WM_OK) status = rc;
+        f0 += cnt;
+    }
+    return status;
+}
+
 int run_sweep(wm_ctx *c, const wm_geom *g, const wm_solve_opts &o, std::vector<WorkItem> &items, int nrhs)
 {
+    if (o.recycle > 1.0) return run_sweep_recycled(c, g, o, items, nrhs);
     const int nfreq = (int)items.size();
     int batch = o.batch > 0 ? o.batch : auto_batch(c, g->ld, nrhs, o.prec, o.hbm_fraction);
     int status = WM_OK;
@@ -786,8 +853,10 @@ int run_sweep(wm_ctx *c, const wm_geom *g, const wm_solve_opts &o, std::vector<W
             }
         }
         const int rc = solve_items(c, o, items.data() + f0, cnt, c->stream);
-        if (rc < 0) return rc;
-        if (rc > 0 && status == WM_OK) status = rc;
+        // refinement that did not converge without the FP64 fallback: the other
+        // frequencies are still solved and the status is returned at the end
+        if (rc < 0 && rc != WM_ERR_NOCONV) return rc;
+        if (rc != WM_OK && status == WM_OK) status = rc;
         if (mixed) record_rates(items.data() + f0, cnt);
         f0 += cnt;
     }

@@ -20,10 +20,12 @@
 // mask, fixed reduction splits), so an item's result is the same alone or
 // in any batch.
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstring>
 #include <vector>
 
+#include "wm_gmres.h"
 #include "wm_internal.h"
 
 namespace wm {
@@ -150,6 +152,10 @@ __global__ void k_update(double2 *X, const double2 *D, int n, int ldb, int64_t s
         X[o] = x;
         dm = hypot(d.x, d.y);
         xm = hypot(x.x, x.y);
+        // fmax drops NaN operands: a non-finite correction or iterate is
+        // reported as +inf, which the host takes as divergence
+        if (!(dm <= DBL_MAX)) dm = INFINITY;
+        if (!(xm <= DBL_MAX)) xm = INFINITY;
     }
     for (int off = 32; off >= 1; off >>= 1) {
         dm = fmax(dm, __shfl_xor(dm, off));
@@ -187,6 +193,23 @@ __global__ void k_to_c64(float2 *Z32, const double2 *Z, int64_t nn, int64_t stri
     Z32[o] = make_float2((float)z.x, (float)z.y);
 }
 
+// 1-norm of Z (largest column sum of moduli over the leading np[b] x np[b]
+// block) per item, as the ordered bit pattern of a non-negative double.  One
+// wave per column.
+__global__ void k_norm1(const double2 *Z, int64_t strideZ, int ld, const int *np, const int *bidx,
+                        unsigned long long *out)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int col = blockIdx.x * 4 + wave, bl = blockIdx.y, b = bidx[bl];
+    const int n = np[bl];
+    if (col >= n) return;  // wave-uniform
+    const double2 *zc = Z + (size_t)b * strideZ + (size_t)col * ld;
+    double sum = 0.0;
+    for (int i = lane; i < n; i += 64) sum += hypot(zc[i].x, zc[i].y);
+    for (int off = 32; off >= 1; off >>= 1) sum += __shfl_xor(sum, off);
+    if (lane == 0) atomicMax(&out[bl], (unsigned long long)__double_as_longlong(sum));
+}
+
 __global__ void k_zero_u64(unsigned long long *p, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -209,6 +232,167 @@ wm_c128 cdiv(wm_c128 a, wm_c128 b)  // Smith (1962)
 }
 
 }  // namespace
+
+// Estimates ||A^-1||_1 for the items lst (factors F, complex128 for prec
+// WM_PREC_F64, complex64 otherwise, permutations perm; A restricted to its
+// leading np[i] x np[i] block, the padded identity excluded) with Hager's
+// method as refined by Higham (ACM TOMS 14:381-396, 1988; the algorithm of
+// LAPACK's zlacon/zlacn2): from x = (1/n, ..., 1/n), alternate y = A^-1 x and
+// z = A^-H sign(y), moving x to the unit vector at the largest |z_j| while the
+// estimate ||y||_1 grows (at most 5 steps), then take the larger of that and
+// 2 ||A^-1 v||_1 / (3n) for v_i = (-1)^i (1 + i / (n - 1)).  Items advance in
+// lockstep; each round solves, in one batched launch per direction, the items
+// that need it.  V (device, stride sV) holds the vectors; dact is the index
+// table buffer.
+int inv_norm1_estimate(wm_ctx *c, int prec, int n, const void *F, int64_t sF, const int *perm, double2 *V,
+                       int64_t sV, const std::vector<int> &lst, const std::vector<int> &np, int *dact, hipStream_t s,
+                       std::vector<double> &est)
+{
+    const int m = (int)lst.size();
+    est.assign(m, 0.0);
+    enum Step { FIRST, SIGN_Z, UNIT_Y, UNIT_Z, ALT, DONE };
+    struct St {
+        Step step = FIRST;
+        int j = 0, iter = 0;
+        double estold = 0.0;
+    };
+    std::vector<St> st(m);
+    std::vector<std::vector<double2>> vec(m, std::vector<double2>((size_t)n, make_double2(0.0, 0.0)));
+    auto set_first = [&](int i) {
+        for (int r = 0; r < n; ++r) vec[i][r] = make_double2(r < np[i] ? 1.0 / np[i] : 0.0, 0.0);
+    };
+    auto set_unit = [&](int i, int j) {
+        for (int r = 0; r < n; ++r) vec[i][r] = make_double2(r == j ? 1.0 : 0.0, 0.0);
+    };
+    auto set_alt = [&](int i) {
+        const int k = np[i];
+        for (int r = 0; r < n; ++r)
+            vec[i][r] = make_double2(r < k ? (r % 2 ? -1.0 : 1.0) * (1.0 + (k > 1 ? (double)r / (k - 1) : 0.0)) : 0.0, 0.0);
+    };
+    auto norm1 = [&](int i) {
+        double t = 0.0;
+        for (int r = 0; r < np[i]; ++r) t += std::hypot(vec[i][r].x, vec[i][r].y);
+        return t;
+    };
+    auto to_sign = [&](int i) {  // y_r / |y_r| (1 where y_r = 0), zero on the padding
+        for (int r = 0; r < n; ++r) {
+            const double a = std::hypot(vec[i][r].x, vec[i][r].y);
+            vec[i][r] = r >= np[i] ? make_double2(0.0, 0.0)
+                        : a > DBL_MIN ? make_double2(vec[i][r].x / a, vec[i][r].y / a) : make_double2(1.0, 0.0);
+        }
+    };
+    auto argmax = [&](int i) {
+        int j = 0;
+        double best = -1.0;
+        for (int r = 0; r < np[i]; ++r) {
+            const double a = std::hypot(vec[i][r].x, vec[i][r].y);
+            if (a > best) { best = a; j = r; }
+        }
+        return j;
+    };
+    for (int i = 0; i < m; ++i) set_first(i);
+    for (int round = 0; round < 32; ++round) {
+        // FIRST, UNIT_Y, ALT want y = A^-1 x; SIGN_Z, UNIT_Z want z = A^-H x
+        for (int h = 0; h < 2; ++h) {
+            std::vector<int> sel, mem;
+            for (int i = 0; i < m; ++i) {
+                const bool wants_h = st[i].step == SIGN_Z || st[i].step == UNIT_Z;
+                if (st[i].step != DONE && wants_h == (h == 1)) { sel.push_back(i); mem.push_back(lst[i]); }
+            }
+            if (sel.empty()) continue;
+            const int k = (int)sel.size();
+            for (int q = 0; q < k; ++q)
+                WM_TRY(copy_h2d(V + (size_t)mem[q] * sV, vec[sel[q]].data(), (size_t)n * sizeof(double2), s));
+            WM_TRY(copy_h2d(dact, mem.data(), k * sizeof(int), s));
+            if (h == 0) WM_TRY(getrs_perm_batched(c, prec, n, 1, F, n, sF, perm, n, V, n, sV, k, s, dact));
+            else WM_TRY(getrsh_perm_batched(c, prec, n, 1, F, n, sF, perm, n, V, n, sV, k, s, dact));
+            WM_TRY(check_device_error(c, s));
+            for (int q = 0; q < k; ++q)
+                WM_TRY(copy_d2h(vec[sel[q]].data(), V + (size_t)mem[q] * sV, (size_t)n * sizeof(double2), s));
+            for (int i : sel) {
+                St &t = st[i];
+                switch (t.step) {
+                    case FIRST:
+                        est[i] = norm1(i);
+                        if (np[i] == 1) { t.step = DONE; break; }
+                        to_sign(i);
+                        t.step = SIGN_Z;
+                        break;
+                    case SIGN_Z:
+                        t.j = argmax(i);
+                        t.iter = 2;
+                        set_unit(i, t.j);
+                        t.step = UNIT_Y;
+                        break;
+                    case UNIT_Y:
+                        t.estold = est[i];
+                        est[i] = norm1(i);
+                        if (est[i] <= t.estold) { est[i] = std::max(est[i], t.estold); set_alt(i); t.step = ALT; break; }
+                        to_sign(i);
+                        t.step = UNIT_Z;
+                        break;
+                    case UNIT_Z: {
+                        const int jlast = t.j;
+                        t.j = argmax(i);
+                        const double zl = std::hypot(vec[i][jlast].x, vec[i][jlast].y);
+                        const double zj = std::hypot(vec[i][t.j].x, vec[i][t.j].y);
+                        if (zl != zj && t.iter < 5) {
+                            ++t.iter;
+                            set_unit(i, t.j);
+                            t.step = UNIT_Y;
+                        } else {
+                            set_alt(i);
+                            t.step = ALT;
+                        }
+                        break;
+                    }
+                    case ALT: {
+                        const double temp = 2.0 * norm1(i) / (3.0 * np[i]);
+                        if (temp > est[i]) est[i] = temp;
+                        t.step = DONE;
+                        break;
+                    }
+                    default: break;
+                }
+            }
+        }
+        bool all = true;
+        for (int i = 0; i < m; ++i) all = all && st[i].step == DONE;
+        if (all) break;
+    }
+    return WM_OK;
+}
+
+int refine_decide(int nrhs, const double *dm, const double *xm, double *prev, int it, double tol, int max_iter,
+                  double &first, double &rate, double &worst)
+{
+    worst = 0.0;
+    bool stalled = true, finite = true;
+    for (int r = 0; r < nrhs; ++r) {
+        if (!std::isfinite(dm[r]) || !std::isfinite(xm[r])) finite = false;
+        const double rel = xm[r] > 0 ? dm[r] / xm[r] : (dm[r] > 0 ? HUGE_VAL : 0.0);
+        worst = std::max(worst, rel);
+        double &p = prev[r];
+        if (!(rel > 0.5 * p && p < 1e-9)) stalled = false;
+        p = rel;
+    }
+    if (!finite || !(worst < HUGE_VAL)) {  // a non-finite iterate: these factors cannot refine
+        rate = HUGE_VAL;
+        return -1;
+    }
+    // recorded rate: geometric mean contraction since the first correction
+    if (it == 0) first = worst;
+    if (it >= 1 && !stalled && first > 0) rate = std::pow(worst / first, 1.0 / it);
+    else if (it == 0 && worst <= tol) rate = worst;  // converged at once: rate at most this
+    if (worst <= tol || stalled) return 1;
+    // contraction too slow to reach tol within max_iter: stop now
+    if (it >= 1) {
+        const double need = rate > 0 && rate < 1 ? std::log(tol / worst) / std::log(rate) : 1e9;
+        if (it + 1 + need > max_iter) return -1;
+    }
+    if (it + 1 >= max_iter) return -1;  // out of iterations
+    return 0;
+}
 
 double wavenumber_mhz(float freq_mhz)  // "wersja referencyjna/Wire_MoM.cu":5-6,338-340
 {
@@ -415,7 +599,19 @@ int solve_items(wm_ctx *c, const wm_solve_opts &o, WorkItem *items, int count, h
         ne = std::max(ne, (int)ent[b].size());
         nl = std::max(nl, (int)lds[b].size());
     }
-    const bool mixed = o.prec != WM_PREC_F64;
+    // recycled sweep: items with an anchor (a batch index) are solved by GMRES
+    // refinement with their anchor's FP64 factors; anchors are factored
+    bool recycle = false;
+    for (int b = 0; b < count; ++b)
+        if (items[b].anchor >= 0) {
+            recycle = true;
+            if (items[b].anchor >= count || items[items[b].anchor].anchor != items[b].anchor) {
+                set_error("solve_items: item %d names anchor %d, which is not an anchor of the batch", b,
+                          items[b].anchor);
+                return WM_ERR_ARG;
+            }
+        }
+    const bool mixed = o.prec != WM_PREC_F64 && !recycle;
     if (o.prec != WM_PREC_F64 && o.prec != WM_PREC_F32 && o.prec != WM_PREC_BF16X3) {
         set_error("unknown precision %d", o.prec);
         return WM_ERR_ARG;
@@ -427,7 +623,13 @@ int solve_items(wm_ctx *c, const wm_solve_opts &o, WorkItem *items, int count, h
     const size_t oZ = carve(nn * sizeof(double2) * count);
     const size_t oZ32 = carve(mixed ? nn * sizeof(float2) * count : 0);
     const size_t szV = (size_t)sV * count * sizeof(double2);
-    const size_t oB = carve(szV), oX = carve(szV), oD = carve(mixed ? szV : 0), oR = carve(mixed ? szV : 0);
+    const size_t oB = carve(szV), oX = carve(szV), oD = carve(mixed || recycle || o.cond_est ? szV : 0),
+                 oR = carve(mixed || recycle ? szV : 0);
+    const int gm = kGmresRestart;
+    const size_t oK = carve(recycle ? gmres_bytes(n, nr, gm) * count : 0);
+    const size_t oRed = carve(recycle ? (size_t)(3 * gm + 4) * count * nr * sizeof(double2) : 0);
+    const size_t oF = carve(recycle ? count * sizeof(int) : 0);
+    const size_t oM2 = carve(recycle ? (size_t)count * (1 + nr) * sizeof(unsigned long long) : 0);
     const size_t oP = carve((size_t)n * count * sizeof(int)), oI = carve(count * sizeof(int));
     const size_t oM = carve((size_t)2 * count * nr * sizeof(unsigned long long));
     const size_t oA = carve(count * sizeof(int));
@@ -448,6 +650,9 @@ int solve_items(wm_ctx *c, const wm_solve_opts &o, WorkItem *items, int count, h
     double2 *gval = (double2 *)(base + oG + cne * 2 * sizeof(int));
     int *lidx = (int *)(base + oL);
     double2 *lval = (double2 *)(base + oL + cnl * sizeof(int));
+    double2 *Kry = (double2 *)(base + oK), *red = (double2 *)(base + oRed);
+    int *dfac = (int *)(base + oF);
+    unsigned long long *mx2 = (unsigned long long *)(base + oM2);
 
     // excitation: ampl*exp(j*pi*phas/180) at each feed ("wersja referencyjna/Wire_MoM.cu":307-309)
     std::vector<int> hcol(cne, 0), hidx(cne, -1), hlidx(cnl, -1);
@@ -496,10 +701,82 @@ int solve_items(wm_ctx *c, const wm_solve_opts &o, WorkItem *items, int count, h
                            nl, count);
     }
 
+    // 1-norms of the matrices before they are factored (condition estimate)
+    std::vector<unsigned long long> hnorm(count, 0ull);
+    if (o.cond_est) {
+        std::vector<int> all(count), np(count);
+        for (int b = 0; b < count; ++b) { all[b] = b; np[b] = items[b].g->npls; }
+        int *dnp = perm;  // free until the first factorization
+        WM_TRY(copy_h2d(dact, all.data(), count * sizeof(int), s));
+        WM_TRY(copy_h2d(dnp, np.data(), count * sizeof(int), s));
+        hipLaunchKernelGGL(k_zero_u64, dim3((count + 255) / 256), dim3(256), 0, s, mx, count);
+        hipLaunchKernelGGL(k_norm1, dim3((unsigned)(n / 4), (unsigned)count), dim3(256), 0, s, Z, sZ, n, dnp, dact, mx);
+        WM_TRY(copy_d2h(hnorm.data(), mx, count * sizeof(unsigned long long), s));
+    }
+
     std::vector<int> hinfo(count, 0), iters(count, 0), fell(count, 0);
-    std::vector<double> dxr(count, 0.0);
+    std::vector<double> dxr(count, 0.0), berr(count, 0.0);
+    int noconv = 0;
     auto upload_list = [&](const std::vector<int> &lst) { return copy_h2d(dact, lst.data(), lst.size() * sizeof(int), s); };
-    if (!mixed) {
+    if (recycle) {
+        // anchors (and items without one): FP64 factors, direct solve
+        std::vector<int> own, deps, fac;
+        for (int b = 0; b < count; ++b) {
+            if (items[b].anchor < 0 || items[b].anchor == b) own.push_back(b);
+            else { deps.push_back(b); fac.push_back(items[b].anchor); }
+        }
+        auto factor_solve = [&](const std::vector<int> &lst) -> int {
+            const int k = (int)lst.size();
+            WM_TRY(upload_list(lst));
+            WM_TRY(getrf_batched(c, WM_PREC_F64, n, Z, n, sZ, ipiv, n, k, dinfo, s, dact));
+            WM_TRY(perm_batched(c, n, ipiv, n, perm, n, k, s, dact));
+            hipLaunchKernelGGL(k_copy_items, dim3((unsigned)((sV + 255) / 256), k), dim3(256), 0, s, X, B, sV, sV, dact);
+            WM_TRY(getrs_perm_batched(c, WM_PREC_F64, n, nr, Z, n, sZ, perm, n, X, n, sV, k, s, dact));
+            WM_TRY(check_device_error(c, s));
+            std::vector<int> h(count, 0);  // info words of the listed items only
+            WM_TRY(copy_d2h(h.data(), dinfo, sizeof(int) * count, s));
+            for (int b : lst) hinfo[b] = h[b];
+            return WM_OK;
+        };
+        if (!own.empty()) WM_TRY(factor_solve(own));
+        for (int b : own) items[b].anchor_used = b;
+        // the others: FP64 GMRES refinement preconditioned by the anchor's factors
+        // (an anchor with a zero pivot preconditions nothing: its dependents are factored)
+        std::vector<int> gdeps, gfac, redo;
+        for (size_t q = 0; q < deps.size(); ++q) {
+            if (hinfo[fac[q]] != 0) redo.push_back(deps[q]);
+            else { gdeps.push_back(deps[q]); gfac.push_back(fac[q]); }
+        }
+        if (!gdeps.empty()) {
+            GmresArgs ga;
+            ga.n = n; ga.nr = nr; ga.m = gm; ga.count = count;
+            ga.Z = Z; ga.sZ = sZ; ga.F = Z; ga.fprec = WM_PREC_F64; ga.sF = sZ; ga.perm = perm;
+            ga.B = B; ga.X = X; ga.R = R; ga.D = D; ga.sV = sV;
+            ga.K = Kry; ga.red = red; ga.dact = dact; ga.dfac = dfac; ga.mx = mx; ga.mx2 = mx2;
+            ga.np = 0;
+            for (int b : gdeps) ga.np = std::max(ga.np, items[b].g->npls);
+            GmresOut go(count);
+            WM_TRY(gmres_ir(c, o, ga, gdeps, gfac, go, s));
+            for (size_t q = 0; q < gdeps.size(); ++q) {
+                const int b = gdeps[q];
+                iters[b] = go.iters[b];
+                dxr[b] = go.dx_rel[b];
+                berr[b] = go.berr[b];
+                if (go.conv[b]) items[b].anchor_used = gfac[q];
+                else redo.push_back(b);
+            }
+        }
+        // items whose refinement did not converge: factored themselves
+        if (!redo.empty()) {
+            std::sort(redo.begin(), redo.end());
+            WM_TRY(factor_solve(redo));
+            for (int b : redo) {
+                fell[b] = 1;
+                items[b].anchor_used = b;
+            }
+        }
+        for (int b = 0; b < count; ++b) items[b].fprec = WM_PREC_F64;
+    } else if (!mixed) {
         WM_TRY(getrf_batched(c, WM_PREC_F64, n, Z, n, sZ, ipiv, n, count, dinfo, s));
         WM_TRY(perm_batched(c, n, ipiv, n, perm, n, count, s));
         WM_HIP_TRY(hipMemcpyAsync(X, B, szV, hipMemcpyDeviceToDevice, s));
@@ -521,6 +798,7 @@ int solve_items(wm_ctx *c, const wm_solve_opts &o, WorkItem *items, int count, h
         std::vector<double> prev((size_t)count * nr), first(count);
         std::vector<unsigned long long> hm((size_t)2 * count * nr);
         std::vector<int> retry;  // level-0 items refactored at level 1
+        std::vector<char> tried(count, 0);  // entered a complex64 refinement level
         for (int q = 0; q < 2; ++q) {
             std::vector<int> &L = lvl[q];
             if (L.empty()) continue;
@@ -549,6 +827,7 @@ int solve_items(wm_ctx *c, const wm_solve_opts &o, WorkItem *items, int count, h
             for (int b : L) {
                 for (int r = 0; r < nr; ++r) prev[(size_t)b * nr + r] = 1e300;
                 items[b].fprec = pq;
+                tried[b] = 1;
                 if (hinfo[b] == 0) active.push_back(b);
                 else {
                     items[b].rate = HUGE_VAL;  // singular complex64 factor
@@ -569,40 +848,17 @@ int solve_items(wm_ctx *c, const wm_solve_opts &o, WorkItem *items, int count, h
                 WM_TRY(copy_d2h(hm.data(), mx, hm.size() * sizeof(unsigned long long), s));
                 std::vector<int> still;
                 for (int b : active) {
-                    double worst = 0.0, rate = 0.0;
-                    bool stalled = true;
+                    double dm[64], xm[64];
                     for (int r = 0; r < items[b].nrhs; ++r) {
-                        double dm, xm;
-                        std::memcpy(&dm, &hm[2 * ((size_t)b * nr + r)], 8);
-                        std::memcpy(&xm, &hm[2 * ((size_t)b * nr + r) + 1], 8);
-                        const double rel = xm > 0 ? dm / xm : 0.0;
-                        worst = std::max(worst, rel);
-                        double &p = prev[(size_t)b * nr + r];
-                        rate = std::max(rate, p < 1e299 && p > 0 ? rel / p : 0.0);
-                        if (!(rel > 0.5 * p && p < 1e-9)) stalled = false;
-                        p = rel;
+                        std::memcpy(&dm[r], &hm[2 * ((size_t)b * nr + r)], 8);
+                        std::memcpy(&xm[r], &hm[2 * ((size_t)b * nr + r) + 1], 8);
                     }
-                    dxr[b] = worst;
                     iters[b] += 1;
-                    // recorded rate: geometric mean contraction since the first correction
-                    if (it == 0) first[b] = worst;
-                    if (it >= 1 && !stalled && first[b] > 0)
-                        items[b].rate = std::pow(worst / first[b], 1.0 / it);
-                    else if (it == 0 && worst <= o.tol) items[b].rate = worst;  // converged at once: rate at most this
-                    if (worst <= o.tol || stalled) {
-                        conv[b] = 1;
-                        continue;
-                    }
-                    // contraction too slow to reach tol within max_iter: stop now
-                    if (it >= 1) {
-                        const double gm = items[b].rate;
-                        const double need = gm > 0 && gm < 1 ? std::log(o.tol / worst) / std::log(gm) : 1e9;
-                        if (it + 1 + need > o.max_iter) {
-                            failed.push_back(b);
-                            continue;
-                        }
-                    }
-                    still.push_back(b);
+                    const int v = refine_decide(items[b].nrhs, dm, xm, &prev[(size_t)b * nr], it, o.tol, o.max_iter,
+                                                first[b], items[b].rate, dxr[b]);
+                    if (v > 0) conv[b] = 1;
+                    else if (v < 0) failed.push_back(b);
+                    else still.push_back(b);
                 }
                 active.swap(still);
             }
@@ -617,15 +873,24 @@ int solve_items(wm_ctx *c, const wm_solve_opts &o, WorkItem *items, int count, h
                 }
             }
         }
-        // items that did not converge (or had singular complex64 factors): one batched FP64 refactorization
+        // Items routed to FP64 factors, and (with fallback_f64) items whose
+        // refinement did not converge or whose complex64 factors were singular:
+        // one batched FP64 factorization.  Without fallback_f64 the items that
+        // failed refinement keep their last iterate and the call reports
+        // WM_ERR_NOCONV after every other item is finished.
         std::vector<int> bad;
-        for (int b = 0; b < count; ++b)
-            if (!conv[b]) bad.push_back(b);
-        if (!bad.empty()) {
-            if (!o.fallback_f64) {
-                set_error("refinement did not converge for %zu of %d items", bad.size(), count);
-                return WM_ERR_NOCONV;
+        int nfail = 0;
+        for (int b = 0; b < count; ++b) {
+            if (conv[b]) continue;
+            if (tried[b] && !o.fallback_f64) {
+                ++nfail;
+                fell[b] = -1;
+                continue;
             }
+            bad.push_back(b);
+        }
+        if (nfail > 0) noconv = nfail;
+        if (!bad.empty()) {
             const int nb = (int)bad.size();
             WM_TRY(upload_list(bad));
             WM_TRY(getrf_batched(c, WM_PREC_F64, n, Z, n, sZ, ipiv, n, nb, dinfo, s, dact));
@@ -635,9 +900,32 @@ int solve_items(wm_ctx *c, const wm_solve_opts &o, WorkItem *items, int count, h
             WM_TRY(check_device_error(c, s));
             WM_TRY(copy_d2h(hinfo.data(), dinfo, sizeof(int) * count, s));
             for (int b : bad) {
-                fell[b] = items[b].fprec == WM_PREC_F64 ? 2 : 1;
+                fell[b] = tried[b] ? 1 : 2;
                 items[b].fprec = WM_PREC_F64;
             }
+        }
+    }
+
+    // condition estimates with the final factors of every item
+    std::vector<double> cond(count, 0.0), anorm(count, 0.0);
+    if (o.cond_est) {
+        for (int b = 0; b < count; ++b) {
+            unsigned long long bits = hnorm[b];
+            std::memcpy(&anorm[b], &bits, 8);
+        }
+        for (int f64 = 0; f64 < 2; ++f64) {
+            std::vector<int> lst;
+            for (int b = 0; b < count; ++b) {
+                if (recycle && items[b].anchor_used != b) continue;  // solved with another item's factors
+                if ((mixed ? items[b].fprec == WM_PREC_F64 : true) == (f64 == 1)) lst.push_back(b);
+            }
+            if (lst.empty()) continue;
+            std::vector<int> np(lst.size());
+            for (size_t i = 0; i < lst.size(); ++i) np[i] = items[lst[i]].g->npls;
+            std::vector<double> est;
+            WM_TRY(inv_norm1_estimate(c, f64 ? WM_PREC_F64 : WM_PREC_F32, n, f64 ? (const void *)Z : (const void *)Z32,
+                                      sZ, perm, D, sV, lst, np, dact, s, est));
+            for (size_t i = 0; i < lst.size(); ++i) cond[lst[i]] = anorm[lst[i]] * est[i];
         }
     }
 
@@ -670,9 +958,26 @@ int solve_items(wm_ctx *c, const wm_solve_opts &o, WorkItem *items, int count, h
             w.st->prec = mixed ? w.fprec : WM_PREC_F64;
             w.st->rate = mixed ? w.rate : -1.0;
             w.st->dx_rel = dxr[b];
+            w.st->cond1 = cond[b];
+            w.st->anorm1 = anorm[b];
+            w.st->anchor_mhz = recycle && w.anchor_used >= 0 ? (double)items[w.anchor_used].freq : 0.0;
+            w.st->berr = berr[b];
         }
+    }
+    if (noconv > 0 && status == WM_OK) {
+        set_error("refinement did not converge for %d of %d items (fallback to FP64 factors disabled)", noconv, count);
+        return WM_ERR_NOCONV;
     }
     return status;
 }
 
 }  // namespace wm
+
+// Internal test hook (not part of include/wiremom.h): the host convergence test
+// of one refinement iteration, wm::refine_decide, callable without a device.
+extern "C" int wmx_refine_decide(int nrhs, const double *dm, const double *xm, double *prev, int it, double tol,
+                                 int max_iter, double *first, double *rate, double *worst)
+{
+    if (nrhs <= 0 || !dm || !xm || !prev || !first || !rate || !worst) return -100;
+    return wm::refine_decide(nrhs, dm, xm, prev, it, tol, max_iter, *first, *rate, *worst);
+}

@@ -110,6 +110,7 @@ struct wm_geom {
     wm_ctx *ctx = nullptr;
     int npls = 0;     // N, unknowns
     int ld = 0;       // padded size (multiple of 128)
+    double diam = 0;  // bounding-box diagonal of the structure (with its image over the PEC ground), m
     double *d_obs = nullptr;   // 12 arrays of ld doubles: rc1xyz, lm1xyz, rc2xyz, lm2xyz
     int *d_obs_key = nullptr;  // 2 arrays of ld ints: segment key of obs half 1, 2
     double *d_src = nullptr;   // ld records of 16 doubles: rs1[3] dc1[3] dl1 an1 rs2[3] dc2[3] dl2 an2
@@ -148,14 +149,22 @@ int getrf_batched(wm_ctx *ctx, int prec, int n, void *A, int lda, int64_t stride
 int perm_batched(wm_ctx *ctx, int n, const int *ipiv, int64_t strideP, int *perm, int64_t strideQ, int batch,
                  hipStream_t s, const int *d_bidx = nullptr);
 // Triangular solves with the LU factors and a precomputed permutation; B complex128.
+// d_aidx (optional): member i solves with the factors and permutation of
+// matrix d_aidx[i] (e.g. another frequency's) on the right-hand sides of
+// matrix d_bidx[i].
 int getrs_perm_batched(wm_ctx *ctx, int prec, int n, int nrhs, const void *A, int lda, int64_t strideA,
                        const int *perm, int64_t strideQ, void *B, int ldb, int64_t strideB, int batch,
-                       hipStream_t s, const int *d_bidx = nullptr);
+                       hipStream_t s, const int *d_bidx = nullptr, const int *d_aidx = nullptr);
+// Solves A^H X = B with the same factors and permutation (A = P^T L U, so
+// X = P^T L^-H U^-H B); used by the condition estimate.
+int getrsh_perm_batched(wm_ctx *ctx, int prec, int n, int nrhs, const void *A, int lda, int64_t strideA,
+                        const int *perm, int64_t strideQ, void *B, int ldb, int64_t strideB, int batch,
+                        hipStream_t s, const int *d_bidx = nullptr);
 // Same, computing the permutation from ipiv first.
 int getrs_batched(wm_ctx *ctx, int prec, int n, int nrhs, const void *A, int lda, int64_t strideA,
                   const int *ipiv, int64_t strideP, void *B, int ldb, int64_t strideB, int batch,
                   hipStream_t s);
-// R = B - Z X for complex128 Z (n x n), X, B (n x nrhs); batched.
+// R = B - Z X for complex128 Z (n x n), X, B (n x nrhs); batched.  B null: R = Z X.
 int residual_batched(wm_ctx *ctx, int n, int nrhs, const void *Z, int ldz, int64_t strideZ,
                      const void *X, const void *B, int ldb, int64_t strideB, void *R, int batch,
                      hipStream_t s, const int *d_bidx = nullptr);
@@ -201,7 +210,13 @@ struct WorkItem {
                                // out: the precision of the factors that produced the result
     double rate = -1.0;        // out: last refinement contraction rate (inf: complex64 factor singular)
     double rate_b3 = -1.0;     // out: contraction rate of a failed split-BF16x3 attempt
+    int anchor = -1;           // recycled sweep: batch index of the item whose FP64 factors precondition
+                               // this one's GMRES refinement (itself: factored and solved directly)
+    int anchor_used = -1;      // out: batch index of the item whose factors produced the result
 };
+
+// GMRES restart length of recycled sweeps.
+constexpr int kGmresRestart = 32;
 
 // Without an observation in one precision, split-BF16x3 factors are taken to
 // contract this many times slower than FP32 factors on the same matrix.  The
@@ -237,6 +252,16 @@ int getrf_width(wm_ctx *ctx, int prec, int n);
 // Panels per outer block for a factorization of size n in precision prec.
 int getrf_block(const wm_ctx *ctx, int prec, int n);
 
+// Convergence test of one item after refinement iteration `it` (0-based):
+// dm[r], xm[r] = max |dx|, max |x| over right-hand side r (non-finite values
+// as +inf), prev[r] the previous relative correction (1e300 before the
+// first).  Updates prev, first (the first correction), rate (geometric-mean
+// contraction, HUGE_VAL for a non-finite iterate) and worst (largest relative
+// correction).  Returns 1 converged (worst <= tol, or stalled at the FP64
+// floor), -1 failed (non-finite, or too slow to reach tol within max_iter), 0
+// continue.
+int refine_decide(int nrhs, const double *dm, const double *xm, double *prev, int it, double tol, int max_iter,
+                  double &first, double &rate, double &worst);
 // Fill, factor and solve a batch of items on stream s (synchronous).
 // Returns <0 on error, >0 (first zero pivot) if a matrix was singular.
 int solve_items(wm_ctx *ctx, const wm_solve_opts &o, WorkItem *items, int count, hipStream_t s);

@@ -67,7 +67,8 @@ struct TrsvArgs {
     uint64_t *pkt;        // solved values as tagged packets: per member [block][rhs][row][4 words]
     int epoch;            // packet tag of this launch
     int *err;
-    const int *bidx;      // launch member -> matrix index (null: identity)
+    const int *bidx;      // launch member -> matrix index of B and Y (null: identity)
+    const int *aidx;      // launch member -> matrix index of the factors and permutation (null: as bidx)
 };
 
 __device__ inline double bcast(double v, int lane)  // value of `lane` to the whole wave
@@ -97,16 +98,30 @@ __device__ inline int du_idx(int i, int j) { return j * (j + 1) / 2 + i; }
 // workgroup ids run over the members first: the resident workgroups hold the
 // leading blocks of every member and all members' chains advance together.
 // Waits only on blocks of workgroups with smaller ids (dispatched earlier).
-template <typename T, int NR, bool UPPER>
+//
+// TRANS solves with the conjugate transposes of the factors (A^H = U^H L^H P):
+// the lower pass takes U^H (lower, non-unit diagonal conj(u_ii)) on the
+// right-hand sides as staged (no gather), the upper pass L^H (upper, unit
+// diagonal) in place; element (i, j) of the triangle is conj(A(j, i)).  The
+// caller scatters the result through the permutation (x = P^T w).
+template <typename T, int NR, bool UPPER, bool TRANS>
 __global__ __launch_bounds__(256) void k_trsv(TrsvArgs a, int r_off)
 {
     using V = typename CT<T>::V;
+    constexpr bool UNIT = TRANS ? UPPER : !UPPER;  // unit diagonal (L or L^H)
+    // element (i, j) of the triangle being solved
+    auto elem = [&](const V *A, int i, int j) -> double2 {
+        if (!TRANS) return ld2<T>(A + i + (size_t)j * a.lda);
+        const double2 v = ld2<T>(A + j + (size_t)i * a.lda);
+        return make_double2(v.x, -v.y);
+    };
     const int bl = blockIdx.x, tid = threadIdx.x, row = tid & 63;
     const int b = a.bidx ? a.bidx[bl] : bl;
+    const int fb = a.aidx ? a.aidx[bl] : b;  // factors (and their permutation) of this member
     const int part = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int nblk = a.n / TB;
     const int bi = UPPER ? nblk - 1 - (int)blockIdx.y : (int)blockIdx.y, r0 = bi * TB;
-    const V *A = (const V *)a.A + (size_t)b * a.strideA;
+    const V *A = (const V *)a.A + (size_t)fb * a.strideA;
     double2 *Y = a.Y + (size_t)b * a.strideB + (size_t)r_off * a.ldb;
     uint64_t *pk = a.pkt + (size_t)bl * nblk * NR * 256;
     const uint32_t tag = (uint32_t)a.epoch;
@@ -116,12 +131,14 @@ __global__ __launch_bounds__(256) void k_trsv(TrsvArgs a, int r_off)
     __shared__ double2 s_d[TB * (TB + 1) / 2];
     __shared__ double2 s_rd[TB];
     for (int e = tid; e < TB * TB; e += 256) {
-        const int i = e % TB, j = e / TB;
-        if (!UPPER && i > j) s_d[dl_idx(i, j)] = ld2<T>(A + (r0 + i) + (size_t)(r0 + j) * a.lda);
-        if (UPPER && i <= j) {
-            const double2 u = ld2<T>(A + (r0 + i) + (size_t)(r0 + j) * a.lda);
-            s_d[du_idx(i, j)] = u;
-            if (i == j) s_rd[i] = crecip(u.x, u.y);
+        // consecutive threads walk down a column of the stored factor
+        const int i = TRANS ? e / TB : e % TB, j = TRANS ? e % TB : e / TB;
+        if (!UPPER && i > j) s_d[dl_idx(i, j)] = elem(A, r0 + i, r0 + j);
+        if (UPPER && i < j) s_d[du_idx(i, j)] = elem(A, r0 + i, r0 + j);
+        if (!UNIT && i == j) {
+            const double2 u = elem(A, r0 + i, r0 + j);
+            if (UPPER) s_d[du_idx(i, j)] = u;
+            s_rd[i] = crecip(u.x, u.y);
         }
     }
 
@@ -129,7 +146,7 @@ __global__ __launch_bounds__(256) void k_trsv(TrsvArgs a, int r_off)
 #pragma unroll
     for (int r = 0; r < NR; ++r) acc[r] = make_double2(0.0, 0.0);
     if (part == 0) {
-        const int src = UPPER ? r0 + row : a.perm[(size_t)b * a.strideQ + r0 + row];
+        const int src = (UPPER || TRANS) ? r0 + row : a.perm[(size_t)fb * a.strideQ + r0 + row];
         const double2 *in = UPPER ? Y : a.Bin + (size_t)bl * a.strideB + (size_t)r_off * a.ldb;
 #pragma unroll
         for (int r = 0; r < NR; ++r)
@@ -143,7 +160,7 @@ __global__ __launch_bounds__(256) void k_trsv(TrsvArgs a, int r_off)
     if (nb_off > 0) {
         const int c0 = blk(0) * TB + part * 16;
 #pragma unroll
-        for (int c = 0; c < 16; ++c) lcur[c] = ld2<T>(A + (r0 + row) + (size_t)(c0 + c) * a.lda);
+        for (int c = 0; c < 16; ++c) lcur[c] = elem(A, r0 + row, c0 + c);
     }
     for (int q = 0; q < nb_off; ++q) {
         // The part's 16 solved values of block q: 64 packets per right-hand
@@ -171,7 +188,7 @@ __global__ __launch_bounds__(256) void k_trsv(TrsvArgs a, int r_off)
         if (q + 1 < nb_off) {
             const int c1 = blk(q + 1) * TB + part * 16;
 #pragma unroll
-            for (int c = 0; c < 16; ++c) lnext[c] = ld2<T>(A + (r0 + row) + (size_t)(c1 + c) * a.lda);
+            for (int c = 0; c < 16; ++c) lnext[c] = elem(A, r0 + row, c1 + c);
         }
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -198,22 +215,29 @@ __global__ __launch_bounds__(256) void k_trsv(TrsvArgs a, int r_off)
         // step's LDS read of the block is issued ahead of the step
         if (!UPPER) {
 #pragma unroll
-            for (int j = 0; j < TB - 1; ++j) {
+            for (int j = 0; j < TB; ++j) {
                 const double2 l = row > j ? s_d[dl_idx(row, j)] : make_double2(0.0, 0.0);
+                const double2 ri = UNIT ? make_double2(1.0, 0.0) : s_rd[j];
 #pragma unroll
                 for (int r = 0; r < NR; ++r) {
-                    const double2 yj = make_double2(bcast(acc[r].x, j), bcast(acc[r].y, j));
-                    if (row > j) cfms(acc[r], l, yj);
+                    if (!UNIT && row == j) {
+                        const double2 x = acc[r];
+                        acc[r] = make_double2(x.x * ri.x - x.y * ri.y, x.x * ri.y + x.y * ri.x);
+                    }
+                    if (j < TB - 1) {
+                        const double2 yj = make_double2(bcast(acc[r].x, j), bcast(acc[r].y, j));
+                        if (row > j) cfms(acc[r], l, yj);
+                    }
                 }
             }
         } else {
 #pragma unroll
             for (int j = TB - 1; j >= 0; --j) {
                 const double2 u = row < j ? s_d[du_idx(row, j)] : make_double2(0.0, 0.0);
-                const double2 ri = s_rd[j];
+                const double2 ri = UNIT ? make_double2(1.0, 0.0) : s_rd[j];
 #pragma unroll
                 for (int r = 0; r < NR; ++r) {
-                    if (row == j) {
+                    if (!UNIT && row == j) {
                         const double2 x = acc[r];
                         acc[r] = make_double2(x.x * ri.x - x.y * ri.y, x.x * ri.y + x.y * ri.x);
                     }
@@ -299,29 +323,29 @@ __global__ void k_resid_final(const double2 *P, int nsplit, int n, int nrhs, con
             s.y += p.y;
         }
         const size_t o = (size_t)b * strideB + (size_t)(r_off + r) * ldb + i;
-        R[o] = make_double2(B[o].x - s.x, B[o].y - s.y);
+        R[o] = B ? make_double2(B[o].x - s.x, B[o].y - s.y) : s;  // no B: the product Z X
     }
 }
 
-template <typename T, int NR>
+template <typename T, int NR, bool TRANS>
 void launch_trsv(const TrsvArgs &a, int batch, int r_off, bool upper, hipStream_t s)
 {
     const dim3 grid(batch, a.n / TB);
-    if (upper) hipLaunchKernelGGL((k_trsv<T, NR, true>), grid, dim3(256), 0, s, a, r_off);
-    else hipLaunchKernelGGL((k_trsv<T, NR, false>), grid, dim3(256), 0, s, a, r_off);
+    if (upper) hipLaunchKernelGGL((k_trsv<T, NR, true, TRANS>), grid, dim3(256), 0, s, a, r_off);
+    else hipLaunchKernelGGL((k_trsv<T, NR, false, TRANS>), grid, dim3(256), 0, s, a, r_off);
 }
 
 // Right-hand sides handled by one pass starting at r_off: 8, 4, 2 or 1.
 inline int rhs_chunk(int left) { return left >= 8 ? 8 : left >= 4 ? 4 : left >= 2 ? 2 : 1; }
 
-template <typename T>
+template <typename T, bool TRANS = false>
 void trsv_dispatch(const TrsvArgs &a, int batch, int r_off, bool upper, hipStream_t s)
 {
     switch (rhs_chunk(a.nrhs - r_off)) {
-        case 8: launch_trsv<T, 8>(a, batch, r_off, upper, s); break;
-        case 4: launch_trsv<T, 4>(a, batch, r_off, upper, s); break;
-        case 2: launch_trsv<T, 2>(a, batch, r_off, upper, s); break;
-        default: launch_trsv<T, 1>(a, batch, r_off, upper, s); break;
+        case 8: launch_trsv<T, 8, TRANS>(a, batch, r_off, upper, s); break;
+        case 4: launch_trsv<T, 4, TRANS>(a, batch, r_off, upper, s); break;
+        case 2: launch_trsv<T, 2, TRANS>(a, batch, r_off, upper, s); break;
+        default: launch_trsv<T, 1, TRANS>(a, batch, r_off, upper, s); break;
     }
 }
 
@@ -332,6 +356,26 @@ __global__ void k_stage(const double2 *B, int ldb, int64_t strideB, int n, int n
     const int b = bidx ? bidx[bl] : bl;
     if (i >= n) return;
     for (int r = 0; r < nrhs; ++r) S[(size_t)bl * strideB + (size_t)r * ldb + i] = B[(size_t)b * strideB + (size_t)r * ldb + i];
+}
+
+// S[member][perm[t]] = B[matrix][t] (the scatter x = P^T w of a transposed solve)
+__global__ void k_unperm(const double2 *B, int ldb, int64_t strideB, int n, int nrhs, const int *perm,
+                         int64_t strideQ, double2 *S, const int *bidx, const int *aidx)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, bl = blockIdx.y;
+    const int b = bidx ? bidx[bl] : bl;
+    if (i >= n) return;
+    const int t = perm[(size_t)(aidx ? aidx[bl] : b) * strideQ + i];
+    for (int r = 0; r < nrhs; ++r) S[(size_t)bl * strideB + (size_t)r * ldb + t] = B[(size_t)b * strideB + (size_t)r * ldb + i];
+}
+
+// B[matrix] = S[member]
+__global__ void k_unstage(const double2 *S, int ldb, int64_t strideB, int n, int nrhs, double2 *B, const int *bidx)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, bl = blockIdx.y;
+    const int b = bidx ? bidx[bl] : bl;
+    if (i >= n) return;
+    for (int r = 0; r < nrhs; ++r) B[(size_t)b * strideB + (size_t)r * ldb + i] = S[(size_t)bl * strideB + (size_t)r * ldb + i];
 }
 
 __global__ void k_zero_words(int *p, int64_t n)
@@ -354,11 +398,14 @@ int perm_batched(wm_ctx *ctx, int n, const int *ipiv, int64_t strideP, int *perm
     return WM_OK;
 }
 
+namespace {
+
 // Staging of B per launch member in ctx->ws_solve; the solved values'
 // packets in ctx->ws_pkt, per triangle and member nblk x NR x 256 packets.
-int getrs_perm_batched(wm_ctx *ctx, int prec, int n, int nrhs, const void *A, int lda, int64_t strideA,
-                       const int *perm, int64_t strideQ, void *B, int ldb, int64_t strideB, int batch, hipStream_t s,
-                       const int *d_bidx)
+// trans: solve A^H X = B instead of A X = B.
+int getrs_any(wm_ctx *ctx, int prec, int n, int nrhs, const void *A, int lda, int64_t strideA, const int *perm,
+              int64_t strideQ, void *B, int ldb, int64_t strideB, int batch, hipStream_t s, const int *d_bidx,
+              bool trans, const int *d_aidx)
 {
     if (n <= 0 || n % TB != 0 || nrhs <= 0 || batch <= 0 || ldb < n) {
         set_error("getrs: bad arguments (n=%d multiple of %d, nrhs=%d, batch=%d)", n, TB, nrhs, batch);
@@ -393,23 +440,51 @@ int getrs_perm_batched(wm_ctx *ctx, int prec, int n, int nrhs, const void *A, in
         a.A = A; a.lda = lda; a.strideA = strideA; a.n = n; a.nrhs = nrhs;
         a.perm = perm; a.strideQ = strideQ; a.ldb = ldb; a.strideB = strideB; a.err = ctx->d_err;
         a.bidx = d_bidx;
+        a.aidx = d_aidx;
         a.Bin = Ystage; a.Y = (double2 *)B; a.pkt = pkt;
         a.epoch = ++ctx->solve_epoch;
         const int nr = rhs_chunk(nrhs - r0);
         const double esz = prec == WM_PREC_F64 ? 16.0 : 8.0;
         const double fl = batch * 4.0 * (double)n * n * nr, by = batch * 0.5 * (double)n * n * esz;
-        prof_begin(ctx, K_TRSV, s);
-        if (prec == WM_PREC_F64) trsv_dispatch<double>(a, batch, r0, false, s);
-        else trsv_dispatch<float>(a, batch, r0, false, s);
-        prof_end(ctx, K_TRSV, s, fl, by);
-        a.Bin = nullptr; a.pkt = pkt + tri_pkts;
-        prof_begin(ctx, K_TRSV, s);
-        if (prec == WM_PREC_F64) trsv_dispatch<double>(a, batch, r0, true, s);
-        else trsv_dispatch<float>(a, batch, r0, true, s);
-        prof_end(ctx, K_TRSV, s, fl, by);
+        for (int up = 0; up < 2; ++up) {
+            if (up) { a.Bin = nullptr; a.pkt = pkt + tri_pkts; }
+            prof_begin(ctx, K_TRSV, s);
+            if (prec == WM_PREC_F64) {
+                if (trans) trsv_dispatch<double, true>(a, batch, r0, up, s);
+                else trsv_dispatch<double>(a, batch, r0, up, s);
+            } else {
+                if (trans) trsv_dispatch<float, true>(a, batch, r0, up, s);
+                else trsv_dispatch<float>(a, batch, r0, up, s);
+            }
+            prof_end(ctx, K_TRSV, s, fl, by);
+        }
+    }
+    if (trans) {  // x = P^T w, through the staging buffer
+        hipLaunchKernelGGL(k_unperm, dim3((n + 255) / 256, batch), dim3(256), 0, s, (const double2 *)B, ldb, strideB, n,
+                           nrhs, perm, strideQ, Ystage, d_bidx, d_aidx);
+        hipLaunchKernelGGL(k_unstage, dim3((n + 255) / 256, batch), dim3(256), 0, s, (const double2 *)Ystage, ldb,
+                           strideB, n, nrhs, (double2 *)B, d_bidx);
     }
     WM_HIP_TRY(hipGetLastError());
     return WM_OK;
+}
+
+}  // namespace
+
+int getrs_perm_batched(wm_ctx *ctx, int prec, int n, int nrhs, const void *A, int lda, int64_t strideA,
+                       const int *perm, int64_t strideQ, void *B, int ldb, int64_t strideB, int batch, hipStream_t s,
+                       const int *d_bidx, const int *d_aidx)
+{
+    return getrs_any(ctx, prec, n, nrhs, A, lda, strideA, perm, strideQ, B, ldb, strideB, batch, s, d_bidx, false,
+                     d_aidx);
+}
+
+int getrsh_perm_batched(wm_ctx *ctx, int prec, int n, int nrhs, const void *A, int lda, int64_t strideA,
+                        const int *perm, int64_t strideQ, void *B, int ldb, int64_t strideB, int batch, hipStream_t s,
+                        const int *d_bidx)
+{
+    return getrs_any(ctx, prec, n, nrhs, A, lda, strideA, perm, strideQ, B, ldb, strideB, batch, s, d_bidx, true,
+                     nullptr);
 }
 
 int getrs_batched(wm_ctx *ctx, int prec, int n, int nrhs, const void *A, int lda, int64_t strideA, const int *ipiv,
@@ -457,3 +532,20 @@ int residual_batched(wm_ctx *ctx, int n, int nrhs, const void *Z, int ldz, int64
 }
 
 }  // namespace wm
+
+// Internal test hook (not part of include/wiremom.h): solves A^H X = B with the
+// factors and pivots of wm_getrf_batched (device buffers, as wm_getrs_batched).
+extern "C" int wmx_getrs_trans(wm_ctx *ctx, int prec, int n, int nrhs, const void *A, int lda, int64_t strideA,
+                               const int *ipiv, int64_t strideP, wm_c128 *B, int ldb, int64_t strideB, int batch)
+{
+    using namespace wm;
+    if (!ctx || !A || !ipiv || !B || batch <= 0) return WM_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    WM_TRY(ctx->ws_perm.ensure((size_t)batch * n * sizeof(int)));
+    int *perm = (int *)ctx->ws_perm.ptr;
+    WM_TRY(perm_batched(ctx, n, ipiv, strideP, perm, n, batch, ctx->stream));
+    WM_TRY(getrsh_perm_batched(ctx, prec, n, nrhs, A, lda, strideA, perm, n, B, ldb, strideB, batch, ctx->stream));
+    WM_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return check_device_error(ctx, ctx->stream);
+}

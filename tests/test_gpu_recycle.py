@@ -1,0 +1,97 @@
+"""Recycled sweeps (wm_solve_opts.recycle): FP64 factors at anchor frequencies,
+FP64 GMRES refinement with the nearest anchor's factors everywhere else.
+
+A recycled result is an FP64 solve of its own frequency's matrix, refined until
+the correction is below 1e-12 of the solution, so it must meet the same bars as
+the FP64 path: LAPACK's backward-error criterion against the oracle's matrix
+(||Z_o (x - x_o)||_1 / (||Z_o||_1 ||x_o||_1 n eps) <= 30), Zin within 1e-10 of
+the FP64 sweep wherever kappa_1 u64 <= 1e-11, and elsewhere within the
+kappa-limited distance that separates any two backward-stable FP64 solutions.
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+U64 = 2.0 ** -53
+
+
+def _light(G, d, n):
+    p = os.path.join(d, f"light{n}.geo")
+    if not os.path.exists(p):
+        G.lightning(p, n)
+    return p
+
+
+def test_recycled_lightning_band_vs_oracle(wm, geometries, oracle, ctx, geo_dir):
+    """12 log-spaced frequencies over 10 kHz - 10 MHz, anchors within a factor 2:
+    every frequency backward-stable against the oracle's own matrix."""
+    p = _light(geometries, geo_dir, 1024)
+    t = oracle.load(p)
+    g = wm.Geometry.load(ctx, p)
+    gap = geometries.nearest_basis(t, (0, 0, 19.9))
+    f = np.logspace(-2, 1, 12).astype(np.float32)
+    z64, I64, s64 = wm.solve_sweep(g, f, [gap], "f64", cond_est=True)
+    zr, Ir, sr = wm.solve_sweep(g, f, [gap], "f64", recycle=2.0)
+    for fr, s in zip(f, sr):
+        print(f"f={fr:.4g} anchor={s['anchor_mhz']:.4g} iters={s['iters']} fell_back={s['fell_back']} "
+              f"dx_rel={s['dx_rel']:.2e}")
+    anchors = [i for i, s in enumerate(sr) if s["anchor_mhz"] == float(f[i])]
+    deps = [i for i, s in enumerate(sr) if s["anchor_mhz"] != float(f[i])]
+    assert 3 <= len(anchors) <= 8 and deps, [s["anchor_mhz"] for s in sr]
+    n = t.npls
+    for i, fr in enumerate(f):
+        s = sr[i]
+        assert s["fell_back"] == 0 and s["anchor_mhz"] > 0
+        ratio_f = max(s["anchor_mhz"] / fr, fr / s["anchor_mhz"])
+        assert ratio_f <= 2.0 * (1 + 1e-6)  # never beyond the requested factor
+        zo, Xo, Zo = oracle.solve_freq(t, float(fr), gaps=(gap,))
+        d = Ir[i, 0] - Xo[:, 0]
+        dr = np.abs(Zo.astype(np.clongdouble) @ d.astype(np.clongdouble)).sum()
+        ratio = float(dr / (np.abs(Zo).sum(axis=0).max() * np.abs(Xo[:, 0]).sum() * n * U64))
+        rel = abs(zr[i, 0] - z64[i, 0]) / abs(z64[i, 0])
+        ku = s64[i]["cond1"] * U64
+        print(f"f={fr:.4g} anchor={s['anchor_mhz']:.4g} iters={s['iters']} kappa*u={ku:.2e} "
+              f"rel vs fp64={rel:.2e} backward ratio={ratio:.3f}")
+        assert ratio <= 30.0, (fr, ratio)
+        if ku <= 1e-11:
+            assert rel <= 1e-10, (fr, rel)
+        if i in deps:
+            assert 1 <= s["iters"] <= 96, s
+
+
+def test_recycled_grid_sweep_matches_fp64(wm, geometries, ctx, geo_dir):
+    """A planar grid (N = 1024, two feeds) over 12 frequencies linearly spaced in
+    1 - 100 MHz (the C2 sweep's shape): anchors within a factor 1.5, every Zin
+    within 1e-10 of the FP64 sweep (the grid is well conditioned there)."""
+    p = os.path.join(geo_dir, "rgrid1024.geo")
+    geometries.planar_grid(p, 1024, seed=7)
+    g = wm.Geometry.load(ctx, p)
+    f = np.linspace(1.0, 100.0, 12).astype(np.float32)
+    z64, I64, _ = wm.solve_sweep(g, f, [5, 700], "f64")
+    zr, Ir, sr = wm.solve_sweep(g, f, [5, 700], "f64", recycle=1.5)
+    rel = (np.abs(zr - z64) / np.abs(z64)).max(axis=1)
+    print([(float(a), s["anchor_mhz"], s["iters"], float(e)) for a, s, e in zip(f, sr, rel)])
+    assert (rel <= 1e-10).all(), rel
+    relI = np.abs(Ir - I64).max(axis=2) / np.abs(I64).max(axis=2)
+    assert (relI <= 1e-9).all(), relI
+    assert any(s["anchor_mhz"] != float(x) for x, s in zip(f, sr))
+
+
+def test_recycled_result_independent_of_batch(wm, geometries, ctx, geo_dir):
+    """An item's GMRES refinement depends only on its own matrix and its anchor's
+    factors: the same frequencies in batches of 4 (anchors chosen within each
+    batch) agree with one batch wherever the anchor is the same, bit for bit."""
+    p = _light(geometries, geo_dir, 1024)
+    g = wm.Geometry.load(ctx, p)
+    t = wm.configwire(p, cwd=geo_dir)
+    gap = geometries.nearest_basis(t, (0, 0, 19.9))
+    f = np.logspace(0, 1, 8).astype(np.float32)
+    za, _, sa = wm.solve_sweep(g, f, [gap], "f64", recycle=1.4)
+    zb, _, sb = wm.solve_sweep(g, f, [gap], "f64", recycle=1.4, batch=4)
+    same = [i for i in range(len(f)) if sa[i]["anchor_mhz"] == sb[i]["anchor_mhz"]]
+    assert same
+    for i in same:
+        assert za[i, 0] == zb[i, 0], (f[i], za[i, 0], zb[i, 0])

@@ -63,13 +63,14 @@ def test_recycled_lightning_band_vs_oracle(wm, geometries, oracle, ctx, geo_dir)
 
 
 def test_recycled_grid_sweep_matches_fp64(wm, geometries, ctx, geo_dir):
-    """A planar grid (N = 1024, two feeds) over 12 frequencies linearly spaced in
-    1 - 100 MHz (the C2 sweep's shape): anchors within a factor 1.5, every Zin
-    within 1e-10 of the FP64 sweep (the grid is well conditioned there)."""
+    """A planar grid (N = 1024, two feeds, 10 x 10 m) over 12 frequencies linearly
+    spaced in 1 - 5 MHz, where it is electrically small to moderate (k D <= 1.5):
+    anchors within a factor 1.5, every Zin within 1e-10 of the FP64 sweep (the
+    grid is well conditioned there)."""
     p = os.path.join(geo_dir, "rgrid1024.geo")
     geometries.planar_grid(p, 1024, seed=7)
     g = wm.Geometry.load(ctx, p)
-    f = np.linspace(1.0, 100.0, 12).astype(np.float32)
+    f = np.linspace(1.0, 5.0, 12).astype(np.float32)
     z64, I64, _ = wm.solve_sweep(g, f, [5, 700], "f64")
     zr, Ir, sr = wm.solve_sweep(g, f, [5, 700], "f64", recycle=1.5)
     rel = (np.abs(zr - z64) / np.abs(z64)).max(axis=1)
@@ -78,6 +79,14 @@ def test_recycled_grid_sweep_matches_fp64(wm, geometries, ctx, geo_dir):
     relI = np.abs(Ir - I64).max(axis=2) / np.abs(I64).max(axis=2)
     assert (relI <= 1e-9).all(), relI
     assert any(s["anchor_mhz"] != float(x) for x, s in zip(f, sr))
+    # an electrically large sweep (1 - 100 MHz on the 10 m grid, k D up to 29):
+    # the covering factor shrinks to 1 + 0.5 / (k D), below the spacing, so every
+    # frequency is factored itself and the result is the FP64 sweep's
+    f2 = np.linspace(1.0, 100.0, 12).astype(np.float32)
+    z2, _, s2 = wm.solve_sweep(g, f2, [5, 700], "f64", recycle=1.5)
+    z2_64, _, _ = wm.solve_sweep(g, f2, [5, 700], "f64")
+    assert all(s["anchor_mhz"] == float(x) for x, s in zip(f2, s2))
+    assert np.array_equal(z2, z2_64)
 
 
 def test_recycled_result_independent_of_batch(wm, geometries, ctx, geo_dir):

@@ -147,6 +147,20 @@ int wm_tune_getrf(wm_ctx *ctx, int prec, int n, int batch, int *best_panels, dou
 int wm_getrs_batched(wm_ctx *ctx, int prec, int n, int nrhs, const void *A, int lda, int64_t strideA,
                      const int *ipiv, int64_t strideP, wm_c128 *B, int ldb, int64_t strideB, int batch);
 
+/* wm_getrs_batched without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  Its staging,
+ * permutation and packet buffers live in a 32 MiB device array of the library
+ * that each GPU holds until the process ends; sizes needing more than that
+ * (batch * (4 n + 16 strideB + 64 n nr') bytes with nr' = min(nrhs, 8)
+ * rounded down to a power of two, each term rounded up to 256) are rejected
+ * with WM_ERR_NOMEM.  Same arguments and results as wm_getrs_batched.  Every
+ * call on one GPU uses the same array, so calls that overlap on one GPU race:
+ * serialise them (one context's calls already are).  The profiler does not
+ * time its launches. */
+int wm_getrs_batched_noalloc(wm_ctx *ctx, int prec, int n, int nrhs, const void *A, int lda, int64_t strideA,
+                             const int *ipiv, int64_t strideP, wm_c128 *B, int ldb, int64_t strideB, int batch);
+
 /* ---- whole solves ------------------------------------------------------------ */
 typedef struct {
     int prec;              /* WM_PREC_* */

@@ -245,6 +245,7 @@ int wm_init(int device, wm_ctx **out)
     WM_HIP_TRY(hipMemset(c->d_tag, 0, sizeof(uint32_t)));
     WM_HIP_TRY(hipMemset(c->d_sync, 0, sizeof(int) * wm_ctx::kSyncCap));
     WM_HIP_TRY(hipMemset(c->d_err, 0, sizeof(int)));
+    WM_TRY(noalloc_init(device));
     *out = c;
     return WM_OK;
 }

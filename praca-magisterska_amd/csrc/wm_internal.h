@@ -171,6 +171,10 @@ int residual_batched(wm_ctx *ctx, int n, int nrhs, const void *Z, int ldz, int64
 
 int check_device_error(wm_ctx *ctx, hipStream_t s);
 
+// Device address of the allocation-free solve's scratch array on `device`
+// (must be current), taken once per device.
+int noalloc_init(int device);
+
 // Captured factorizations of a context (see getrf_batched).
 void drop_graphs(wm_ctx *ctx);
 hipGraphExec_t find_graph(wm_ctx *ctx, const GraphKey &key);

@@ -12,6 +12,8 @@
 // blocks owned by workgroups with a smaller id in the same grid, which are
 // dispatched before it, so the chain always makes progress; every wait is
 // bounded.  Arithmetic is FP64; complex64 factors are widened on load.
+#include <mutex>
+
 #include "wm_cplx.h"
 #include "wm_internal.h"
 
@@ -400,37 +402,61 @@ int perm_batched(wm_ctx *ctx, int n, const int *ipiv, int64_t strideP, int *perm
 
 namespace {
 
+// Buffers of a triangular solve supplied by the caller instead of the
+// context's (the allocation-free entry point): staging of B (batch * strideB
+// complex128) and the packets of both triangles, zeroed by the caller, whose
+// epochs then start at 1.
+struct SolveBufs {
+    double2 *stage = nullptr;
+    uint64_t *pkt = nullptr;
+    int epoch = 0;
+};
+
+// Packets of the two triangles of a batched solve.
+size_t solve_pkts(int n, int nrhs, int batch) { return 2 * (size_t)batch * (n / TB) * rhs_chunk(nrhs) * 256; }
+
 // Staging of B per launch member in ctx->ws_solve; the solved values'
 // packets in ctx->ws_pkt, per triangle and member nblk x NR x 256 packets.
-// trans: solve A^H X = B instead of A X = B.
+// trans: solve A^H X = B instead of A X = B.  ext: caller-supplied buffers
+// (nothing allocated, no profiler events).
 int getrs_any(wm_ctx *ctx, int prec, int n, int nrhs, const void *A, int lda, int64_t strideA, const int *perm,
               int64_t strideQ, void *B, int ldb, int64_t strideB, int batch, hipStream_t s, const int *d_bidx,
-              bool trans, const int *d_aidx)
+              bool trans, const int *d_aidx, SolveBufs *ext = nullptr)
 {
     if (n <= 0 || n % TB != 0 || nrhs <= 0 || batch <= 0 || ldb < n) {
         set_error("getrs: bad arguments (n=%d multiple of %d, nrhs=%d, batch=%d)", n, TB, nrhs, batch);
         return WM_ERR_ARG;
     }
     const int nblk = n / TB;
-    WM_TRY(ctx->ws_solve.ensure((size_t)batch * strideB * sizeof(double2)));
-    double2 *Ystage = (double2 *)ctx->ws_solve.ptr;
-    // The packets live in their own buffer and only ever carry epoch tags of
-    // this context, so a wait can never be satisfied by stale data.  The
-    // buffer is zeroed whenever it is (re)allocated.
     const int nr_max = rhs_chunk(nrhs);
     const size_t tri_pkts = (size_t)batch * nblk * nr_max * 256;  // packets per triangle
-    const size_t pkt_bytes = 2 * tri_pkts * sizeof(uint64_t);
-    if (ctx->ws_pkt.bytes < pkt_bytes || !ctx->ws_pkt.ptr) {
-        WM_TRY(ctx->ws_pkt.ensure(pkt_bytes));
-        const int64_t nf = (int64_t)(ctx->ws_pkt.bytes / sizeof(int));
-        hipLaunchKernelGGL(k_zero_words, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, s, (int *)ctx->ws_pkt.ptr, nf);
-        ctx->solve_epoch = 0;
-    }
-    uint64_t *pkt = (uint64_t *)ctx->ws_pkt.ptr;
-    if (ctx->solve_epoch > (1 << 30)) {  // keep the tags unique: restart from a zeroed buffer
-        const int64_t nf = (int64_t)(ctx->ws_pkt.bytes / sizeof(int));
-        hipLaunchKernelGGL(k_zero_words, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, s, (int *)pkt, nf);
-        ctx->solve_epoch = 0;
+    double2 *Ystage = nullptr;
+    uint64_t *pkt = nullptr;
+    int *epoch = nullptr;
+    if (ext) {
+        Ystage = ext->stage;
+        pkt = ext->pkt;
+        epoch = &ext->epoch;
+    } else {
+        WM_TRY(ctx->ws_solve.ensure((size_t)batch * strideB * sizeof(double2)));
+        Ystage = (double2 *)ctx->ws_solve.ptr;
+        // The packets live in their own buffer and only ever carry epoch tags of
+        // this context, so a wait can never be satisfied by stale data.  The
+        // buffer is zeroed whenever it is (re)allocated.
+        const size_t pkt_bytes = 2 * tri_pkts * sizeof(uint64_t);
+        if (ctx->ws_pkt.bytes < pkt_bytes || !ctx->ws_pkt.ptr) {
+            WM_TRY(ctx->ws_pkt.ensure(pkt_bytes));
+            const int64_t nf = (int64_t)(ctx->ws_pkt.bytes / sizeof(int));
+            hipLaunchKernelGGL(k_zero_words, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, s, (int *)ctx->ws_pkt.ptr, nf);
+            ctx->solve_epoch = 0;
+        }
+        pkt = (uint64_t *)ctx->ws_pkt.ptr;
+        if (ctx->solve_epoch > (1 << 30)) {  // keep the tags unique: restart from a zeroed buffer
+            const int64_t nf = (int64_t)(ctx->ws_pkt.bytes / sizeof(int));
+            hipLaunchKernelGGL(k_zero_words, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, s, (int *)pkt, nf);
+            ctx->solve_epoch = 0;
+        }
+        epoch = &ctx->solve_epoch;
     }
     // the lower solve reads B (staged per member) and writes y into B; the upper solves in place
     hipLaunchKernelGGL(k_stage, dim3((n + 255) / 256, batch), dim3(256), 0, s, (const double2 *)B, ldb, strideB, n, nrhs,
@@ -442,13 +468,13 @@ int getrs_any(wm_ctx *ctx, int prec, int n, int nrhs, const void *A, int lda, in
         a.bidx = d_bidx;
         a.aidx = d_aidx;
         a.Bin = Ystage; a.Y = (double2 *)B; a.pkt = pkt;
-        a.epoch = ++ctx->solve_epoch;
+        a.epoch = ++*epoch;
         const int nr = rhs_chunk(nrhs - r0);
         const double esz = prec == WM_PREC_F64 ? 16.0 : 8.0;
         const double fl = batch * 4.0 * (double)n * n * nr, by = batch * 0.5 * (double)n * n * esz;
         for (int up = 0; up < 2; ++up) {
             if (up) { a.Bin = nullptr; a.pkt = pkt + tri_pkts; }
-            prof_begin(ctx, K_TRSV, s);
+            if (!ext) prof_begin(ctx, K_TRSV, s);
             if (prec == WM_PREC_F64) {
                 if (trans) trsv_dispatch<double, true>(a, batch, r0, up, s);
                 else trsv_dispatch<double>(a, batch, r0, up, s);
@@ -456,7 +482,7 @@ int getrs_any(wm_ctx *ctx, int prec, int n, int nrhs, const void *A, int lda, in
                 if (trans) trsv_dispatch<float, true>(a, batch, r0, up, s);
                 else trsv_dispatch<float>(a, batch, r0, up, s);
             }
-            prof_end(ctx, K_TRSV, s, fl, by);
+            if (!ext) prof_end(ctx, K_TRSV, s, fl, by);
         }
     }
     if (trans) {  // x = P^T w, through the staging buffer
@@ -532,6 +558,80 @@ int residual_batched(wm_ctx *ctx, int n, int nrhs, const void *Z, int ldz, int64
 }
 
 }  // namespace wm
+
+// ---------------------------------------------------------------------------
+// Allocation-free triangular solves: the staging, permutation and packet
+// buffers of wm_getrs_batched live in `scratch`, a file-scope device array of
+// kNoAllocBytes that every GPU the library runs on holds until the process
+// ends.  Its device address is taken once per device (wm::noalloc_init, from
+// wm_init with that device current); host code uses that address, never the
+// array's name.  Calls on one GPU share the array: overlapping calls race.
+namespace {
+constexpr size_t kNoAllocBytes = (size_t)32 << 20;
+constexpr int kNoAllocDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+void *g_scratch_dev[kNoAllocDevices] = {};
+std::mutex g_scratch_mu;
+}  // namespace
+
+namespace wm {
+
+int noalloc_init(int device)
+{
+    if (device < 0 || device >= kNoAllocDevices) {
+        set_error("scratch: device %d beyond %d", device, kNoAllocDevices);
+        return WM_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_dev[device]) return WM_OK;
+    void *p = nullptr;
+    WM_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_dev[device] = p;
+    return WM_OK;
+}
+
+}  // namespace wm
+
+extern "C" int wm_getrs_batched_noalloc(wm_ctx *ctx, int prec, int n, int nrhs, const void *A, int lda,
+                                        int64_t strideA, const int *ipiv, int64_t strideP, wm_c128 *B, int ldb,
+                                        int64_t strideB, int batch)
+{
+    using namespace wm;
+    if (!ctx || !A || !ipiv || !B || n <= 0 || n % TB != 0 || nrhs <= 0 || batch <= 0 || ldb < n || lda < n ||
+        strideB < (int64_t)ldb * nrhs || (prec != WM_PREC_F64 && prec != WM_PREC_F32 && prec != WM_PREC_BF16X3)) {
+        set_error("wm_getrs_batched_noalloc: bad arguments (n=%d a multiple of %d, nrhs=%d, batch=%d, ldb=%d, "
+                  "strideB=%lld)", n, TB, nrhs, batch, ldb, (long long)strideB);
+        return WM_ERR_ARG;
+    }
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t perm_b = up((size_t)batch * n * sizeof(int));
+    const size_t stage_b = up((size_t)batch * strideB * sizeof(double2));
+    const size_t pkt_b = up(solve_pkts(n, nrhs, batch) * sizeof(uint64_t));
+    if (perm_b + stage_b + pkt_b > kNoAllocBytes) {
+        set_error("wm_getrs_batched_noalloc: n=%d, nrhs=%d, batch=%d need %zu bytes of scratch, the maximum is %zu "
+                  "(use wm_getrs_batched)", n, nrhs, batch, perm_b + stage_b + pkt_b, kNoAllocBytes);
+        return WM_ERR_NOMEM;
+    }
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    char *base = (char *)g_scratch_dev[ctx->device];
+    if (!base) {
+        set_error("wm_getrs_batched_noalloc: no scratch address for device %d", ctx->device);
+        return WM_ERR_STATE;
+    }
+    const hipStream_t s = ctx->stream;
+    int *perm = (int *)base;
+    SolveBufs sb;
+    sb.stage = (double2 *)(base + perm_b);
+    sb.pkt = (uint64_t *)(base + perm_b + stage_b);
+    // the previous call's packets carry the same epochs: start from zero
+    const int64_t nw = (int64_t)(pkt_b / sizeof(int));
+    hipLaunchKernelGGL(k_zero_words, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, s, (int *)sb.pkt, nw);
+    WM_TRY(perm_batched(ctx, n, ipiv, strideP, perm, n, batch, s));
+    WM_TRY(getrs_any(ctx, prec, n, nrhs, A, lda, strideA, perm, n, B, ldb, strideB, batch, s, nullptr, false, nullptr,
+                     &sb));
+    return check_device_error(ctx, s);
+}
 
 // Internal test hook (not part of include/wiremom.h): solves A^H X = B with the
 // factors and pivots of wm_getrf_batched (device buffers, as wm_getrs_batched).

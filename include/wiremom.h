@@ -130,7 +130,11 @@ int wm_getrf_batched(wm_ctx *ctx, int prec, int n, void *A, int lda, int64_t str
                      int *ipiv, int64_t strideP, int batch, int *info);
 /* Batched factorizations are captured into HIP graphs per (shape, buffers)
  * and replayed (default on; the profiler, when enabled, times direct launches).
- * wm_set_graphs(ctx, 0) launches directly and drops the captured graphs. */
+ * wm_set_graphs(ctx, 0) launches directly and drops the captured graphs.  A
+ * capture is invalidated by device-synchronizing calls (allocations, frees,
+ * synchronous copies) that another host thread makes while it is open: a
+ * context used while other threads drive the same device should launch
+ * directly (the request server's lanes do). */
 int wm_set_graphs(wm_ctx *ctx, int on);
 /* Factorization tuning.  The blocked LU applies a trailing update after every
  * `panels` panels of 64 columns (K = 64 * panels), per factor precision and
@@ -310,7 +314,8 @@ typedef struct {
     size_t hbm_budget;     /* bytes the server may hold for requests, 0 = 80% of free HBM */
     int max_inflight;      /* frequencies in flight at once, 0 = auto */
     int prec;              /* WM_PREC_* for every request */
-    int streams;           /* device streams used for overlap, default 2 */
+    int streams;           /* lanes: worker threads with their own device stream and workspaces, whose
+                              batches run on the device at the same time; default 2, at most 4 */
     int size_policy;       /* as wm_solve_opts.size_policy, default 1 */
 } wm_server_opts;
 
@@ -322,13 +327,20 @@ int wm_server_create(wm_ctx *ctx, const wm_server_opts *o, wm_server **out);
  * the budget; admission never blocks the caller. */
 int wm_server_submit(wm_server *s, const wm_geom *g, int nfreq, const float *freq_mhz, int nrhs,
                      const int *gaps, float ampl, float phas, int64_t *id);
-/* Drives the device: admits queued requests, launches work, retires
- * finished frequencies.  Returns the number of requests completed by this
- * call in *done.  Non-blocking unless `wait` is nonzero (then it returns
- * after at least one request completed or nothing is pending). */
+/* Drives the device: retires the batches the lanes finished, admits queued
+ * requests, hands a batch to every idle lane and completes the requests whose
+ * frequencies are all done; the number completed by this call is in *done.
+ * Non-blocking unless `wait` is nonzero: then it returns once a request has
+ * completed, a request has been submitted, or nothing is pending.  Call it
+ * from one thread.  A batch that fails completes its requests with the error
+ * status (wm_server_result returns it). */
 int wm_server_step(wm_server *s, int wait, int *done);
+/* Batches solved so far, the host time during which at least one lane was
+ * busy, and the time during which two or more were (device overlap). */
+int wm_server_overlap(wm_server *s, int64_t *batches, double *busy_s, double *overlap_s);
 /* Results of a completed request (host arrays sized as in wm_solve_sweep);
- * latency = completion - submission in seconds.  Releases the request. */
+ * latency = completion - submission in seconds.  Releases the request.
+ * Returns the request's status: negative if a batch it belonged to failed. */
 int wm_server_result(wm_server *s, int64_t id, wm_c128 *I, wm_c128 *zin, wm_freq_stats *st,
                      double *latency_s);
 /* Ids of completed requests whose results have not been read (ascending, at

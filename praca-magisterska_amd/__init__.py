@@ -156,6 +156,7 @@ def lib() -> C.CDLL:
         L.wm_server_status.argtypes = [vp, ip, ip, ip, _p(C.c_size_t)]
         L.wm_server_done.argtypes = [vp, _p(C.c_int64), C.c_int, ip]
         L.wm_server_destroy.argtypes = [vp]
+        L.wm_server_overlap.argtypes = [vp, _p(C.c_int64), dp, dp]
         L.geoinp.argtypes = [C.c_int, C.c_int, C.c_int, ip, fp, fp, fp, ip, ip, fp, ip, ip, ip]
         L.geoinp.restype = None
         L.configwire.argtypes = [ip, C.c_int, fp, fp, fp, fp, fp, fp, C.c_int, ip, fp, ip, ip, ip, ip, ip, ip, fp,
@@ -569,11 +570,12 @@ class Server:
     """Continuous-batching request server (wm_server_*)."""
 
     def __init__(self, ctx: Context, prec="f32", hbm_budget: int = 0, max_inflight: int = 0,
-                 size_policy: bool = True):
+                 size_policy: bool = True, streams: int = 2):
         o = ServerOpts()
         lib().wm_server_opts_default(C.byref(o))
         o.prec = PREC_NAMES[prec] if isinstance(prec, str) else int(prec)
         o.hbm_budget, o.max_inflight, o.size_policy = hbm_budget, max_inflight, int(size_policy)
+        o.streams = int(streams)
         h = C.c_void_p()
         _check(lib().wm_server_create(ctx.h, C.byref(o), C.byref(h)))
         self.h, self.ctx = h, ctx
@@ -588,10 +590,18 @@ class Server:
         self.meta[rid.value] = (len(f), len(g), geom.npls)
         return rid.value
 
-    def step(self) -> int:
+    def step(self, wait: bool = False) -> int:
+        """Retires finished batches and starts new ones; with wait, blocks until a request
+        completes, one is submitted, or nothing is pending.  Returns the requests completed."""
         d = C.c_int()
-        _check(lib().wm_server_step(self.h, 0, C.byref(d)))
+        _check(lib().wm_server_step(self.h, int(wait), C.byref(d)))
         return d.value
+
+    def overlap(self):
+        """(batches, seconds with a lane busy, seconds with two or more lanes busy)."""
+        b, busy, over = C.c_int64(), C.c_double(), C.c_double()
+        _check(lib().wm_server_overlap(self.h, C.byref(b), C.byref(busy), C.byref(over)))
+        return b.value, busy.value, over.value
 
     def done(self, cap: int = 4096):
         """Ids of completed requests whose results have not been read."""

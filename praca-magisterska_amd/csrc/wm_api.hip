@@ -816,6 +816,7 @@ int run_sweep(wm_ctx *c, const wm_geom *g, const wm_solve_opts &o, std::vector<W
         // geometry (cleared here), route the lower ones, whose conditioning is
         // worse, to FP64 factors when refinement cannot reach tol in max_iter.
         std::stable_sort(items.begin(), items.end(), [](const WorkItem &a, const WorkItem &b) { return a.freq > b.freq; });
+        std::lock_guard<std::mutex> lk(g->rate_mu);
         g->rate_obs.clear();
     }
     int f0 = 0;

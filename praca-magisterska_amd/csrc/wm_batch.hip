@@ -421,6 +421,7 @@ int auto_batch(wm_ctx *ctx, int ld, int nrhs, int prec, double frac)
 
 double predict_rate(const wm_geom *g, float f, int prec)
 {
+    std::lock_guard<std::mutex> lk(g->rate_mu);
     for (int same = 1; same >= 0; --same) {
         const wm_geom::RateObs *above = nullptr, *below = nullptr;
         for (const auto &ob : g->rate_obs) {
@@ -444,6 +445,7 @@ double predict_rate(const wm_geom *g, float f, int prec)
 void record_rates(const WorkItem *items, int count)
 {
     auto put = [](const wm_geom *g, float f, int prec, double rate) {
+        std::lock_guard<std::mutex> lk(g->rate_mu);
         for (auto &ob : g->rate_obs)
             if (ob.freq == f && ob.prec == prec) {
                 ob.rate = rate;

@@ -102,6 +102,9 @@ struct wm_ctx {
     // panel workgroups assumed co-resident (0 = the occupancy query's count);
     // lowered only by tests, to split panel launches at small sizes
     int panel_resident_cap = 0;
+    // contexts that may run panel launches on the device at the same time (the
+    // request server's lanes): each takes this share of the resident workgroups
+    int panel_share = 1;
 };
 
 // Geometry as the fill kernel reads it.  Observation data per basis (SoA,
@@ -132,6 +135,7 @@ struct wm_geom {
         double rate;  // last contraction ||dx_k|| / ||dx_k-1|| (inf: singular factor)
     };
     mutable std::vector<RateObs> rate_obs;
+    mutable std::mutex rate_mu;  // guards rate_obs (the server's lanes record rates concurrently)
 };
 
 namespace wm {

@@ -1058,7 +1058,11 @@ int getrf_plan(wm_ctx *ctx, int n, int batch, GetrfPlan *pl)
     const int pmax = (n + RW - 1) / RW;
     int bpc = 0;
     WM_TRY((occupancy_panel<T, TPR>(&bpc)));
-    int resident = bpc * ctx->num_cu;
+    // panel workgroups of this context that may be resident at once: the
+    // occupancy, shared with the other contexts that factor on the device at
+    // the same time (panel_share), so that every cluster of every concurrent
+    // panel launch fits on the device together
+    int resident = bpc * ctx->num_cu / std::max(1, ctx->panel_share);
     if (ctx->panel_resident_cap > 0 && ctx->panel_resident_cap < resident) resident = ctx->panel_resident_cap;
     if (pmax > resident || bpc <= 0) {
         set_error("getrf: panel cluster of %d workgroups exceeds %d resident", pmax, resident);
@@ -1208,7 +1212,7 @@ int getrf_width(wm_ctx *ctx, int prec, int n)
     if ((f64 ? occupancy_panel<double, 2>(&bpc) : occupancy_panel<float, kTprF32>(&bpc)) != WM_OK || bpc <= 0) return 1;
     const int RW = f64 ? 128 : 256 / kTprF32;
     const int pmax = (n + RW - 1) / RW;
-    return std::max(1, bpc * ctx->num_cu / pmax);
+    return std::max(1, bpc * ctx->num_cu / std::max(1, ctx->panel_share) / pmax);
 }
 
 namespace {

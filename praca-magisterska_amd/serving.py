@@ -60,7 +60,10 @@ def replay(server, reqs, geoms, feeds, idle_sleep: float = 5e-4):
         if err:
             th.join()
             raise err[0]
-        server.step()
+        q, r, _, _ = server.status()
+        # block in the server while work is in flight (it wakes on a completion
+        # or a submission); poll while nothing has arrived yet
+        server.step(wait=q + r > 0)
         for k in server.done():
             got[k] = server.result(k)
         if len(got) < len(reqs):

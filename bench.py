@@ -212,7 +212,7 @@ def cpu_baseline(geo, freq, threads, fill_cols=(64, 1024), lu_m=4096):
             **{k: round(v, 3) for k, v in out.items()}, **host_info()}
 
 
-def serving_leg(wm, ctx, tmp, count, rate, warm=0, sizes=(512, 1024, 2048, 4096, 8192), recycle=0.0):
+def serving_leg(wm, ctx, tmp, count, rate, warm=0, sizes=(512, 1024, 2048, 4096, 8192), progress_every=0):
     """Poisson request stream against the continuous-batching server: planar
     grids with N uniform in `sizes`, 1-32 frequencies uniform in 10 kHz - 50 MHz,
     1-8 feeds, FP32 factors + FP64 refinement.  The first `warm` requests are
@@ -228,8 +228,11 @@ def serving_leg(wm, ctx, tmp, count, rate, warm=0, sizes=(512, 1024, 2048, 4096,
         geoms.append(g)
         feeds.append(np.random.RandomState(r["seed"]).choice(g.npls, r["nrhs"], replace=False) + 1)
     srv = wm.Server(ctx, "f32")
-    out = S.replay(srv, reqs, geoms, feeds)
+    out = S.replay(srv, reqs, geoms, feeds, progress_every=progress_every)
+    batches, busy, over = srv.overlap()
     srv.close()
+    for g in geoms:
+        g.close()
     lat = np.array(out["latency"][warm:])
     meas = reqs[warm:]
     nsolve = sum(len(r["freqs"]) for r in meas)
@@ -241,6 +244,7 @@ def serving_leg(wm, ctx, tmp, count, rate, warm=0, sizes=(512, 1024, 2048, 4096,
             "max_latency_s": float(lat.max()), "solves_per_s": nsolve / span,
             "makespan_s": span, "fell_back": int(sum(s["fell_back"] == 1 for s in st)),
             "routed_f64": int(sum(s["fell_back"] == 2 for s in st)), "prec": "f32",
+            "lanes": 2, "batches": batches, "lanes_busy_s": busy, "lanes_overlapped_s": over,
             "workload": f"Poisson stream; planar grids N in {list(sizes)}, 1-32 freqs U(10 kHz, 50 MHz), 1-8 feeds"}
 
 
@@ -480,10 +484,11 @@ def main():
 def serving_spec(wm, ctx, tmp, name):
     """C5 at its specification: capacity measured by a saturated stream (all
     requests at once), then 1000 requests after 50 warm-up at 0.8 x capacity."""
-    cap = serving_leg(wm, ctx, tmp, 60, 1e6)
+    cap = serving_leg(wm, ctx, tmp, 60, 1e6, progress_every=20)
     req_per_s = cap["requests"] / cap["makespan_s"]
     rate = 0.8 * req_per_s
-    res = serving_leg(wm, ctx, tmp, 1000, rate, warm=50)
+    print(f"serving spec: capacity {req_per_s:.2f} requests/s, replaying at {rate:.2f}/s", file=sys.stderr, flush=True)
+    res = serving_leg(wm, ctx, tmp, 1000, rate, warm=50, progress_every=50)
     return {"metric": "C5 serving: p50/p99 request latency at 0.8 x measured capacity", "device": name,
             "capacity_requests_per_s": req_per_s, "capacity_solves_per_s": cap["solves_per_s"],
             "arrival_rate_per_s": rate, **res}

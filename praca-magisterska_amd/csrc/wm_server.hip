@@ -40,6 +40,11 @@
 
 namespace {
 
+// Frequencies per lane batch at most: a batch is retired as a whole, so the
+// latency of its requests grows with it (32 frequencies of N = 8192 in mixed
+// precision hold about 52 GB and take about half a second).
+constexpr int kLaneBatch = 32;
+
 double now_s()
 {
     using namespace std::chrono;
@@ -104,6 +109,26 @@ using namespace wm;
 
 namespace {
 
+// Solves a lane's batch; when the device cannot hold its workspace (the other
+// lanes' batches hold theirs) the workspace is released and the batch is
+// solved in halves.
+int solve_split(wm_ctx *c, const wm_solve_opts &o, WorkItem *items, int count)
+{
+    int rc = solve_items(c, o, items, count, c->stream);
+    if (rc == WM_ERR_NOMEM && count > 1) {
+        c->ws_sweep.release();
+        const int h = count / 2;
+        rc = solve_split(c, o, items, h);
+        if (rc >= 0) {
+            const int r2 = solve_split(c, o, items + h, count - h);
+            if (r2 != WM_OK && (rc == WM_OK || r2 < 0)) rc = r2;
+        }
+        return rc;
+    }
+    if (rc >= 0 && o.prec != WM_PREC_F64) record_rates(items, count);
+    return rc;
+}
+
 void lane_main(wm_server *s, Lane *L)
 {
     for (;;) {
@@ -119,8 +144,7 @@ void lane_main(wm_server *s, Lane *L)
         } else {
             std::lock_guard<std::recursive_mutex> lk(L->ctx->mu);
             (void)hipSetDevice(L->ctx->device);
-            rc = solve_items(L->ctx, L->o, L->items.data(), (int)L->items.size(), L->ctx->stream);
-            if (rc >= 0 && L->o.prec != WM_PREC_F64) record_rates(L->items.data(), (int)L->items.size());
+            rc = solve_split(L->ctx, L->o, L->items.data(), (int)L->items.size());
         }
         {
             std::lock_guard<std::mutex> lk(L->mu);
@@ -214,7 +238,7 @@ bool dispatch(wm_server *s, Lane *L)
         if (r->g->ld == ld && r->next < (int)r->freqs.size()) nr = std::max(nr, r->nrhs);
     const size_t per = item_bytes(ld, nr, s->o.prec);
     int cap = (int)std::max<size_t>(1, s->budget / s->lanes.size() / per);
-    cap = std::min(cap, 64);
+    cap = std::min(cap, kLaneBatch);
     if (s->o.max_inflight > 0) cap = std::min(cap, s->o.max_inflight);
     std::vector<WorkItem> items;
     std::vector<Req *> owner;

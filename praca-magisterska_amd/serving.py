@@ -33,9 +33,10 @@ def poisson_requests(count: int, rate: float, seed: int = 5, sizes=(512, 1024, 2
     return out
 
 
-def replay(server, reqs, geoms, feeds, idle_sleep: float = 5e-4):
+def replay(server, reqs, geoms, feeds, idle_sleep: float = 5e-4, progress_every: int = 0):
     """Submits reqs[i] (structure geoms[i], feeds feeds[i]) at its arrival time
-    and steps the server until every request is complete.
+    and steps the server until every request is complete; with progress_every,
+    a line on stderr after every that many completions.
 
     Returns {"latency": [s per request], "results": [(zin, I, stats) per request],
     "makespan": s from the first arrival slot to the last completion}."""
@@ -66,6 +67,10 @@ def replay(server, reqs, geoms, feeds, idle_sleep: float = 5e-4):
         server.step(wait=q + r > 0)
         for k in server.done():
             got[k] = server.result(k)
+            if progress_every and len(got) % progress_every == 0:
+                import sys
+                print(f"replay: {len(got)}/{len(reqs)} requests done at {time.perf_counter() - t0:.1f} s",
+                      file=sys.stderr, flush=True)
         if len(got) < len(reqs):
             q, r, _, _ = server.status()
             if q + r == 0:

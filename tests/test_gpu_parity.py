@@ -366,14 +366,18 @@ def test_zin_matches_oracle_f64(wm, geometries, oracle, ctx, geo_dir):
         assert rel <= 1e-10, (fr, rel)
 
 
-def test_zin_known_answers_gpu(wm, geometries, ctx, geo_dir):
+@pytest.mark.parametrize("band", [(0, 4096), (4097, 20480)])
+def test_zin_known_answers_gpu(wm, geometries, ctx, geo_dir, band):
     """The FP64 GPU path reproduces every logged Zin (4 s.f.) of the reference
-    for the 25 m dipole, N = 512 ... 4096 segments."""
+    for the 25 m dipole: all 308 cases, N = 512 ... 20480 segments (180 up to
+    4096, 128 above), e.g. "wersja referencyjna/results_1_6_8192.txt":44
+    (100 MHz: 96.37 -j462.6) and results_1_2_20480.txt:44 (98.67 -j1019)."""
     cases = json.load(open(os.path.join(GOLD, "zin_kat.json")))["cases"]
     by_n = {}
     for c in cases:
-        if c["nseg"] <= 4096:
+        if band[0] <= c["nseg"] <= band[1]:
             by_n.setdefault(c["nseg"], []).append(c)
+    assert sum(len(v) for v in by_n.values()) == (180 if band[0] == 0 else 128)
     bad = []
     for nseg, cs in sorted(by_n.items()):
         g = wm.Geometry.load(ctx, _dipole(geometries, geo_dir, nseg))

@@ -53,7 +53,7 @@ CLASSES = {
 }
 # HBM bytes / algorithmic bytes of k_gemm_f64, from FETCH_SIZE + WRITE_SIZE passes
 # calibrated on a plain copy (profiles/gemm_f64_traffic_c3_n8192.md)
-GEMM_F64_HBM_RATIO = 1.88
+GEMM_F64_HBM_RATIO = 1.91
 REF_BEST_PUBLISHED = 0.148  # freq-solves/s at N=8192, reference's best variant (GPU fill + MKL LU, 6 procs)
 U64 = 2.0 ** -53
 
@@ -306,6 +306,8 @@ def main():
     ap.add_argument("--serving-spec", action="store_true",
                     help="only the serving stream at its specification: 1000 requests after 50 warm-up at "
                          "0.8 x the measured capacity")
+    ap.add_argument("--profile-only", action="store_true",
+                    help="only the warmup and timed steps of the value sweep (for rocprofv3), then a short JSON line")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run a plain device copy of MIB MiB (three times), the reference for HBM counters")
     ap.add_argument("--requests", type=int, default=48)
@@ -357,6 +359,15 @@ def main():
     if rx:
         rx.roctxRangePop()
     value = a.steps * len(freqs) / dt
+    if a.profile_only:
+        prof = profiled(ctx, lambda: sweep(a.prec, a.recycle))
+        out = {"value": value, "unit": "freq-solves/s", "steps": a.steps, "warmup": a.warmup,
+               "recycle": a.recycle, "prec": a.prec, "hip_graphs": not a.no_graphs,
+               "roofline": roofline(prof), "kernel_ms_profiled_step": {k: round(v[0], 3) for k, v in prof.items()}}
+        if a.calibrate_copy:
+            out["calibrate_copy_bytes"] = calibrate_copy(wm, ctx, a.calibrate_copy)
+        print(json.dumps(out))
+        return
 
     # the plain FP64 GPU path (one LU per frequency, the drop-in) timed the same way
     ref_steps = max(2, a.steps // 2)
@@ -467,18 +478,24 @@ def main():
                   "and rooflines from one separate profiled step",
     }
     if a.calibrate_copy:
-        import ctypes as C
-        L = wm.lib()
-        nbytes = a.calibrate_copy * 2 ** 20
-        src, dst = C.c_void_p(), C.c_void_p()
-        wm._check(L.wm_dev_alloc(ctx.h, nbytes, C.byref(src)))
-        wm._check(L.wm_dev_alloc(ctx.h, nbytes, C.byref(dst)))
-        for _ in range(3):
-            wm._check(L.wm_memcpy_d2d(ctx.h, dst, src, nbytes))
-        wm._check(L.wm_dev_free(ctx.h, src))
-        wm._check(L.wm_dev_free(ctx.h, dst))
-        out["calibrate_copy_bytes"] = nbytes
+        out["calibrate_copy_bytes"] = calibrate_copy(wm, ctx, a.calibrate_copy)
     print(json.dumps(out))
+
+
+def calibrate_copy(wm, ctx, mib):
+    """Three plain device-to-device copies of mib MiB (the runtime's copy kernel),
+    the reference for HBM byte counters."""
+    import ctypes as C
+    L = wm.lib()
+    nbytes = mib * 2 ** 20
+    src, dst = C.c_void_p(), C.c_void_p()
+    wm._check(L.wm_dev_alloc(ctx.h, nbytes, C.byref(src)))
+    wm._check(L.wm_dev_alloc(ctx.h, nbytes, C.byref(dst)))
+    for _ in range(3):
+        wm._check(L.wm_memcpy_d2d(ctx.h, dst, src, nbytes))
+    wm._check(L.wm_dev_free(ctx.h, src))
+    wm._check(L.wm_dev_free(ctx.h, dst))
+    return nbytes
 
 
 def serving_spec(wm, ctx, tmp, name):

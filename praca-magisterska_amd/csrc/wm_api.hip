@@ -173,6 +173,18 @@ __global__ void k_pad_identity(double2 *A, int lda, int64_t stride, int n0, int 
     A[(size_t)b * stride + i + (size_t)j * lda] = make_double2(i == j ? 1.0 : 0.0, 0.0);
 }
 
+__global__ void k_add_diag_list(double2 *A, int lda, const int *idx, const double2 *val, int n)
+{
+    // A[i][i] += val for the listed (distinct) indices
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const size_t o = (size_t)idx[t] * lda + idx[t];
+    double2 a = A[o];
+    a.x += val[t].x;
+    a.y += val[t].y;
+    A[o] = a;
+}
+
 __global__ void k_zero_c128(double2 *p, int64_t n)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1104,27 +1116,83 @@ void LU_cuBLAS(int n, int nrhs, wm_c128 *ci_cu, wm_c128 *cu_cu, wm_c128 *cz, int
     else if (rc < 0) std::fprintf(stderr, "LU_cuBLAS: %s\n", wm_last_error());
 }
 
+// One frequency of the reference's loop with Z kept on the device between the
+// fill and the LU: the fill writes the padded matrix into the context's
+// workspace (grown once, reused by later calls), Z goes to the caller's cz once,
+// the hard-wired loads are added on the device and on the host copy alike, and
+// the same device matrix is factored and solved.
+static int calc_by_freq(float freq, float *x, float *y, float *z, float *rad, int *ipc, int *ips, int *ise,
+                        int nsmax, int npls, int ienv, float sgnx, float sgny, float sgnz, float sgnenv, int iload,
+                        float phas, int ngap, float ampl, wm_c128 *cz, wm_c128 *cu_cu, wm_c128 *ci_cu, int *info)
+{
+    if (npls <= 0 || ngap < 1 || ngap > npls || !cz || !cu_cu || !ci_cu) {
+        set_error("CalculateByFreq: bad arguments (npls=%d, ngap=%d)", npls, ngap);
+        return WM_ERR_ARG;
+    }
+    wm_ctx *c = default_ctx();
+    if (!c) return WM_ERR_HIP;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    wm_geom *g = nullptr;
+    WM_TRY(wm_geom_create_env(c, x, y, z, rad, ipc, ips, ise, nsmax, npls, ienv, sgnx, sgny, sgnz, sgnenv, &g));
+    std::unique_ptr<wm_geom, int (*)(wm_geom *)> guard(g, wm_geom_destroy);
+    const int ld = g->ld;
+    const hipStream_t s = c->stream;
+    const size_t zb = (size_t)ld * ld * sizeof(double2), vb = (size_t)ld * sizeof(double2);
+    WM_TRY(c->ws_sweep.ensure(zb + vb + (size_t)ld * sizeof(int) + 256));
+    double2 *Zd = (double2 *)c->ws_sweep.ptr, *Bd = (double2 *)((char *)Zd + zb);
+    int *piv = (int *)((char *)Bd + vb), *dinf = (int *)((char *)piv + (size_t)ld * sizeof(int));
+    const double ka = wavenumber(freq);
+    WM_TRY(fill_launch(c, g, 1, &ka, Zd, 0, nullptr, 0, s));
+    if (iload == 1) {  // hard-wired loads of the reference ("wersja gpu/Wire_MoM.cu":371-388)
+        std::vector<int> idx;
+        std::vector<double2> val;
+        auto add = [&](int nl, double re, double im) {
+            if (nl >= 0 && nl < npls) { idx.push_back(nl); val.push_back(make_double2(re, im)); }
+        };
+        for (int nl = 592 - 1; nl <= 2581 - 1; ++nl) add(nl, 1.e0, 2. * kPi * freq * 4.5);
+        add(2613 - 1, 30., 0.);
+        add(2582 - 1, 1.e12, 0.);
+        if (!idx.empty()) {
+            // indices and values staged in the right-hand-side buffer (overwritten below)
+            const size_t nb = idx.size();
+            WM_TRY(c->ws_solve.ensure(nb * (sizeof(int) + sizeof(double2))));
+            int *di = (int *)c->ws_solve.ptr;
+            double2 *dv = (double2 *)(di + nb);
+            WM_TRY(copy_h2d(di, idx.data(), nb * sizeof(int), s));
+            WM_TRY(copy_h2d(dv, val.data(), nb * sizeof(double2), s));
+            hipLaunchKernelGGL(k_add_diag_list, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, s, Zd, ld, di, dv,
+                               (int)nb);
+        }
+    }
+    WM_TRY(copy2d_d2h(cz, (size_t)npls * sizeof(wm_c128), Zd, (size_t)ld * sizeof(double2),
+                      (size_t)npls * sizeof(wm_c128), npls, s));
+    for (int q = 0; q < npls; ++q) cu_cu[q] = wm_c128{0.0, 0.0};
+    const double arg = kPi * phas / 180.;
+    cu_cu[ngap - 1] = wm_c128{(double)ampl * std::cos(arg), (double)ampl * std::sin(arg)};
+    for (int q = 0; q < npls; ++q) ci_cu[q] = cu_cu[q];
+    hipLaunchKernelGGL(k_zero_c128, dim3((unsigned)((ld + 255) / 256)), dim3(256), 0, s, Bd, (int64_t)ld);
+    WM_TRY(copy_h2d(Bd, ci_cu, (size_t)npls * sizeof(wm_c128), s));
+    WM_TRY(getrf_batched(c, WM_PREC_F64, ld, Zd, ld, (int64_t)ld * ld, piv, ld, 1, dinf, s));
+    WM_TRY(getrs_batched(c, WM_PREC_F64, ld, 1, Zd, ld, (int64_t)ld * ld, piv, ld, Bd, ld, ld, 1, s));
+    WM_TRY(check_device_error(c, s));
+    WM_TRY(copy_d2h(info, dinf, sizeof(int), s));
+    return copy_d2h(ci_cu, Bd, (size_t)npls * sizeof(wm_c128), s);
+}
+
 void CalculateByFreq(int i, int npls, int nfreq, float freq, float *x, float *y, float *z, float *rad, int *ipc,
                      int *ips, int *ise, const int nsmax, int ienv, float sgnx, float sgny, float sgnz, float sgnenv,
                      int iload, float phas, int ngap, float ampl, wm_c128 *cz, wm_c128 *cu_cu, wm_c128 *ci_cu, int rank)
 {
     (void)i;
     (void)nfreq;
-    const double ka = wavenumber(freq);
-    zmatrix_gpu(cz, x, y, z, rad, ipc, ips, ise, nsmax, ka, npls, ienv, sgnx, sgny, sgnz, sgnenv);
-    if (iload == 1) {  // hard-wired loads of the reference ("wersja gpu/Wire_MoM.cu":371-388)
-        auto add = [&](int nl, double re, double im) {
-            if (nl >= 0 && nl < npls) { cz[nl + (size_t)nl * npls].re += re; cz[nl + (size_t)nl * npls].im += im; }
-        };
-        for (int nl = 592 - 1; nl <= 2581 - 1; ++nl) add(nl, 1.e0, 2. * kPi * freq * 4.5);
-        add(2613 - 1, 30., 0.);
-        add(2582 - 1, 1.e12, 0.);
+    int info = 0;
+    const int rc = calc_by_freq(freq, x, y, z, rad, ipc, ips, ise, nsmax, npls, ienv, sgnx, sgny, sgnz, sgnenv, iload,
+                                phas, ngap, ampl, cz, cu_cu, ci_cu, &info);
+    if (rc < 0) {
+        std::fprintf(stderr, "CalculateByFreq: %s\n", wm_last_error());
+        return;
     }
-    for (int q = 0; q < npls; ++q) cu_cu[q] = wm_c128{0.0, 0.0};
-    const double arg = kPi * phas / 180.;
-    cu_cu[ngap - 1] = wm_c128{(double)ampl * std::cos(arg), (double)ampl * std::sin(arg)};
-    for (int q = 0; q < npls; ++q) ci_cu[q] = cu_cu[q];
-    LU_cuBLAS(npls, 1, ci_cu, cu_cu, cz, 1, rank);
+    if (info > 0) std::printf("FACTORIZATION ERROR\n");
     const wm_c128 zs = cdiv(cu_cu[ngap - 1], ci_cu[ngap - 1]);
     std::printf("\n Process - %d Freq = %.4g\t%.4g\t%.4g\n", rank, (double)freq, zs.re, zs.im);
     std::fflush(stdout);

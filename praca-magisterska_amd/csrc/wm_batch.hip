@@ -34,6 +34,22 @@ namespace {
 
 constexpr double kPi = 3.14159265358979323846;
 
+// Z[f] (n x n, leading dimension n) = T[f] (lg x lg) padded with the identity,
+// and the complex64 copy alike when Z32 is given.
+__global__ void k_expand(double2 *Z, int64_t sZ, float2 *Z32, int n, const double2 *T, const float2 *T32, int64_t sT,
+                         int lg)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)n * n) return;
+    const int f = blockIdx.y;
+    const int i = (int)(e % n), j = (int)(e / n);
+    const bool in = i < lg && j < lg;
+    const int64_t t = (int64_t)f * sT + i + (int64_t)j * lg;
+    const double2 v = in ? T[t] : make_double2(i == j ? 1.0 : 0.0, 0.0);
+    Z[(int64_t)f * sZ + e] = v;
+    if (Z32) Z32[(int64_t)f * sZ + e] = in ? T32[t] : make_float2(i == j ? 1.f : 0.f, 0.f);
+}
+
 __global__ void k_zero2(double2 *p, int64_t n)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -536,15 +552,21 @@ void route_items(const wm_solve_opts &o, WorkItem *items, int count)
 int solve_items(wm_ctx *c, const wm_solve_opts &o, WorkItem *items, int count, hipStream_t s)
 {
     if (count <= 0) return WM_OK;
-    const int n = items[0].g->ld;
+    // items of different sizes share the largest padded size: each matrix is
+    // the identity beyond its own (the factorization skips that part)
+    int n = 0;
+    for (int b = 0; b < count; ++b) n = std::max(n, items[b].g->ld);
+    bool uneven = false;
+    int ld_small = 0;  // largest padded size below n
+    for (int b = 0; b < count; ++b)
+        if (items[b].g->ld != n) {
+            uneven = true;
+            ld_small = std::max(ld_small, items[b].g->ld);
+        }
     int nr = 0;
     for (int b = 0; b < count; ++b) {
         const WorkItem &w = items[b];
         const int np = w.g->npls;
-        if (w.g->ld != n) {
-            set_error("solve_items: items of different sizes in one batch");
-            return WM_ERR_ARG;
-        }
         if (!w.gaps) {  // generators and plane waves
             for (int q = 0; q < w.nsrc; ++q)
                 if (w.src[q].basis < 1 || w.src[q].basis > np || w.src[q].rhs < 0 || w.src[q].rhs >= w.nrhs) {
@@ -632,6 +654,13 @@ int solve_items(wm_ctx *c, const wm_solve_opts &o, WorkItem *items, int count, h
     const size_t oRed = carve(recycle ? (size_t)(3 * gm + 4) * count * nr * sizeof(double2) : 0);
     const size_t oF = carve(recycle ? count * sizeof(int) : 0);
     const size_t oM2 = carve(recycle ? (size_t)count * (1 + nr) * sizeof(unsigned long long) : 0);
+    // uneven sizes: per-item sizes, and the fill's staging of up to kFillChunk
+    // smaller matrices before they are expanded to the batch's layout
+    constexpr int kFillChunk = 4;
+    const size_t oN = carve(uneven ? count * sizeof(int) : 0);
+    const size_t tZ = (size_t)ld_small * ld_small;
+    const size_t oT = carve(uneven ? tZ * sizeof(double2) * kFillChunk : 0);
+    const size_t oT32 = carve(uneven && mixed ? tZ * sizeof(float2) * kFillChunk : 0);
     const size_t oP = carve((size_t)n * count * sizeof(int)), oI = carve(count * sizeof(int));
     const size_t oM = carve((size_t)2 * count * nr * sizeof(unsigned long long));
     const size_t oA = carve(count * sizeof(int));
@@ -655,6 +684,9 @@ int solve_items(wm_ctx *c, const wm_solve_opts &o, WorkItem *items, int count, h
     double2 *Kry = (double2 *)(base + oK), *red = (double2 *)(base + oRed);
     int *dfac = (int *)(base + oF);
     unsigned long long *mx2 = (unsigned long long *)(base + oM2);
+    int *dnmem = uneven ? (int *)(base + oN) : nullptr;
+    double2 *Tst = (double2 *)(base + oT);
+    float2 *Tst32 = (float2 *)(base + oT32);
 
     // excitation: ampl*exp(j*pi*phas/180) at each feed ("wersja referencyjna/Wire_MoM.cu":307-309)
     std::vector<int> hcol(cne, 0), hidx(cne, -1), hlidx(cnl, -1);
@@ -682,17 +714,36 @@ int solve_items(wm_ctx *c, const wm_solve_opts &o, WorkItem *items, int count, h
             const WorkItem &w = items[b];
             const PlaneWaveArgs pa = plane_wave_args(w.pw[q], wavenumber_mhz(w.freq));
             hipLaunchKernelGGL(k_plane_wave, dim3((unsigned)((w.g->npls + 255) / 256)), dim3(256), 0, s,
-                               B + (size_t)b * sV + (size_t)w.pw[q].rhs * n, w.g->d_obs, n, w.g->npls, pa);
+                               B + (size_t)b * sV + (size_t)w.pw[q].rhs * n, w.g->d_obs, w.g->ld, w.g->npls, pa);
         }
 
-    // fill, one launch group per run of items sharing a geometry
+    // fill, one launch group per run of items sharing a geometry; a smaller
+    // structure's matrices are filled at their own padded size and expanded
+    if (uneven) {
+        std::vector<int> hn(count);
+        for (int b = 0; b < count; ++b) hn[b] = items[b].g->ld;
+        WM_TRY(copy_h2d(dnmem, hn.data(), count * sizeof(int), s));
+    }
     for (int b0 = 0; b0 < count;) {
         int b1 = b0 + 1;
         while (b1 < count && items[b1].g == items[b0].g) ++b1;
+        const wm_geom *g = items[b0].g;
         std::vector<double> ka(b1 - b0);
         for (int b = b0; b < b1; ++b) ka[b - b0] = wavenumber_mhz(items[b].freq);
-        WM_TRY(fill_launch(c, items[b0].g, b1 - b0, ka.data(), Z + (size_t)b0 * sZ, sZ,
-                           mixed ? (void *)(Z32 + (size_t)b0 * sZ) : nullptr, sZ, s));
+        if (g->ld == n) {
+            WM_TRY(fill_launch(c, g, b1 - b0, ka.data(), Z + (size_t)b0 * sZ, sZ,
+                               mixed ? (void *)(Z32 + (size_t)b0 * sZ) : nullptr, sZ, s));
+        } else {
+            const int lg = g->ld;
+            const int64_t sT = (int64_t)lg * lg;
+            for (int f0 = b0; f0 < b1; f0 += kFillChunk) {
+                const int nf = std::min(kFillChunk, b1 - f0);
+                WM_TRY(fill_launch(c, g, nf, ka.data() + (f0 - b0), Tst, sT, mixed ? (void *)Tst32 : nullptr, sT, s));
+                hipLaunchKernelGGL(k_expand, dim3((unsigned)((nn + 255) / 256), (unsigned)nf), dim3(256), 0, s,
+                                   Z + (size_t)f0 * sZ, sZ, mixed ? Z32 + (size_t)f0 * sZ : nullptr, n,
+                                   (const double2 *)Tst, (const float2 *)Tst32, sT, lg);
+            }
+        }
         b0 = b1;
     }
     // lumped loads on the diagonals ("wersja referencyjna/Wire_MoM.cu":344-363)
@@ -730,7 +781,7 @@ int solve_items(wm_ctx *c, const wm_solve_opts &o, WorkItem *items, int count, h
         auto factor_solve = [&](const std::vector<int> &lst) -> int {
             const int k = (int)lst.size();
             WM_TRY(upload_list(lst));
-            WM_TRY(getrf_batched(c, WM_PREC_F64, n, Z, n, sZ, ipiv, n, k, dinfo, s, dact));
+            WM_TRY(getrf_batched(c, WM_PREC_F64, n, Z, n, sZ, ipiv, n, k, dinfo, s, dact, dnmem));
             WM_TRY(perm_batched(c, n, ipiv, n, perm, n, k, s, dact));
             hipLaunchKernelGGL(k_copy_items, dim3((unsigned)((sV + 255) / 256), k), dim3(256), 0, s, X, B, sV, sV, dact);
             WM_TRY(getrs_perm_batched(c, WM_PREC_F64, n, nr, Z, n, sZ, perm, n, X, n, sV, k, s, dact));
@@ -779,7 +830,7 @@ int solve_items(wm_ctx *c, const wm_solve_opts &o, WorkItem *items, int count, h
         }
         for (int b = 0; b < count; ++b) items[b].fprec = WM_PREC_F64;
     } else if (!mixed) {
-        WM_TRY(getrf_batched(c, WM_PREC_F64, n, Z, n, sZ, ipiv, n, count, dinfo, s));
+        WM_TRY(getrf_batched(c, WM_PREC_F64, n, Z, n, sZ, ipiv, n, count, dinfo, s, nullptr, dnmem));
         WM_TRY(perm_batched(c, n, ipiv, n, perm, n, count, s));
         WM_HIP_TRY(hipMemcpyAsync(X, B, szV, hipMemcpyDeviceToDevice, s));
         WM_TRY(getrs_perm_batched(c, WM_PREC_F64, n, nr, Z, n, sZ, perm, n, X, n, sV, count, s));
@@ -814,10 +865,10 @@ int solve_items(wm_ctx *c, const wm_solve_opts &o, WorkItem *items, int count, h
             }
             WM_TRY(upload_list(L));
             if (np == count) {
-                WM_TRY(getrf_batched(c, pq, n, Z32, n, sZ, ipiv, n, count, dinfo, s));
+                WM_TRY(getrf_batched(c, pq, n, Z32, n, sZ, ipiv, n, count, dinfo, s, nullptr, dnmem));
                 WM_TRY(perm_batched(c, n, ipiv, n, perm, n, count, s));
             } else {
-                WM_TRY(getrf_batched(c, pq, n, Z32, n, sZ, ipiv, n, np, dinfo, s, dact));
+                WM_TRY(getrf_batched(c, pq, n, Z32, n, sZ, ipiv, n, np, dinfo, s, dact, dnmem));
                 WM_TRY(perm_batched(c, n, ipiv, n, perm, n, np, s, dact));
             }
             // X = 0, R = B for the items of this level
@@ -895,7 +946,7 @@ int solve_items(wm_ctx *c, const wm_solve_opts &o, WorkItem *items, int count, h
         if (!bad.empty()) {
             const int nb = (int)bad.size();
             WM_TRY(upload_list(bad));
-            WM_TRY(getrf_batched(c, WM_PREC_F64, n, Z, n, sZ, ipiv, n, nb, dinfo, s, dact));
+            WM_TRY(getrf_batched(c, WM_PREC_F64, n, Z, n, sZ, ipiv, n, nb, dinfo, s, dact, dnmem));
             WM_TRY(perm_batched(c, n, ipiv, n, perm, n, nb, s, dact));
             hipLaunchKernelGGL(k_copy_items, dim3((unsigned)((sV + 255) / 256), nb), dim3(256), 0, s, X, B, sV, sV, dact);
             WM_TRY(getrs_perm_batched(c, WM_PREC_F64, n, nr, Z, n, sZ, perm, n, X, n, sV, nb, s, dact));

@@ -64,7 +64,7 @@ struct Prof {
 namespace wm {
 // Captured factorization: precision, n, lda, strideA, A, ipiv, strideP, batch,
 // info, index table, outer-block depth, exchange buffer.
-typedef std::array<int64_t, 12> GraphKey;
+typedef std::array<int64_t, 13> GraphKey;
 }  // namespace wm
 
 struct wm_ctx {
@@ -147,8 +147,12 @@ int fill_launch(wm_ctx *ctx, const wm_geom *g, int nf, const double *ka, void *Z
 // LU: batched blocked right-looking factorization with partial pivoting.
 // d_bidx (device, optional): member i of the launch works on matrix d_bidx[i]
 // (the matrix of index m lives at base + m * stride); null means 0..batch-1.
+// d_nmem (device, optional, per matrix index): the matrix's own size, a
+// multiple of 64 not above n, rows and columns beyond it the identity; the
+// factorization skips the work there (its result is the identity's).
 int getrf_batched(wm_ctx *ctx, int prec, int n, void *A, int lda, int64_t strideA, int *ipiv,
-                  int64_t strideP, int batch, int *d_info, hipStream_t s, const int *d_bidx = nullptr);
+                  int64_t strideP, int batch, int *d_info, hipStream_t s, const int *d_bidx = nullptr,
+                  const int *d_nmem = nullptr);
 // Row permutation of each factorization: perm[t] = source row of position t.
 int perm_batched(wm_ctx *ctx, int n, const int *ipiv, int64_t strideP, int *perm, int64_t strideQ, int batch,
                  hipStream_t s, const int *d_bidx = nullptr);

@@ -99,6 +99,7 @@ struct PanelArgs {
     const int *bidx;  // launch member -> matrix index (null: identity)
     const uint32_t *tagp;  // device tag base of the current factorization (bumped per launch chunk)
     uint32_t toff;         // packet tags of this launch are *tagp + toff + 1 .. *tagp + toff + NB
+    const int *nmem;       // per matrix index: its own size (rows/columns beyond are the identity), or null
 };
 
 // Argmax combine: larger value wins, equal values go to the lower row.
@@ -182,6 +183,13 @@ __global__ __launch_bounds__(256, 3) void k_panel(PanelArgs<T> a)
     V *Ab = a.A + (size_t)b * a.strideA;
     const XLayout<V> X(a.pmax);
     char *xm = a.xbuf + (size_t)bl * X.mat_bytes;
+    if (a.nmem && k >= a.nmem[b]) {
+        // the member's own matrix ends above this panel: an identity panel, no
+        // interchanges (pivots k+1 .. k+NB) and an empty permutation
+        if (w == 0 && tid < NB) a.ipiv[(size_t)b * a.strideP + k + tid] = k + tid + 1;
+        if (w == 0 && tid == 0) X.perm(xm)[0] = 0;
+        return;
+    }
 
     T re[W], im[W];
 #pragma unroll
@@ -421,6 +429,7 @@ __global__ __launch_bounds__(256, 3) void k_panel(PanelArgs<T> a)
 //                      rows k..k+NB-1 still await earlier panels' updates)
 struct SwapArgs {
     int n, k, pmax, ntl, ntr, c_tr0, c_tr1, c_sw0, do_swap;
+    const int *nmem;  // per matrix index: its own size (identity beyond), or null
 };
 
 template <typename T, int CB>
@@ -442,6 +451,9 @@ __global__ __launch_bounds__(256) void k_swap_trsm(typename CT<T>::V *A, int lda
     const int c0 = tile < a.ntl ? tile * CB : trsm ? a.c_tr0 + (tile - a.ntl) * CB
                                                    : a.c_sw0 + (tile - a.ntl - a.ntr) * CB;
     const int cend = tile < a.ntl ? k : trsm ? a.c_tr1 : n;
+    // beyond the member's own size: no interchanges (its panel was the
+    // identity) and zero blocks right of it (their solve changes nothing)
+    if (a.nmem && (k >= a.nmem[b] || c0 >= a.nmem[b])) return;
 
     // gather the old contents of the touched rows, keep the new top block
     // (rows k..k+NB-1) of solved columns in LDS, then scatter.  Consecutive
@@ -544,6 +556,7 @@ __global__ __launch_bounds__(256) void k_swap_trsm(typename CT<T>::V *A, int lda
 // C = A[r0.., c0..] (mrows x ncols) -= A[r0.., kk..kk+K) * A[kk..kk+K, c0..); K a multiple of 64.
 struct GemmArgs {
     int r0, c0, mrows, ncols, kk, K;
+    const int *nmem;  // per matrix index: its own size (identity beyond, so its updates there are zero), or null
 };
 
 // Tile of workgroup g out of T, grouped per XCD; >= T means "no tile".
@@ -599,6 +612,7 @@ __global__ __launch_bounds__(256, 3) void k_gemm_f64(double2 *Abase, int lda, in
     int ti, tj;
     tile_coords(t, TI, TJ, ti, tj);
     const int i0 = ti * GZ_BM, j0 = tj * GZ_BN;
+    if (g.nmem && (g.r0 + i0 >= g.nmem[b] || g.c0 + j0 >= g.nmem[b])) return;
     const double2 *A = M + g.r0 + (size_t)g.kk * lda;  // L block
     const double2 *B = M + g.kk + (size_t)g.c0 * lda;  // U block
     double2 *C = M + g.r0 + (size_t)g.c0 * lda;
@@ -717,6 +731,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_f32(float2 *Abase, int lda, int
     int ti, tj;
     tile_coords(t, TI, TJ, ti, tj);
     const int i0 = ti * GC_BM, j0 = tj * GC_BN;
+    if (g.nmem && (g.r0 + i0 >= g.nmem[b] || g.c0 + j0 >= g.nmem[b])) return;
     const float2 *A = M + g.r0 + (size_t)g.kk * lda;
     const float2 *B = M + g.kk + (size_t)g.c0 * lda;
     float2 *C = M + g.r0 + (size_t)g.c0 * lda;
@@ -833,6 +848,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_b3(float2 *Abase, int lda, int6
     int ti, tj;
     tile_coords(t, TI, TJ, ti, tj);
     const int i0 = ti * GB_BM, j0 = tj * GB_BN;
+    if (g.nmem && (g.r0 + i0 >= g.nmem[b] || g.c0 + j0 >= g.nmem[b])) return;
     const float2 *A = M + g.r0 + (size_t)g.kk * lda;  // L block
     const float2 *B = M + g.kk + (size_t)g.c0 * lda;  // U block
     float2 *C = M + g.r0 + (size_t)g.c0 * lda;
@@ -1079,7 +1095,8 @@ int getrf_plan(wm_ctx *ctx, int n, int batch, GetrfPlan *pl)
 // so the sequence can be captured into a graph).
 template <typename T, int TPR>
 int getrf_impl(wm_ctx *ctx, const GetrfPlan &pl, int n, typename CT<T>::V *A, int lda, int64_t strideA, int *ipiv,
-               int64_t strideP, int batch, int *d_info, const int *d_bidx, hipStream_t s, int gk, int kSub)
+               int64_t strideP, int batch, int *d_info, const int *d_bidx, hipStream_t s, int gk, int kSub,
+               const int *d_nmem)
 {
     using V = typename CT<T>::V;
     constexpr int RW = 256 / TPR;
@@ -1121,6 +1138,7 @@ int getrf_impl(wm_ctx *ctx, const GetrfPlan &pl, int n, typename CT<T>::V *A, in
             pa.xbuf = G.xbuf + (size_t)b0 * X.mat_bytes;
             pa.bidx = G.bidx ? G.bidx + b0 : nullptr;
             pa.tagp = ctx->d_tag; pa.toff = (uint32_t)c;
+            pa.nmem = d_nmem;
             // per column j, (mr-j) scalings and (mr-j)(NB-j-1) complex multiply-adds
             prof_begin(ctx, kPanelClass, G.st);
             hipLaunchKernelGGL((k_panel<T, TPR>), dim3(P, m), dim3(256), 0, G.st, pa);
@@ -1132,6 +1150,7 @@ int getrf_impl(wm_ctx *ctx, const GetrfPlan &pl, int n, typename CT<T>::V *A, in
     auto swap = [&](const Grp &G, int c, int tr1, int do_swap, int tr0) {
         SwapArgs sa;
         sa.n = n; sa.k = c; sa.pmax = pmax; sa.do_swap = do_swap;
+        sa.nmem = d_nmem;
         sa.ntl = do_swap ? (c + CB - 1) / CB : 0;
         sa.c_tr0 = tr0; sa.c_tr1 = tr1;
         sa.ntr = tr1 > tr0 ? (tr1 - tr0 + CB - 1) / CB : 0;
@@ -1145,7 +1164,7 @@ int getrf_impl(wm_ctx *ctx, const GetrfPlan &pl, int n, typename CT<T>::V *A, in
     };
     auto gemm = [&](const Grp &G, int r0, int c0, int mrows, int ncols, int kk, int K) {
         if (mrows <= 0 || ncols <= 0) return;
-        const GemmArgs g{r0, c0, mrows, ncols, kk, K};
+        const GemmArgs g{r0, c0, mrows, ncols, kk, K, d_nmem};
         // C read and written, L and U read once; flops of the kernel's algorithm:
         // three real products per complex multiply-add (6 flops) for complex128
         // and complex64, four FP32-equivalent real products (8 flops, each taken
@@ -1220,17 +1239,18 @@ namespace {
 // Enqueues a whole factorization (info reset, tag bumps, all launch chunks).
 template <typename T, int TPR>
 int getrf_enqueue(wm_ctx *ctx, const GetrfPlan &pl, int n, void *A, int lda, int64_t strideA, int *ipiv,
-                  int64_t strideP, int batch, int *d_info, const int *d_bidx, hipStream_t s, int gk, int ks)
+                  int64_t strideP, int batch, int *d_info, const int *d_bidx, hipStream_t s, int gk, int ks,
+                  const int *d_nmem)
 {
     if (!d_bidx) hipLaunchKernelGGL(k_zero_int, dim3((batch + 255) / 256), dim3(256), 0, s, d_info, batch);
     else hipLaunchKernelGGL(k_zero_idx, dim3((batch + 255) / 256), dim3(256), 0, s, d_info, d_bidx, batch);
     return getrf_impl<T, TPR>(ctx, pl, n, (typename CT<T>::V *)A, lda, strideA, ipiv, strideP, batch, d_info, d_bidx,
-                              s, gk, ks);
+                              s, gk, ks, d_nmem);
 }
 
 template <typename T, int TPR>
 int getrf_run(wm_ctx *ctx, int prec, int n, void *A, int lda, int64_t strideA, int *ipiv, int64_t strideP, int batch,
-              int *d_info, const int *d_bidx, hipStream_t s, int gk)
+              int *d_info, const int *d_bidx, hipStream_t s, int gk, const int *d_nmem)
 {
     const int ks = getrf_block(ctx, prec, n);
     GetrfPlan pl;
@@ -1238,19 +1258,21 @@ int getrf_run(wm_ctx *ctx, int prec, int n, void *A, int lda, int64_t strideA, i
     const uint64_t ntags = (uint64_t)n;
     WM_TRY(prepare_xbuf(ctx, pl.xbytes, ntags, s));
     if (!ctx->use_graphs || ctx->prof.on) {
-        WM_TRY((getrf_enqueue<T, TPR>(ctx, pl, n, A, lda, strideA, ipiv, strideP, batch, d_info, d_bidx, s, gk, ks)));
+        WM_TRY((getrf_enqueue<T, TPR>(ctx, pl, n, A, lda, strideA, ipiv, strideP, batch, d_info, d_bidx, s, gk, ks,
+                                      d_nmem)));
     } else {
         // Replay a captured factorization of the same shape and buffers, or
         // capture one.  Its kernels read their tags from the device counter, so
         // every replay uses fresh tags.
         const GraphKey key{(int64_t)prec, (int64_t)n, (int64_t)lda, strideA, (int64_t)(intptr_t)A,
                            (int64_t)(intptr_t)ipiv, strideP, (int64_t)batch, (int64_t)(intptr_t)d_info,
-                           (int64_t)(intptr_t)d_bidx, (int64_t)ks, (int64_t)(intptr_t)ctx->ws_lu.ptr};
+                           (int64_t)(intptr_t)d_bidx, (int64_t)ks, (int64_t)(intptr_t)ctx->ws_lu.ptr,
+                           (int64_t)(intptr_t)d_nmem};
         hipGraphExec_t exec = find_graph(ctx, key);
         if (!exec) {
             WM_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            const int rc =
-                getrf_enqueue<T, TPR>(ctx, pl, n, A, lda, strideA, ipiv, strideP, batch, d_info, d_bidx, s, gk, ks);
+            const int rc = getrf_enqueue<T, TPR>(ctx, pl, n, A, lda, strideA, ipiv, strideP, batch, d_info, d_bidx, s,
+                                                 gk, ks, d_nmem);
             hipGraph_t graph = nullptr;
             const hipError_t ec = hipStreamEndCapture(s, &graph);
             if (rc != WM_OK) {
@@ -1290,7 +1312,7 @@ void keep_graph(wm_ctx *ctx, const GraphKey &key, hipGraphExec_t exec)
 }
 
 int getrf_batched(wm_ctx *ctx, int prec, int n, void *A, int lda, int64_t strideA, int *ipiv, int64_t strideP,
-                  int batch, int *d_info, hipStream_t s, const int *d_bidx)
+                  int batch, int *d_info, hipStream_t s, const int *d_bidx, const int *d_nmem)
 {
     if (n <= 0 || n % NB != 0 || lda < n || batch <= 0 || !A || !ipiv || !d_info) {
         set_error("getrf: bad arguments (n=%d must be a positive multiple of %d, lda=%d, batch=%d)", n, NB, lda,
@@ -1298,11 +1320,13 @@ int getrf_batched(wm_ctx *ctx, int prec, int n, void *A, int lda, int64_t stride
         return WM_ERR_ARG;
     }
     if (prec == WM_PREC_F64)
-        return getrf_run<double, 2>(ctx, prec, n, A, lda, strideA, ipiv, strideP, batch, d_info, d_bidx, s, 8);
+        return getrf_run<double, 2>(ctx, prec, n, A, lda, strideA, ipiv, strideP, batch, d_info, d_bidx, s, 8, d_nmem);
     if (prec == WM_PREC_F32)
-        return getrf_run<float, kTprF32>(ctx, prec, n, A, lda, strideA, ipiv, strideP, batch, d_info, d_bidx, s, 4);
+        return getrf_run<float, kTprF32>(ctx, prec, n, A, lda, strideA, ipiv, strideP, batch, d_info, d_bidx, s, 4,
+                                         d_nmem);
     if (prec == WM_PREC_BF16X3)
-        return getrf_run<float, kTprF32>(ctx, prec, n, A, lda, strideA, ipiv, strideP, batch, d_info, d_bidx, s, 2);
+        return getrf_run<float, kTprF32>(ctx, prec, n, A, lda, strideA, ipiv, strideP, batch, d_info, d_bidx, s, 2,
+                                         d_nmem);
     set_error("getrf: precision %d not available", prec);
     return WM_ERR_ARG;
 }
@@ -1354,7 +1378,7 @@ extern "C" int wmx_gemm_k(wm_ctx *ctx, int prec, int n, void *A, int K, int reps
 {
     using namespace wm;
     if (K <= 0 || K >= n || K % NB || reps <= 0) return WM_ERR_ARG;
-    const GemmArgs g{K, K, n - K, n - K, 0, K};
+    const GemmArgs g{K, K, n - K, n - K, 0, K, nullptr};
     for (int r = 0; r < reps; ++r)
         launch_gemm(prec == WM_PREC_F64 ? 8 : prec == WM_PREC_BF16X3 ? 2 : 4, A, n, 0, g, 1, nullptr, ctx->stream);
     WM_HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -1366,7 +1390,7 @@ extern "C" int wmx_gemm_step(wm_ctx *ctx, int prec, int n, void *A, int k)
     using namespace wm;
     const int m2 = n - k - NB;
     if (m2 <= 0 || n % NB || k % NB) return WM_ERR_ARG;
-    const GemmArgs g{k + NB, k + NB, m2, m2, k, NB};
+    const GemmArgs g{k + NB, k + NB, m2, m2, k, NB, nullptr};
     launch_gemm(prec == WM_PREC_F64 ? 8 : prec == WM_PREC_BF16X3 ? 2 : 4, A, n, 0, g, 1, nullptr, ctx->stream);
     WM_HIP_TRY(hipStreamSynchronize(ctx->stream));
     return WM_OK;
@@ -1395,6 +1419,7 @@ extern "C" int wmx_panel_step(wm_ctx *ctx, int prec, int n, void *A, int *ipiv, 
         pa.P = (n - k + RW - 1) / RW; pa.pmax = pmax;
         pa.ipiv = ipiv; pa.strideP = 0; pa.info = info;
         pa.err = ctx->d_err; pa.xbuf = xbuf; pa.bidx = nullptr; pa.tagp = ctx->d_tag; pa.toff = (uint32_t)k;
+        pa.nmem = nullptr;
         constexpr int CB = (sizeof(T) == 8) ? 16 : 32;
         if (sizeof(T) == 8)
             hipLaunchKernelGGL((k_panel<double, 2>), dim3(pa.P, 1), dim3(256), 0, ctx->stream, *(PanelArgs<double> *)&pa);
@@ -1402,7 +1427,7 @@ extern "C" int wmx_panel_step(wm_ctx *ctx, int prec, int n, void *A, int *ipiv, 
             hipLaunchKernelGGL((k_panel<float, kTprF32>), dim3(pa.P, 1), dim3(256), 0, ctx->stream, *(PanelArgs<float> *)&pa);
         if (do_swap) {
             SwapArgs sa;
-            sa.n = n; sa.k = k; sa.pmax = pmax; sa.do_swap = 1;
+            sa.n = n; sa.k = k; sa.pmax = pmax; sa.do_swap = 1; sa.nmem = nullptr;
             sa.ntl = (k + CB - 1) / CB;
             sa.ntr = (n - k - NB + CB - 1) / CB;
             sa.c_tr0 = k + NB; sa.c_tr1 = n; sa.c_sw0 = n;

@@ -13,8 +13,10 @@
 //   2. admits queued requests in arrival order while their footprint fits the
 //      HBM budget (a request that can never fit is refused at submit);
 //   3. hands a batch to every idle lane: pending frequencies of admitted
-//      requests that share the padded size of the oldest one with pending
-//      work, oldest request first (each request's frequencies highest first),
+//      requests whose padded size is that of the oldest one with pending work
+//      or down to half of it (uneven sizes share a batch, the smaller matrices
+//      padded with the identity), oldest request first (each request's
+//      frequencies highest first),
 //      as many as the lane's share of the budget and the in-flight cap allow;
 //      in mixed precision, frequencies that the refinement rates already seen
 //      on their structure predict not to refine are factored in FP64 directly;
@@ -229,13 +231,17 @@ void admit(wm_server *s)
 // Forms a batch for an idle lane; false if no frequency is pending.
 bool dispatch(wm_server *s, Lane *L)
 {
+    // the batch's padded size is the oldest pending request's; requests down
+    // to half that size join it (their matrices padded with the identity, the
+    // factorization skipping the padding), larger ones wait for another batch
     int ld = -1;
     for (auto &r : s->running)
         if (r->next < (int)r->freqs.size()) { ld = r->g->ld; break; }
     if (ld < 0) return false;
+    auto joins = [&](const Req *r) { return r->g->ld <= ld && 2 * r->g->ld > ld; };
     int nr = 1;
     for (auto &r : s->running)
-        if (r->g->ld == ld && r->next < (int)r->freqs.size()) nr = std::max(nr, r->nrhs);
+        if (joins(r.get()) && r->next < (int)r->freqs.size()) nr = std::max(nr, r->nrhs);
     const size_t per = item_bytes(ld, nr, s->o.prec);
     int cap = (int)std::max<size_t>(1, s->budget / s->lanes.size() / per);
     cap = std::min(cap, kLaneBatch);
@@ -243,7 +249,7 @@ bool dispatch(wm_server *s, Lane *L)
     std::vector<WorkItem> items;
     std::vector<Req *> owner;
     for (auto &r : s->running) {
-        if (r->g->ld != ld) continue;
+        if (!joins(r.get())) continue;
         while (r->next < (int)r->freqs.size() && (int)items.size() < cap) {
             const int f = r->order[r->next++];
             WorkItem w;

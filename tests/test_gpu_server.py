@@ -109,3 +109,33 @@ def test_serving_replay_c5_scale(wm, geometries, ctx, geo_dir):
             worst = max(worst, e / bound)
             assert e <= bound, (r["n"], float(f), float(e), k["cond1"], s_)
     print(f"worst error / bound = {worst:.3f}")
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_server_uneven_sizes_share_a_batch(wm, geometries, ctx, geo_dir, prec):
+    """Requests of padded sizes 2048 and 1536 go into one batch (the smaller
+    matrices padded with the identity; the factorization skips the padding).
+    FP64: each result equals the same request solved alone, bit for bit (the
+    skipped work is exactly the identity's).  FP32 + refinement: within 1e-10 of
+    the FP64 sweep of its structure."""
+    gs = []
+    for i, n in enumerate((2000, 1500)):
+        p = os.path.join(geo_dir, f"uneven_{prec}_{i}.geo")
+        geometries.planar_grid(p, n, seed=50 + i)
+        gs.append(wm.Geometry.load(ctx, p))
+    assert [g.ld for g in gs] == [2048, 1536]
+    freqs = [np.array([30.0, 12.0, 4.0], np.float32), np.array([25.0, 8.0], np.float32)]
+    feeds = [[3, 400], [9]]
+    srv = wm.Server(ctx, prec, size_policy=False, streams=2)
+    ids = [srv.submit(g, f, fd) for g, f, fd in zip(gs, freqs, feeds)]
+    res = _drain(srv, ids)
+    batches, _, _ = srv.overlap()
+    srv.close()
+    assert batches == 1
+    for g, f, fd, (zin, I, st, lat) in zip(gs, freqs, feeds, res):
+        z64, I64, _ = wm.solve_sweep(g, f, fd, "f64")
+        if prec == "f64":
+            assert np.array_equal(zin, z64) and np.array_equal(I, I64)
+        else:
+            rel = np.abs(zin - z64) / np.abs(z64)
+            assert (rel <= 1e-10).all(), rel

@@ -33,16 +33,61 @@ HIPFLAGS = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "
             "-munsafe-fp-atomics"]
 
 
+# Sources whose kernels must live in VGPRs alone: the fill kernels hold no MFMA
+# accumulators, so AGPRs in use there are spill space, and neither AGPR spill
+# space, VGPR spills nor scratch is accepted for them.  (SGPR spills go to VGPR
+# lanes, not memory, and are allowed.)
+SPILL_FREE = {"wm_fill.hip": ("k_fill", "k_self_terms")}
+_REMARK = "kernel-resource-usage"
+
+
+def resource_usage(stderr: str) -> dict:
+    """{mangled kernel name: {field: int}} from -Rpass-analysis=kernel-resource-usage remarks."""
+    out, cur = {}, None
+    for line in stderr.splitlines():
+        if _REMARK not in line or "remark:" not in line:
+            continue
+        body = line.split("remark:", 1)[1].split("[-Rpass", 1)[0].strip()
+        key, _, val = body.partition(":")
+        key, val = key.strip(), val.strip()
+        if key == "Function Name":
+            cur = out.setdefault(val, {})
+        elif cur is not None:
+            try:
+                cur[key] = int(val.split()[0])
+            except (ValueError, IndexError):
+                pass
+    return out
+
+
+def check_spill_free(src: str, stderr: str) -> None:
+    names = SPILL_FREE.get(src)
+    if not names:
+        return
+    usage = {k: v for k, v in resource_usage(stderr).items() if any(n in k for n in names)}
+    if not usage:
+        raise RuntimeError(f"{src}: no resource-usage remarks for {names}")
+    bad = [(k, v) for k, v in usage.items()
+           if v.get("AGPRs", 0) or v.get("ScratchSize [bytes/lane]", 0) or v.get("VGPRs Spill", 0)]
+    if bad:
+        raise RuntimeError(f"{src}: kernels spill (AGPRs, scratch and VGPR spills must be 0):\n"
+                           + "\n".join(f"  {k}: {v}" for k, v in bad))
+
+
 def _compile(src: str) -> str:
     obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
     cmd = [HIPCC, "-c"]
     if src.endswith(".hip"):
         cmd += ["-x", "hip"] + HIPFLAGS
+    if src in SPILL_FREE:
+        cmd += [f"-Rpass-analysis={_REMARK}", "-fno-caret-diagnostics"]
     cmd += [os.path.join(CSRC, src), "-o", obj] + COMMON
     r = subprocess.run(cmd, capture_output=True, text=True)
-    err = "\n".join(l for l in r.stderr.splitlines() if "packed-fp32-ops" not in l and l.strip())
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {src}\n{' '.join(cmd)}\n{r.stderr}")
+    check_spill_free(src, r.stderr)
+    err = "\n".join(l for l in r.stderr.splitlines()
+                    if "packed-fp32-ops" not in l and _REMARK not in l and l.strip())
     if err:
         sys.stderr.write(err + "\n")
     return obj

@@ -34,12 +34,7 @@ namespace wm {
 
 namespace {
 
-constexpr int kMaxF = 4;          // frequencies per launch
-// Frequencies per launch with the PEC image terms.  At four the image kernel
-// needs 256 VGPRs plus 98 AGPRs of spill space and gave wrong values in one
-// frequency slot on gfx950 (5.7e-4 relative, the others exact); at two it
-// fits in 213 VGPRs with no spill and matches the oracle.
-constexpr int kMaxFImg = 2;
+constexpr int kMaxF = 4;          // frequencies per launch (with or without the image terms)
 constexpr int kColsPerWave = 4;   // source bases per wave
 constexpr int kWaves = 4;         // waves per block
 constexpr double kPi = 3.14159265358979323846;
@@ -177,7 +172,11 @@ __global__ __launch_bounds__(256) void k_fill(const double *__restrict__ obs, co
             for (int f = 0; f < F; ++f) { cv[f] = make_double2(0.0, 0.0); cs[f] = make_double2(0.0, 0.0); }
 #pragma unroll
             for (int g = 0; g < 2; ++g) {
-#pragma unroll
+                // the image pass is a loop, not unrolled: unrolled, four
+                // frequencies need more than 256 VGPRs and the compiler parks
+                // values in AGPRs; as a loop the image kernel needs 189 VGPRs
+                // (build.py rejects a fill kernel that uses AGPRs or scratch)
+#pragma unroll 1
                 for (int pass = 0; pass < (IMG ? 2 : 1); ++pass) {
                     const double *sr = (pass ? im.rec : src) + (size_t)n * 16;
                     const int skey = pass ? im.key[2 * n + g] : src_key[2 * n + g];
@@ -259,13 +258,8 @@ template <int F>
 void launch_fill_f(dim3 grid, hipStream_t s, const wm_geom *g, const double2 *self, const FreqArgs &fa,
                    double2 *Z, int64_t sZ, float2 *Z32, int64_t sZ32)
 {
-    if constexpr (F <= kMaxFImg) {
-        if (g->ienv == 2) {
-            launch_fill_fi<F, true>(grid, s, g, self, fa, Z, sZ, Z32, sZ32);
-            return;
-        }
-    }
-    launch_fill_fi<F, false>(grid, s, g, self, fa, Z, sZ, Z32, sZ32);
+    if (g->ienv == 2) launch_fill_fi<F, true>(grid, s, g, self, fa, Z, sZ, Z32, sZ32);
+    else launch_fill_fi<F, false>(grid, s, g, self, fa, Z, sZ, Z32, sZ32);
 }
 
 }  // namespace
@@ -288,9 +282,8 @@ int fill_launch(wm_ctx *ctx, const wm_geom *g, int nf, const double *ka, void *Z
     WM_TRY(ws.ensure(self_bytes));
     double2 *self = (double2 *)ws.ptr;
     const dim3 grid(ld / 64, ld / (kWaves * kColsPerWave));
-    const int fmax = g->ienv == 2 ? kMaxFImg : kMaxF;
-    for (int f0 = 0; f0 < nf; f0 += fmax) {
-        const int F = (nf - f0 < fmax) ? nf - f0 : fmax;
+    for (int f0 = 0; f0 < nf; f0 += kMaxF) {
+        const int F = (nf - f0 < kMaxF) ? nf - f0 : kMaxF;
         FreqArgs fa;
         for (int f = 0; f < kMaxF; ++f) {
             const double k = ka[f0 + (f < F ? f : 0)];

@@ -1293,8 +1293,15 @@ int getrf_run(wm_ctx *ctx, int prec, int n, void *A, int lda, int64_t strideA, i
 
 }  // namespace
 
+// Destroys the captured factorizations.  A replay may still be running on the
+// stream it was launched on (nothing waits for it before the exchange buffer
+// grows or the cache fills), and HIP does not promise that an executable graph
+// outlives its destruction while in flight, so the device is drained first.
 void drop_graphs(wm_ctx *ctx)
 {
+    if (ctx->graphs.empty()) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipDeviceSynchronize();
     for (auto &kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
     ctx->graphs.clear();
 }

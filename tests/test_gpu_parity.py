@@ -73,7 +73,7 @@ def test_pec_fill_matches_oracle(wm, geometries, oracle, ctx, geo_dir, case):
     """PEC ground (ienv 2): fill with the image terms, the self rule on in-plane
     segments and the virtual ground wires, against the oracle.  Five
     frequencies in one request: every frequency slot of the image kernel's
-    launches (two frequencies per launch) is compared."""
+    launches (four frequencies per launch, then one) is compared."""
     p = _pec_case(geometries, geo_dir, case)
     t = oracle.load(p, ienv=2)
     g = wm.Geometry.load(ctx, p, ienv=2)
@@ -85,6 +85,22 @@ def test_pec_fill_matches_oracle(wm, geometries, oracle, ctx, geo_dir, case):
         err = np.abs(Zg[i] - Zo).max() / np.abs(Zo).max()
         print(case, f, err)
         assert err <= 1e-12, (case, f, err)
+
+
+@pytest.mark.parametrize("ienv", [1, 2])
+def test_fill_slots_bit_identical(wm, geometries, ctx, geo_dir, ienv):
+    """An entry of Z does not depend on which slot of a launch its frequency
+    takes or how many frequencies share the launch: seven frequencies filled
+    together (launches of four and three) equal each frequency filled alone,
+    bit for bit, with and without the PEC image terms."""
+    p = _pec_case(geometries, geo_dir, "rods")
+    g = wm.Geometry.load(ctx, p, ienv=ienv)
+    freqs = [40.0, 25.0, 10.0, 2.0, 0.5, 7.5, 0.02]
+    Zall = g.fill(freqs)
+    for i, f in enumerate(freqs):
+        Z1 = g.fill([f])[0]
+        a, b = np.ascontiguousarray(Zall[i]), np.ascontiguousarray(Z1)
+        assert np.array_equal(a.view(np.uint64), b.view(np.uint64)), (ienv, i, f)
 
 
 def test_pec_zin_matches_oracle_f64(wm, geometries, oracle, ctx, geo_dir):

@@ -90,3 +90,34 @@ def test_out_of_iterations_fails(decide):
 def test_zero_solution_converges(decide):
     it = decide()
     assert it.step(0, [0.0], [0.0]) == 1
+
+
+def test_build_rejects_spilling_fill_kernels():
+    """build.py's guard on the fill kernels: resource-usage remarks with AGPRs,
+    scratch or VGPR spills in a k_fill / k_self_terms instantiation fail the
+    build; SGPR spills (to VGPR lanes) and other kernels do not."""
+    import importlib.util
+    import os
+    here = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "praca-magisterska_amd")
+    spec = importlib.util.spec_from_file_location("wm_build", os.path.join(here, "build.py"))
+    b = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(b)
+
+    def remarks(name, **f):
+        r = [f"x.hip:1:1: remark: Function Name: {name} [-Rpass-analysis=kernel-resource-usage]"]
+        r += [f"x.hip:1:1: remark:     {k}: {v} [-Rpass-analysis=kernel-resource-usage]" for k, v in f.items()]
+        return "\n".join(r)
+
+    ok = remarks("_Z6k_fillILi4E", VGPRs=189, AGPRs=0, **{"ScratchSize [bytes/lane]": 0, "VGPRs Spill": 0,
+                                                        "SGPRs Spill": 52})
+    other = remarks("_Z9k_gemm_b3", VGPRs=128, AGPRs=64)
+    b.check_spill_free("wm_fill.hip", ok + "\n" + other)
+    assert b.resource_usage(ok)["_Z6k_fillILi4E"]["VGPRs"] == 189
+    for bad in ({"AGPRs": 106}, {"ScratchSize [bytes/lane]": 16}, {"VGPRs Spill": 3}):
+        f = {"VGPRs": 256, "AGPRs": 0, "ScratchSize [bytes/lane]": 0, "VGPRs Spill": 0}
+        f.update(bad)
+        with pytest.raises(RuntimeError):
+            b.check_spill_free("wm_fill.hip", ok + "\n" + remarks("_Z12k_self_terms", **f))
+    with pytest.raises(RuntimeError):
+        b.check_spill_free("wm_fill.hip", other)   # no fill remarks at all
+    b.check_spill_free("wm_lu.hip", other)          # not a checked source

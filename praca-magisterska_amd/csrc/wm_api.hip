@@ -276,6 +276,9 @@ int wm_shutdown(wm_ctx *c)
     c->ws_perm.release();
     c->ws_resid.release();
     c->ws_sweep.release();
+    c->ws_group.release();
+    c->ws_dinv.release();
+    c->ws_dinv_idx.release();
     (void)hipFree(c->d_sync);
     (void)hipFree(c->d_err);
     (void)hipFree(c->d_tag);

@@ -247,6 +247,22 @@ int gmres_ir(wm_ctx *c, const wm_solve_opts &o, const GmresArgs &a, const std::v
     std::vector<unsigned long long> hm((size_t)2 * cnt * nr);
     std::vector<double2> hred;
 
+    // the inverted diagonal blocks of every factor in use: the preconditioner
+    // applies them as products (its solves need not be backward stable; the
+    // FP64 residuals decide the result)
+    std::vector<int> facs, slot_of;
+    for (int f : fac) {
+        if (f < 0) continue;
+        if ((size_t)f >= slot_of.size()) slot_of.resize((size_t)f + 1, -1);
+        if (slot_of[f] < 0) {
+            slot_of[f] = (int)facs.size();
+            facs.push_back(f);
+        }
+    }
+    WM_TRY(c->ws_dinv.ensure(diag_inv_bytes(n, std::max<int>(1, (int)facs.size()))));
+    double2 *dinv = (double2 *)c->ws_dinv.ptr;
+    WM_TRY(diag_inv_batched(c, a.fprec, n, a.F, n, a.sF, facs, dinv, s));
+
     auto upload_idx = [&](const std::vector<int> &lst) -> int {
         std::vector<int> f(lst.size());
         for (size_t q = 0; q < lst.size(); ++q) f[q] = facOf[lst[q]];
@@ -314,7 +330,9 @@ int gmres_ir(wm_ctx *c, const wm_solve_opts &o, const GmresArgs &a, const std::v
             double2 *vk = Vb + (size_t)k * setS, *zk = Zb + (size_t)k * setS, *w = Vb + (size_t)(k + 1) * setS;
             // z_k = M^-1 v_k (the anchor's factors), w = Z z_k
             hipLaunchKernelGGL(k_copy_set, dim3((unsigned)((sV + 255) / 256), nl), blk, 0, s, zk, vk, sV, a.dact);
-            WM_TRY(getrs_perm_batched(c, a.fprec, n, nr, a.F, n, a.sF, a.perm, n, zk, n, sV, nl, s, a.dact, a.dfac));
+            std::vector<int> fl(lst.size());
+            for (size_t q = 0; q < lst.size(); ++q) fl[q] = facOf[lst[q]];
+            WM_TRY(getrs_grouped(c, a.fprec, n, nr, a.F, n, a.sF, a.perm, n, zk, sV, lst, fl, s, dinv, &slot_of));
             WM_TRY(residual_batched(c, n, nr, a.Z, n, a.sZ, zk, nullptr, n, sV, w, nl, s, a.dact));
             // classical Gram-Schmidt, twice; the coefficients stay on the device
             double2 *h1 = a.red, *h2 = a.red + (size_t)nl * (m + 1) * nr, *nw = h2 + (size_t)nl * (m + 1) * nr;

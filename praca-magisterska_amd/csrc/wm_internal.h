@@ -87,6 +87,9 @@ struct wm_ctx {
     wm::DevBuf ws_perm;     // row permutations for getrs on raw pivots
     wm::DevBuf ws_resid;    // residual partial sums
     wm::DevBuf ws_sweep;    // matrices and vectors of a sweep batch
+    wm::DevBuf ws_group;    // grouped triangular solves: factor and column tables, staging
+    wm::DevBuf ws_dinv;     // inverted diagonal blocks of preconditioner factors
+    wm::DevBuf ws_dinv_idx; // their factor indices
     int solve_epoch = 0;    // monotonic tag of triangular-solve packets
     uint64_t panel_tag = 0; // packet tags used since the exchange buffer was last cleared
     uint32_t *d_tag = nullptr;  // device tag base, bumped by every factorization launch chunk
@@ -163,6 +166,29 @@ int perm_batched(wm_ctx *ctx, int n, const int *ipiv, int64_t strideP, int *perm
 int getrs_perm_batched(wm_ctx *ctx, int prec, int n, int nrhs, const void *A, int lda, int64_t strideA,
                        const int *perm, int64_t strideQ, void *B, int ldb, int64_t strideB, int batch,
                        hipStream_t s, const int *d_bidx = nullptr, const int *d_aidx = nullptr);
+// Solves with shared factors: column r < nr of item items[q] (at V + items[q] *
+// strideV + r * n) is solved in place with the factors and permutation of
+// index fac[q].  The columns of items that share a factor are the right-hand
+// sides of one launch member (up to max_w per member), so each factor is read
+// once per member rather than once per column; a column's arithmetic is the
+// same whatever it is grouped with.  A triangular solve's time per 64-row
+// block grows with its right-hand sides while its chain of blocks is
+// latency-bound, so members of a few columns each run faster than one wide
+// member (kGroupCols).
+// With dinv (the inverted diagonal blocks of diag_inv_batched, factor f in slot
+// (*slot_of)[f]) each diagonal block is applied as a product instead of a
+// substitution: not backward stable like the substitution, for preconditioner
+// applications only.
+constexpr int kGroupCols = 4;
+int getrs_grouped(wm_ctx *ctx, int prec, int n, int nr, const void *F, int lda, int64_t strideF, const int *perm,
+                  int64_t strideQ, double2 *V, int64_t strideV, const std::vector<int> &items,
+                  const std::vector<int> &fac, hipStream_t s, const double2 *dinv = nullptr,
+                  const std::vector<int> *slot_of = nullptr, int max_w = kGroupCols);
+// Inverses of the 64 x 64 diagonal blocks of the factors of index factors[z]
+// into slot z of Dinv (diag_inv_bytes(n, factors.size()) bytes), FP64.
+size_t diag_inv_bytes(int n, int nfac);
+int diag_inv_batched(wm_ctx *ctx, int prec, int n, const void *F, int lda, int64_t strideF,
+                     const std::vector<int> &factors, double2 *Dinv, hipStream_t s);
 // Solves A^H X = B with the same factors and permutation (A = P^T L U, so
 // X = P^T L^-H U^-H B); used by the condition estimate.
 int getrsh_perm_batched(wm_ctx *ctx, int prec, int n, int nrhs, const void *A, int lda, int64_t strideA,

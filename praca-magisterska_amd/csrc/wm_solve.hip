@@ -12,7 +12,10 @@
 // blocks owned by workgroups with a smaller id in the same grid, which are
 // dispatched before it, so the chain always makes progress; every wait is
 // bounded.  Arithmetic is FP64; complex64 factors are widened on load.
+#include <algorithm>
+#include <cstring>
 #include <mutex>
+#include <vector>
 
 #include "wm_cplx.h"
 #include "wm_internal.h"
@@ -30,10 +33,24 @@ __device__ inline double2 ld2(const typename CT<T>::V *p)
     return make_double2((double)v.x, (double)v.y);
 }
 
-__device__ inline void cfms(double2 &acc, double2 a, double2 b)  // acc -= a*b
+// acc -= a*b with every rounding written out (fused multiply-adds), so a
+// value does not depend on how the compiler contracts the instantiation it
+// is inlined into (the right-hand-side count of a triangular solve).
+__device__ inline void cfms(double2 &acc, double2 a, double2 b)
 {
-    acc.x -= a.x * b.x - a.y * b.y;
-    acc.y -= a.x * b.y + a.y * b.x;
+    acc.x = fma(-a.x, b.x, acc.x);
+    acc.x = fma(a.y, b.y, acc.x);
+    acc.y = fma(-a.x, b.y, acc.y);
+    acc.y = fma(-a.y, b.x, acc.y);
+}
+
+// acc += a*b, roundings written out as in cfms.
+__device__ inline void cfma(double2 &acc, double2 a, double2 b)
+{
+    acc.x = fma(a.x, b.x, acc.x);
+    acc.x = fma(-a.y, b.y, acc.x);
+    acc.y = fma(a.x, b.y, acc.y);
+    acc.y = fma(a.y, b.x, acc.y);
 }
 
 // perm[t] = source row of position t after applying ipiv in order.
@@ -71,7 +88,12 @@ struct TrsvArgs {
     int *err;
     const int *bidx;      // launch member -> matrix index of B and Y (null: identity)
     const int *aidx;      // launch member -> matrix index of the factors and permutation (null: as bidx)
+    const double2 *dinv;  // inverted diagonal blocks (k_diag_inv), used instead of substitution, or null
+    const int *dslot;     // launch member -> slot of its factors' blocks in dinv
 };
+
+// Packed entries of one 64 x 64 triangle with its diagonal.
+constexpr int kPack = TB * (TB + 1) / 2;
 
 __device__ inline double bcast(double v, int lane)  // value of `lane` to the whole wave
 {
@@ -88,9 +110,11 @@ __device__ inline double word_pair(uint32_t w, int l)
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
-// Packed index of element (i, j) of the 64 x 64 diagonal block: strictly
-// lower part for L (i > j), upper part with the diagonal for U (i <= j).
-__device__ inline int dl_idx(int i, int j) { return i * (i - 1) / 2 + j; }
+// Packed index of element (i, j) of the 64 x 64 diagonal block, column by
+// column (so the lanes of a wave, one row each, read consecutive entries of
+// a column: no LDS bank conflicts): strictly lower part for L (i > j), upper
+// part with the diagonal for U (i <= j).
+__device__ inline int dl_idx(int i, int j) { return j * (2 * TB - 1 - j) / 2 + (i - j - 1); }
 __device__ inline int du_idx(int i, int j) { return j * (j + 1) / 2 + i; }
 
 // Triangular solve, one launch.  Workgroup (member, g) owns row block bi of
@@ -106,9 +130,12 @@ __device__ inline int du_idx(int i, int j) { return j * (j + 1) / 2 + i; }
 // right-hand sides as staged (no gather), the upper pass L^H (upper, unit
 // diagonal) in place; element (i, j) of the triangle is conj(A(j, i)).  The
 // caller scatters the result through the permutation (x = P^T w).
-template <typename T, int NR, bool UPPER, bool TRANS>
+template <typename T, int NR, bool UPPER, bool TRANS, bool INV = false>
 __global__ __launch_bounds__(256) void k_trsv(TrsvArgs a, int r_off)
 {
+    // no contraction: a right-hand side's arithmetic is the same in every
+    // instantiation (NR), so grouping columns never changes their values
+#pragma clang fp contract(off)
     using V = typename CT<T>::V;
     constexpr bool UNIT = TRANS ? UPPER : !UPPER;  // unit diagonal (L or L^H)
     // element (i, j) of the triangle being solved
@@ -130,9 +157,13 @@ __global__ __launch_bounds__(256) void k_trsv(TrsvArgs a, int r_off)
     const int nr = a.nrhs - r_off;
 
     // diagonal block to LDS (independent of other blocks, so before any wait)
-    __shared__ double2 s_d[TB * (TB + 1) / 2];
+    __shared__ double2 s_d[kPack];
     __shared__ double2 s_rd[TB];
-    for (int e = tid; e < TB * TB; e += 256) {
+    if (INV) {
+        const double2 *D = a.dinv + (((size_t)a.dslot[bl] * 2 + (UPPER ? 1 : 0)) * nblk + bi) * kPack;
+        for (int e = tid; e < kPack; e += 256) s_d[e] = D[e];
+    }
+    for (int e = tid; !INV && e < TB * TB; e += 256) {
         // consecutive threads walk down a column of the stored factor
         const int i = TRANS ? e / TB : e % TB, j = TRANS ? e % TB : e / TB;
         if (!UPPER && i > j) s_d[dl_idx(i, j)] = elem(A, r0 + i, r0 + j);
@@ -203,32 +234,65 @@ __global__ __launch_bounds__(256) void k_trsv(TrsvArgs a, int r_off)
         for (int c = 0; c < 16; ++c) lcur[c] = lnext[c];
     }
 
-    __shared__ double2 s_acc[3][TB][NR];
-    if (part > 0)
+    // The four parts' sums meet in LDS; wave p then solves the right-hand
+    // sides p, p + 4, ... (each sum in the order part 0 + 1 + 2 + 3, so a
+    // value does not depend on NR or on which wave solves it).
+    __shared__ double2 s_acc[4][TB][NR];
 #pragma unroll
-        for (int r = 0; r < NR; ++r) s_acc[part - 1][row][r] = acc[r];
+    for (int r = 0; r < NR; ++r) s_acc[part][row][r] = acc[r];
     __syncthreads();
-    if (part == 0) {
+    constexpr int RW = (NR + 3) / 4;  // right-hand sides per wave
+    if (part < NR) {
+        double2 x[RW];
 #pragma unroll
-        for (int r = 0; r < NR; ++r)
+        for (int k = 0; k < RW; ++k) {
+            const int r = part + 4 * k;
+            x[k] = make_double2(0.0, 0.0);
+            if (r < NR) {
+                x[k] = s_acc[0][row][r];
 #pragma unroll
-            for (int p = 0; p < 3; ++p) { acc[r].x += s_acc[p][row][r].x; acc[r].y += s_acc[p][row][r].y; }
+                for (int p = 1; p < 4; ++p) { x[k].x += s_acc[p][row][r].x; x[k].y += s_acc[p][row][r].y; }
+            }
+        }
+        if (INV) {
+            // x = D^-1 a with the inverted diagonal block: a product instead of
+            // a 64-step substitution (the wave's sums go through its own LDS
+            // entries; the block of L^-1 has a unit diagonal)
+#pragma unroll
+            for (int k = 0; k < RW; ++k)
+                if (part + 4 * k < NR) s_acc[0][row][part + 4 * k] = x[k];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+            for (int k = 0; k < RW; ++k) {
+                const int r = part + 4 * k;
+                if (r >= NR) break;
+                double2 v = UPPER ? make_double2(0.0, 0.0) : x[k];
+#pragma unroll
+                for (int j = 0; j < TB; ++j) {
+                    if (UPPER ? j >= row : j < row)
+                        cfma(v, s_d[UPPER ? du_idx(row, j) : dl_idx(row, j)], s_acc[0][j][r]);
+                }
+                x[k] = v;
+            }
+        } else if (!UPPER) {
         // the substitution is the launch's critical path: unrolled, so each
         // step's LDS read of the block is issued ahead of the step
-        if (!UPPER) {
 #pragma unroll
             for (int j = 0; j < TB; ++j) {
                 const double2 l = row > j ? s_d[dl_idx(row, j)] : make_double2(0.0, 0.0);
                 const double2 ri = UNIT ? make_double2(1.0, 0.0) : s_rd[j];
 #pragma unroll
-                for (int r = 0; r < NR; ++r) {
+                for (int k = 0; k < RW; ++k) {
+                    if (part + 4 * k >= NR) break;
                     if (!UNIT && row == j) {
-                        const double2 x = acc[r];
-                        acc[r] = make_double2(x.x * ri.x - x.y * ri.y, x.x * ri.y + x.y * ri.x);
+                        const double2 v = x[k];
+                        x[k] = make_double2(v.x * ri.x - v.y * ri.y, v.x * ri.y + v.y * ri.x);
                     }
                     if (j < TB - 1) {
-                        const double2 yj = make_double2(bcast(acc[r].x, j), bcast(acc[r].y, j));
-                        if (row > j) cfms(acc[r], l, yj);
+                        const double2 yj = make_double2(bcast(x[k].x, j), bcast(x[k].y, j));
+                        if (row > j) cfms(x[k], l, yj);
                     }
                 }
             }
@@ -238,57 +302,72 @@ __global__ __launch_bounds__(256) void k_trsv(TrsvArgs a, int r_off)
                 const double2 u = row < j ? s_d[du_idx(row, j)] : make_double2(0.0, 0.0);
                 const double2 ri = UNIT ? make_double2(1.0, 0.0) : s_rd[j];
 #pragma unroll
-                for (int r = 0; r < NR; ++r) {
+                for (int k = 0; k < RW; ++k) {
+                    if (part + 4 * k >= NR) break;
                     if (!UNIT && row == j) {
-                        const double2 x = acc[r];
-                        acc[r] = make_double2(x.x * ri.x - x.y * ri.y, x.x * ri.y + x.y * ri.x);
+                        const double2 v = x[k];
+                        x[k] = make_double2(v.x * ri.x - v.y * ri.y, v.x * ri.y + v.y * ri.x);
                     }
-                    const double2 xj = make_double2(bcast(acc[r].x, j), bcast(acc[r].y, j));
-                    if (row < j) cfms(acc[r], u, xj);
+                    const double2 xj = make_double2(bcast(x[k].x, j), bcast(x[k].y, j));
+                    if (row < j) cfms(x[k], u, xj);
                 }
             }
         }
 #pragma unroll
-        for (int r = 0; r < NR; ++r)
+        for (int k = 0; k < RW; ++k) {
+            const int r = part + 4 * k;
             if (r < nr) {
-                Y[(r0 + row) + (size_t)r * a.ldb] = acc[r];
+                Y[(r0 + row) + (size_t)r * a.ldb] = x[k];
                 uint64_t *pw = pk + ((size_t)bi * NR + r) * 256 + row * 4;
-                const uint64_t xb = (uint64_t)__double_as_longlong(acc[r].x), yb = (uint64_t)__double_as_longlong(acc[r].y);
+                const uint64_t xb = (uint64_t)__double_as_longlong(x[k].x), yb = (uint64_t)__double_as_longlong(x[k].y);
                 put_pkt(pw, tag, (uint32_t)xb);
                 put_pkt(pw + 1, tag, (uint32_t)(xb >> 32));
                 put_pkt(pw + 2, tag, (uint32_t)yb);
                 put_pkt(pw + 3, tag, (uint32_t)(yb >> 32));
             }
+        }
     }
 }
 
 // Partial products P[split][row][r] = sum_{j in split} Z[row][j] X[j][r].
+// Lane = row (each column of Z is one coalesced 1 KiB read per wave); the
+// wave's column range is wave-uniform, so X arrives through scalar loads.  Z
+// streams through in batches of kZB columns (all loads of a batch issued
+// before its products, non-temporal: Z is read once per product), and each
+// column's term is added in column order with fused multiply-adds, so a
+// row's sum depends on n only.
+typedef double d2v __attribute__((ext_vector_type(2)));
+
 template <int NR>
 __global__ __launch_bounds__(256) void k_resid_part(const double2 *Z, int ldz, int64_t strideZ, int n, int nrhs,
                                                     const double2 *X, int ldb, int64_t strideB, int cols_per_split,
                                                     int nsplit, double2 *P, int r_off, const int *bidx)
 {
-    const int bl = blockIdx.z, split = blockIdx.y, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    constexpr int kZB = 8;
+    const int bl = blockIdx.z, split = blockIdx.y, lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int b = bidx ? bidx[bl] : bl;
     const int row = blockIdx.x * 64 + lane;
     const double2 *Zb = Z + (size_t)b * strideZ;
     const double2 *Xb = X + (size_t)b * strideB + (size_t)r_off * ldb;
     const int cw = cols_per_split / 4;
     const int c0 = split * cols_per_split + wave * cw;
+    // NR <= nrhs - r_off: the launcher runs right-hand sides in chunks of 8, 4, 2, 1
     double2 acc[NR];
 #pragma unroll
     for (int r = 0; r < NR; ++r) acc[r] = make_double2(0.0, 0.0);
-#pragma unroll 4
-    for (int c = 0; c < cw; ++c) {
-        const int col = c0 + c;
-        const double2 z = Zb[row + (size_t)col * ldz];
+    const double2 *zp = Zb + row + (size_t)c0 * ldz;
+    for (int c = 0; c < cw; c += kZB) {  // cw is a multiple of kZB
+        double2 z[kZB];
 #pragma unroll
-        for (int r = 0; r < NR; ++r) {
-            if (r < nrhs - r_off) {
-                const double2 x = Xb[col + (size_t)r * ldb];
-                acc[r].x += z.x * x.x - z.y * x.y;
-                acc[r].y += z.x * x.y + z.y * x.x;
-            }
+        for (int e = 0; e < kZB; ++e) {
+            const d2v v = __builtin_nontemporal_load((const d2v *)(zp + (size_t)(c + e) * ldz));
+            z[e] = make_double2(v.x, v.y);
+        }
+#pragma unroll
+        for (int e = 0; e < kZB; ++e) {
+#pragma unroll
+            for (int r = 0; r < NR; ++r) cfma(acc[r], z[e], Xb[c0 + c + e + (size_t)r * ldb]);
         }
     }
     __shared__ double2 s[3][64][NR];
@@ -333,8 +412,61 @@ template <typename T, int NR, bool TRANS>
 void launch_trsv(const TrsvArgs &a, int batch, int r_off, bool upper, hipStream_t s)
 {
     const dim3 grid(batch, a.n / TB);
+    if (a.dinv && !TRANS) {
+        if (upper) hipLaunchKernelGGL((k_trsv<T, NR, true, false, true>), grid, dim3(256), 0, s, a, r_off);
+        else hipLaunchKernelGGL((k_trsv<T, NR, false, false, true>), grid, dim3(256), 0, s, a, r_off);
+        return;
+    }
     if (upper) hipLaunchKernelGGL((k_trsv<T, NR, true, TRANS>), grid, dim3(256), 0, s, a, r_off);
     else hipLaunchKernelGGL((k_trsv<T, NR, false, TRANS>), grid, dim3(256), 0, s, a, r_off);
+}
+
+// Inverses of the 64 x 64 diagonal blocks of LU factors, FP64 arithmetic:
+// blockIdx = (block, triangle 0 = L (unit lower) / 1 = U, slot); slot z
+// takes the factors of index fidx[z].  Lane j solves for column j of the
+// inverse by substitution against the block staged in LDS; the inverse is
+// stored packed as the triangular solve stages it (strictly lower part of
+// L^-1, upper part of U^-1 with its diagonal), kPack entries per block.
+template <typename T>
+__global__ __launch_bounds__(64) void k_diag_inv(const void *F, int lda, int64_t strideF, const int *fidx, int nblk,
+                                                 double2 *Dinv)
+{
+#pragma clang fp contract(off)
+    using V = typename CT<T>::V;
+    const int bi = blockIdx.x, upper = blockIdx.y, j = threadIdx.x, r0 = bi * TB;
+    const V *A = (const V *)F + (size_t)fidx[blockIdx.z] * strideF;
+    __shared__ double2 s_t[TB][TB + 1];  // the block, s_t[i][k] = A(r0 + i, r0 + k)
+    __shared__ double2 s_x[TB][TB + 1];  // the inverse, column j by lane j
+    for (int e = j; e < TB * TB; e += TB) {
+        const int i = e % TB, k = e / TB;
+        s_t[i][k] = ld2<T>(A + (r0 + i) + (size_t)(r0 + k) * lda);
+    }
+    __syncthreads();
+    if (!upper) {
+        for (int i = 0; i < TB; ++i) {
+            double2 v = make_double2(i == j ? 1.0 : 0.0, 0.0);
+            if (i > j)
+                for (int k = j; k < i; ++k) cfms(v, s_t[i][k], s_x[k][j]);
+            s_x[i][j] = v;
+        }
+    } else {
+        for (int i = TB - 1; i >= 0; --i) {
+            double2 v = make_double2(i == j ? 1.0 : 0.0, 0.0);
+            if (i <= j) {
+                for (int k = i + 1; k <= j; ++k) cfms(v, s_t[i][k], s_x[k][j]);
+                const double2 d = s_t[i][i], rd = crecip(d.x, d.y);
+                v = make_double2(v.x * rd.x - v.y * rd.y, v.x * rd.y + v.y * rd.x);
+            }
+            s_x[i][j] = v;
+        }
+    }
+    __syncthreads();
+    double2 *D = Dinv + (((size_t)blockIdx.z * 2 + upper) * nblk + bi) * kPack;
+    for (int e = j; e < TB * TB; e += TB) {
+        const int i = e % TB, k = e / TB;
+        if (!upper && i > k) D[dl_idx(i, k)] = s_x[i][k];
+        if (upper && i <= k) D[du_idx(i, k)] = s_x[i][k];
+    }
 }
 
 // Right-hand sides handled by one pass starting at r_off: 8, 4, 2 or 1.
@@ -421,7 +553,8 @@ size_t solve_pkts(int n, int nrhs, int batch) { return 2 * (size_t)batch * (n / 
 // (nothing allocated, no profiler events).
 int getrs_any(wm_ctx *ctx, int prec, int n, int nrhs, const void *A, int lda, int64_t strideA, const int *perm,
               int64_t strideQ, void *B, int ldb, int64_t strideB, int batch, hipStream_t s, const int *d_bidx,
-              bool trans, const int *d_aidx, SolveBufs *ext = nullptr)
+              bool trans, const int *d_aidx, SolveBufs *ext = nullptr, const double2 *dinv = nullptr,
+              const int *d_dslot = nullptr)
 {
     if (n <= 0 || n % TB != 0 || nrhs <= 0 || batch <= 0 || ldb < n) {
         set_error("getrs: bad arguments (n=%d multiple of %d, nrhs=%d, batch=%d)", n, TB, nrhs, batch);
@@ -467,6 +600,8 @@ int getrs_any(wm_ctx *ctx, int prec, int n, int nrhs, const void *A, int lda, in
         a.perm = perm; a.strideQ = strideQ; a.ldb = ldb; a.strideB = strideB; a.err = ctx->d_err;
         a.bidx = d_bidx;
         a.aidx = d_aidx;
+        a.dinv = trans ? nullptr : dinv;
+        a.dslot = d_dslot;
         a.Bin = Ystage; a.Y = (double2 *)B; a.pkt = pkt;
         a.epoch = ++*epoch;
         const int nr = rhs_chunk(nrhs - r0);
@@ -511,6 +646,125 @@ int getrsh_perm_batched(wm_ctx *ctx, int prec, int n, int nrhs, const void *A, i
 {
     return getrs_any(ctx, prec, n, nrhs, A, lda, strideA, perm, strideQ, B, ldb, strideB, batch, s, d_bidx, true,
                      nullptr);
+}
+
+namespace {
+
+// S[member * W + j] = column cols[member * W + j] of V (element offset; -1: zeros).
+__global__ void k_gather_cols(const double2 *V, int n, const int64_t *cols, double2 *S)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, mj = blockIdx.y;
+    if (i >= n) return;
+    const int64_t c = cols[mj];
+    S[(size_t)mj * n + i] = c >= 0 ? V[c + i] : make_double2(0.0, 0.0);
+}
+
+// The columns back: V[cols[member * W + j]] = S[member * W + j].
+__global__ void k_scatter_cols(const double2 *S, int n, const int64_t *cols, double2 *V)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, mj = blockIdx.y;
+    if (i >= n) return;
+    const int64_t c = cols[mj];
+    if (c >= 0) V[c + i] = S[(size_t)mj * n + i];
+}
+
+}  // namespace
+
+size_t diag_inv_bytes(int n, int nfac) { return (size_t)nfac * 2 * (n / TB) * kPack * sizeof(double2); }
+
+int diag_inv_batched(wm_ctx *ctx, int prec, int n, const void *F, int lda, int64_t strideF,
+                     const std::vector<int> &factors, double2 *Dinv, hipStream_t s)
+{
+    if (factors.empty()) return WM_OK;
+    if (n % TB != 0) {
+        set_error("diag_inv: n=%d not a multiple of %d", n, TB);
+        return WM_ERR_ARG;
+    }
+    WM_TRY(ctx->ws_dinv_idx.ensure(factors.size() * sizeof(int)));
+    WM_TRY(copy_h2d(ctx->ws_dinv_idx.ptr, factors.data(), factors.size() * sizeof(int), s));
+    const dim3 grid((unsigned)(n / TB), 2, (unsigned)factors.size());
+    const int *fidx = (const int *)ctx->ws_dinv_idx.ptr;
+    if (prec == WM_PREC_F64)
+        hipLaunchKernelGGL(k_diag_inv<double>, grid, dim3(TB), 0, s, F, lda, strideF, fidx, n / TB, Dinv);
+    else
+        hipLaunchKernelGGL(k_diag_inv<float>, grid, dim3(TB), 0, s, F, lda, strideF, fidx, n / TB, Dinv);
+    WM_HIP_TRY(hipGetLastError());
+    return WM_OK;
+}
+
+int getrs_grouped(wm_ctx *ctx, int prec, int n, int nr, const void *F, int lda, int64_t strideF, const int *perm,
+                  int64_t strideQ, double2 *V, int64_t strideV, const std::vector<int> &items,
+                  const std::vector<int> &fac, hipStream_t s, const double2 *dinv, const std::vector<int> *slot_of,
+                  int max_w)
+{
+    if (items.empty()) return WM_OK;
+    if (items.size() != fac.size() || nr <= 0 || strideV < (int64_t)n * nr) {
+        set_error("getrs_grouped: bad arguments");
+        return WM_ERR_ARG;
+    }
+    // the columns of each factor, factors in the order of their first item
+    std::vector<int> order;
+    std::vector<std::vector<int64_t>> cols_of;
+    std::vector<int> pos_of;  // factor index -> position in order (grown on demand)
+    for (size_t q = 0; q < items.size(); ++q) {
+        const int f = fac[q];
+        if (f < 0) {
+            set_error("getrs_grouped: item %d has no factors", items[q]);
+            return WM_ERR_ARG;
+        }
+        if ((size_t)f >= pos_of.size()) pos_of.resize((size_t)f + 1, -1);
+        if (pos_of[f] < 0) {
+            pos_of[f] = (int)order.size();
+            order.push_back(f);
+            cols_of.emplace_back();
+        }
+        for (int r = 0; r < nr; ++r) cols_of[pos_of[f]].push_back((int64_t)items[q] * strideV + (int64_t)r * n);
+    }
+    size_t widest = 0;
+    for (const auto &c : cols_of) widest = std::max(widest, c.size());
+    int W = 1;  // right-hand sides per launch member: a power of two, at most max_w (<= 8)
+    while (W < std::min(max_w, 8) && (size_t)W < widest) W *= 2;
+    std::vector<int> aidx;
+    std::vector<int64_t> cols;
+    for (size_t g = 0; g < order.size(); ++g)
+        for (size_t j0 = 0; j0 < cols_of[g].size(); j0 += W) {
+            aidx.push_back(order[g]);
+            for (int j = 0; j < W; ++j)
+                cols.push_back(j0 + j < cols_of[g].size() ? cols_of[g][j0 + j] : (int64_t)-1);
+        }
+    const int members = (int)aidx.size();
+    std::vector<int> dslot(members, 0);
+    if (dinv) {
+        for (int m = 0; m < members; ++m) {
+            const int f = aidx[m];
+            if (!slot_of || f >= (int)slot_of->size() || (*slot_of)[f] < 0) {
+                set_error("getrs_grouped: factor %d has no inverted diagonal blocks", f);
+                return WM_ERR_ARG;
+            }
+            dslot[m] = (*slot_of)[f];
+        }
+    }
+    const size_t aidx_b = ((size_t)members * sizeof(int) + 255) & ~(size_t)255;
+    const size_t cols_b = ((size_t)members * W * sizeof(int64_t) + 255) & ~(size_t)255;
+    const size_t stage_b = (size_t)members * W * n * sizeof(double2);
+    WM_TRY(ctx->ws_group.ensure(2 * aidx_b + cols_b + stage_b));
+    char *base = (char *)ctx->ws_group.ptr;
+    std::vector<char> host(2 * aidx_b + cols_b, 0);
+    std::memcpy(host.data(), aidx.data(), aidx.size() * sizeof(int));
+    std::memcpy(host.data() + aidx_b, dslot.data(), dslot.size() * sizeof(int));
+    std::memcpy(host.data() + 2 * aidx_b, cols.data(), cols.size() * sizeof(int64_t));
+    WM_TRY(copy_h2d(base, host.data(), host.size(), s));
+    const int *d_aidx = (const int *)base;
+    const int *d_dslot = (const int *)(base + aidx_b);
+    const int64_t *d_cols = (const int64_t *)(base + 2 * aidx_b);
+    double2 *S = (double2 *)(base + 2 * aidx_b + cols_b);
+    const dim3 grid((unsigned)((n + 255) / 256), (unsigned)(members * W));
+    hipLaunchKernelGGL(k_gather_cols, grid, dim3(256), 0, s, (const double2 *)V, n, d_cols, S);
+    WM_TRY(getrs_any(ctx, prec, n, W, F, lda, strideF, perm, strideQ, S, n, (int64_t)W * n, members, s, nullptr,
+                     false, d_aidx, nullptr, dinv, dinv ? d_dslot : nullptr));
+    hipLaunchKernelGGL(k_scatter_cols, grid, dim3(256), 0, s, (const double2 *)S, n, d_cols, V);
+    WM_HIP_TRY(hipGetLastError());
+    return WM_OK;
 }
 
 int getrs_batched(wm_ctx *ctx, int prec, int n, int nrhs, const void *A, int lda, int64_t strideA, const int *ipiv,
@@ -631,6 +885,38 @@ extern "C" int wm_getrs_batched_noalloc(wm_ctx *ctx, int prec, int n, int nrhs, 
     WM_TRY(getrs_any(ctx, prec, n, nrhs, A, lda, strideA, perm, n, B, ldb, strideB, batch, s, nullptr, false, nullptr,
                      &sb));
     return check_device_error(ctx, s);
+}
+
+// Internal test hook (not part of include/wiremom.h): grouped solves.  Item q
+// (vectors at V + items[q] * strideV, nr columns of n) is solved in place with
+// the factors fac[q] of wm_getrf_batched (device buffers, stride n * n; pivots
+// stride n).
+extern "C" int wmx_getrs_grouped(wm_ctx *ctx, int prec, int n, int nr, const void *A, int nfac, const int *ipiv,
+                                 wm_c128 *V, int64_t strideV, int nitems, const int *items, const int *fac, int inv,
+                                 int max_w)
+{
+    using namespace wm;
+    if (!ctx || !A || !ipiv || !V || nitems <= 0 || nfac <= 0) {
+        set_error("wmx_getrs_grouped: bad arguments");
+        return WM_ERR_ARG;
+    }
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    WM_TRY(ctx->ws_perm.ensure((size_t)nfac * n * sizeof(int)));
+    int *perm = (int *)ctx->ws_perm.ptr;
+    WM_TRY(perm_batched(ctx, n, ipiv, n, perm, n, nfac, ctx->stream));
+    const double2 *dinv = nullptr;
+    std::vector<int> facs(nfac), slot(nfac);
+    for (int f = 0; f < nfac; ++f) facs[f] = slot[f] = f;
+    if (inv) {
+        WM_TRY(ctx->ws_dinv.ensure(diag_inv_bytes(n, nfac)));
+        dinv = (const double2 *)ctx->ws_dinv.ptr;
+        WM_TRY(diag_inv_batched(ctx, prec, n, A, n, (int64_t)n * n, facs, (double2 *)dinv, ctx->stream));
+    }
+    WM_TRY(getrs_grouped(ctx, prec, n, nr, A, n, (int64_t)n * n, perm, n, (double2 *)V, strideV,
+                         std::vector<int>(items, items + nitems), std::vector<int>(fac, fac + nitems), ctx->stream,
+                         dinv, inv ? &slot : nullptr, max_w > 0 ? max_w : kGroupCols));
+    return check_device_error(ctx, ctx->stream);
 }
 
 // Internal test hook (not part of include/wiremom.h): solves A^H X = B with the

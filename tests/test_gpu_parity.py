@@ -489,25 +489,28 @@ def test_getrs_alternating_layouts(wm, ctx):
 def test_mixed_precision_fallback_band(wm, geometries, ctx, geo_dir):
     """Across a 10 kHz - 10 MHz band the low frequencies are too ill-conditioned
     for an FP32 factorization; those items are refactored in FP64 inside the same
-    sweep, and every Zin still matches the FP64 path within 1e-10."""
+    sweep, and every Zin still matches the FP64 path within 1e-10, or within
+    kappa_1 u64 where the matrix's condition makes two FP64-accurate solutions
+    differ by more (kappa_1 from the FP64 sweep's estimate)."""
     p = os.path.join(geo_dir, "light1024.geo")
     geometries.lightning(p, 1024)
     g = wm.Geometry.load(ctx, p)
     t = wm.configwire(p, cwd=geo_dir)
     gap = geometries.nearest_basis(t, (0, 0, 19.9))
     f = np.logspace(-2, 1, 12).astype(np.float32)
-    z64, _, _ = wm.solve_sweep(g, f, [gap], "f64")
+    z64, _, s64 = wm.solve_sweep(g, f, [gap], "f64", cond_est=True)
+    bound = np.array([max(1e-10, s["cond1"] * 2.0 ** -53) for s in s64])
     z32, _, st = wm.solve_sweep(g, f, [gap], "f32", size_policy=False)
     rel = np.abs(z32[:, 0] - z64[:, 0]) / np.abs(z64[:, 0])
-    print([(float(fr), s["iters"], s["fell_back"], float(r)) for fr, s, r in zip(f, st, rel)])
-    assert (rel <= 1e-10).all(), rel
+    print([(float(fr), s["iters"], s["fell_back"], float(r), float(b)) for fr, s, r, b in zip(f, st, rel, bound)])
+    assert (rel <= bound).all(), (rel, bound)
     assert any(s["fell_back"] for s in st)
     # batches of 4: the 4 highest frequencies are refined first and their
     # contraction rates route the lowest ones to FP64 factors directly
     z32b, _, stb = wm.solve_sweep(g, f, [gap], "f32", batch=4, size_policy=False)
     relb = np.abs(z32b[:, 0] - z64[:, 0]) / np.abs(z64[:, 0])
     print([(float(fr), s["iters"], s["fell_back"], float(r)) for fr, s, r in zip(f, stb, relb)])
-    assert (relb <= 1e-10).all(), relb
+    assert (relb <= bound).all(), (relb, bound)
     routed = [s["fell_back"] == 2 for s in stb]
     assert routed[0] and not any(routed[-4:])  # lowest routed, highest refined
     for a, b in zip(st, stb):  # routing only skips refinements that fail or take most of max_iter
@@ -517,7 +520,7 @@ def test_mixed_precision_fallback_band(wm, geometries, ctx, geo_dir):
     zb, _, stl = wm.solve_sweep(g, f, [gap], "bf16x3", batch=4, size_policy=False)
     rell = np.abs(zb[:, 0] - z64[:, 0]) / np.abs(z64[:, 0])
     print([(float(fr), s["iters"], s["prec"], s["fell_back"], float(r)) for fr, s, r in zip(f, stl, rell)])
-    assert (rell <= 1e-10).all(), rell
+    assert (rell <= bound).all(), (rell, bound)
     assert stl[-1]["prec"] == wm.PREC_BF16X3 and stl[0]["prec"] == wm.PREC_F64
 
 

@@ -24,11 +24,16 @@
 // length as products with its reciprocal, where the reference divides
 // ("wersja referencyjna/Zmatrix.cpp":679-688); the results differ from its
 // quotients by about an ulp and the kernel issues a third fewer FP64 operations.
+// cos kR and sin kR come from sincos_phase (wm_sincos.h: Cody-Waite reduction
+// and Taylor polynomials, within about 1.5 ulp of the exact values for kR up
+// to 100 and 2.5 ulp up to its limit), which needs a fraction of the
+// operations and registers of a general-purpose sincos.
 //
 // PEC ground (ienv 2, "wersja referencyjna/Zmatrix.cpp":94-145, 174-224): every
 // source half is followed by its image in z = 0 (coordinates scaled by the
 // environment signs, weight sgnenv), which doubles the kernel evaluations.
 #include "wm_internal.h"
+#include "wm_sincos.h"
 
 namespace wm {
 
@@ -199,7 +204,7 @@ __global__ __launch_bounds__(256) void k_fill(const double *__restrict__ obs, co
 #pragma unroll
                         for (int f = 0; f < F; ++f) {
                             double sn, cn;
-                            sincos(fa.ka[f] * R, &sn, &cn);
+                            sincos_phase(fa.ka[f] * R, sn, cn);
                             const double gr = cn * rR, gi = -sn * rR;
                             V[f].x = fma(w, gr, V[f].x); V[f].y = fma(w, gi, V[f].y);
                             S[f].x += gr;                S[f].y += gi;
@@ -282,6 +287,14 @@ int fill_launch(wm_ctx *ctx, const wm_geom *g, int nf, const double *ka, void *Z
     WM_TRY(ws.ensure(self_bytes));
     double2 *self = (double2 *)ws.ptr;
     const dim3 grid(ld / 64, ld / (kWaves * kColsPerWave));
+    // the phases k R stay inside the range of the fill's argument reduction
+    // (R is at most the structure's diameter, images included, plus a radius)
+    for (int f = 0; f < nf; ++f)
+        if (!(ka[f] > 0.0) || ka[f] * (2.0 * g->diam + 1.0) >= kSinCosMax) {
+            set_error("fill: wavenumber %g on a structure of diameter %g m is outside the fill's phase range", ka[f],
+                      g->diam);
+            return WM_ERR_ARG;
+        }
     for (int f0 = 0; f0 < nf; f0 += kMaxF) {
         const int F = (nf - f0 < kMaxF) ? nf - f0 : kMaxF;
         FreqArgs fa;

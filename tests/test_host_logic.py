@@ -121,3 +121,59 @@ def test_build_rejects_spilling_fill_kernels():
     with pytest.raises(RuntimeError):
         b.check_spill_free("wm_fill.hip", other)   # no fill remarks at all
     b.check_spill_free("wm_lu.hip", other)          # not a checked source
+
+
+def test_phase_sincos_accuracy(tmp_path):
+    """The fill's sin/cos of k R (Cody-Waite reduction, Taylor polynomials):
+    against long-double sinl/cosl at 300k random arguments in [0, 1], [0, 100],
+    [0, 1e4] and [0, 1.5e6], error within 2 ulp below 100 and 3 ulp up to the
+    limit (ulp of the result); exact at 0; n = rint(x 2/pi) quadrants checked
+    at multiples of pi/2, where one of the two is 0 to within the reduction's
+    absolute error."""
+    import os
+    import subprocess
+    hdr = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "praca-magisterska_amd", "csrc")
+    src = tmp_path / "sc.cpp"
+    src.write_text(r'''
+#include <cstdio>
+#include <cmath>
+#include <random>
+#include "wm_sincos.h"
+static double ulps(double v, long double ref) {
+    const double m = std::fmax(std::fabs((double)ref), 1e-300);
+    return std::fabs((double)((v - ref) / (long double)std::ldexp(1.0, std::ilogb(m) - 52)));
+}
+int main() {
+    std::mt19937_64 g(7);
+    const double hi[4] = {1.0, 100.0, 1e4, 1.5e6};
+    for (int k = 0; k < 4; ++k) {
+        std::uniform_real_distribution<double> U(0.0, hi[k]);
+        double w = 0;
+        for (int i = 0; i < 300000; ++i) {
+            const double x = U(g);
+            double s, c;
+            wm::sincos_phase(x, s, c);
+            w = std::fmax(w, std::fmax(ulps(s, sinl((long double)x)), ulps(c, cosl((long double)x))));
+        }
+        printf("%g %.4f\n", hi[k], w);
+    }
+    double s, c;
+    wm::sincos_phase(0.0, s, c);
+    printf("zero %.17g %.17g\n", s, c);
+    for (int q = 1; q <= 8; ++q) {
+        const long double xq = q * 1.5707963267948966192313216916397514L;
+        wm::sincos_phase((double)xq, s, c);
+        printf("q %d %.17g %.17g %.17Lg %.17Lg\n", q, s, c, sinl((long double)(double)xq), cosl((long double)(double)xq));
+    }
+}
+''')
+    exe = tmp_path / "sc"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-I", hdr, str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines()
+    worst = {float(l.split()[0]): float(l.split()[1]) for l in out[:4]}
+    assert worst[1.0] <= 2.0 and worst[100.0] <= 2.0, worst
+    assert worst[1e4] <= 3.0 and worst[1.5e6] <= 3.0, worst
+    assert out[4] == "zero 0 1", out[4]
+    for l in out[5:]:
+        _, q, s, c, sr, cr = l.split()
+        assert abs(float(s) - float(sr)) <= 1e-15 and abs(float(c) - float(cr)) <= 1e-15, l

@@ -54,6 +54,8 @@ CLASSES = {
 # HBM bytes / algorithmic bytes of k_gemm_f64, from FETCH_SIZE + WRITE_SIZE passes
 # calibrated on a plain copy (profiles/gemm_f64_traffic_c3_n8192.md)
 GEMM_F64_HBM_RATIO = 1.91
+# tuning cache written by `bench.py --tune` on the benchmark's device (JSON lines, wm_tune_save)
+TUNE_CACHE = os.path.join(ROOT, "praca-magisterska_amd", "tune_cache.json")
 REF_BEST_PUBLISHED = 0.148  # freq-solves/s at N=8192, reference's best variant (GPU fill + MKL LU, 6 procs)
 U64 = 2.0 ** -53
 
@@ -310,6 +312,11 @@ def main():
                     help="only the warmup and timed steps of the value sweep (for rocprofv3), then a short JSON line")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run a plain device copy of MIB MiB (three times), the reference for HBM counters")
+    ap.add_argument("--tune", action="store_true",
+                    help="run the tuners (fill, and the factorization depth per precision) on the benchmark "
+                         "structure and write the cache to --tune-cache")
+    ap.add_argument("--tune-cache", default=TUNE_CACHE,
+                    help="tuning cache (JSON lines): loaded if present, written by --tune")
     ap.add_argument("--requests", type=int, default=48)
     ap.add_argument("--rate", type=float, default=4.0)
     a = ap.parse_args()
@@ -338,6 +345,18 @@ def main():
     G.lightning(geo, 8192)
     freqs = G.freqs_c3()[: a.nfreq]
     g = wm.Geometry.load(ctx, geo, os.path.join(tmp, "configwire.txt"))
+    tuning = {"cache": None, "entries_applied": 0}
+    if a.tune:
+        tuned = {"fill": ctx.tune_fill(g)}
+        for prec in ("f64", "f32", "bf16x3"):
+            tuned[prec] = ctx.tune_getrf(prec, g.ld)
+        tuning = {"cache": a.tune_cache, "entries_written": ctx.tune_save(a.tune_cache),
+                  "tuned_ms": {k: round(v[1], 3) for k, v in tuned.items()}}
+        print(f"tuned: {tuned}", file=sys.stderr, flush=True)
+    elif a.tune_cache and os.path.exists(a.tune_cache):
+        tuning = {"cache": os.path.relpath(a.tune_cache, ROOT), "entries_applied": ctx.tune_load(a.tune_cache)}
+    tuning["getrf_block"] = {p_: ctx.getrf_block(p_, g.ld) for p_ in ("f64", "f32", "bf16x3")}
+    tuning["fill_freqs_per_launch"] = ctx.fill_freqs(1, g.ld) or 4
     t = wm.configwire(geo, cwd=tmp)
     gap = G.nearest_basis(t, (0.0, 0.0, 19.9))
 
@@ -362,7 +381,7 @@ def main():
     if a.profile_only:
         prof = profiled(ctx, lambda: sweep(a.prec, a.recycle))
         out = {"value": value, "unit": "freq-solves/s", "steps": a.steps, "warmup": a.warmup,
-               "recycle": a.recycle, "prec": a.prec, "hip_graphs": not a.no_graphs,
+               "recycle": a.recycle, "prec": a.prec, "hip_graphs": not a.no_graphs, "tuning": tuning,
                "roofline": roofline(prof), "kernel_ms_profiled_step": {k: round(v[0], 3) for k, v in prof.items()}}
         if a.calibrate_copy:
             out["calibrate_copy_bytes"] = calibrate_copy(wm, ctx, a.calibrate_copy)
@@ -474,6 +493,7 @@ def main():
         "reference_best_published_solves_per_s": REF_BEST_PUBLISHED,
         "device": f"{name} x{ncu} CU",
         "hip_graphs": not a.no_graphs,
+        "tuning": tuning,
         "timing": "value and every solves/s figure: wall time of whole sweeps, profiler off; per-kernel times "
                   "and rooflines from one separate profiled step",
     }

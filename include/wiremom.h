@@ -136,15 +136,34 @@ int wm_getrf_batched(wm_ctx *ctx, int prec, int n, void *A, int lda, int64_t str
  * context used while other threads drive the same device should launch
  * directly (the request server's lanes do). */
 int wm_set_graphs(wm_ctx *ctx, int on);
-/* Factorization tuning.  The blocked LU applies a trailing update after every
+/* Tuning (replaces the reference's fixed (8, 8) launch shape of the fill,
+ * "wersja referencyjna/Wire_MoM.cu":4110-4113, and its fixed cuBLAS calls).
+ *
+ * Factorization: the blocked LU applies a trailing update after every
  * `panels` panels of 64 columns (K = 64 * panels), per factor precision and
- * size class (n / 1024).  wm_set_getrf_block sets it (0 = default, 4; else
- * 1, 2, 4 or 8); wm_tune_getrf times one batch of `batch` (0: one launch
- * width) synthetic n x n factorizations for each choice, keeps the fastest
- * for this context and returns it with its time. */
+ * size class (n / 1024).  wm_set_getrf_block sets it (0 = default: 4, and 8
+ * at n >= 8192; else 1, 2, 4, 8 or 16); wm_tune_getrf times one batch of
+ * `batch` (0: one launch width) synthetic n x n factorizations for each
+ * choice, keeps the fastest for this context and returns it with its time. */
 int wm_set_getrf_block(wm_ctx *ctx, int prec, int n, int panels);
 int wm_get_getrf_block(wm_ctx *ctx, int prec, int n);
 int wm_tune_getrf(wm_ctx *ctx, int prec, int n, int batch, int *best_panels, double *ms_best);
+/* Fill: frequencies per fill launch (1-4; 0 = default, 4), per environment
+ * (ienv 1 free space, 2 PEC ground) and size class (ld / 1024).  A matrix's
+ * entries do not depend on it (bit for bit), only the time does.
+ * wm_tune_fill times fills of four frequencies of `g` at each setting, keeps
+ * the fastest for g's environment and size class and returns it with the time
+ * per frequency. */
+int wm_set_fill_freqs(wm_ctx *ctx, int ienv, int n, int freqs);
+int wm_get_fill_freqs(wm_ctx *ctx, int ienv, int n);
+int wm_tune_fill(wm_ctx *ctx, const wm_geom *g, int *best_freqs, double *ms_per_freq);
+/* Tuning cache: wm_tune_save writes every setting made on this context
+ * (tuned or set) as JSON lines keyed by (architecture, knob, precision or
+ * environment, size class); wm_tune_load applies the entries of a file that
+ * were made on this device's architecture and returns how many it applied
+ * (negative: error).  Unknown keys and other architectures are skipped. */
+int wm_tune_save(wm_ctx *ctx, const char *path);
+int wm_tune_load(wm_ctx *ctx, const char *path);
 /* Solves A X = B with the factors of wm_getrf_batched; B complex128
  * (n x nrhs, ldb) per member, overwritten by X.  The factors may be complex64
  * (prec != WM_PREC_F64); the solve then runs in FP64 on widened values. */

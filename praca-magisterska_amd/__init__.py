@@ -36,7 +36,8 @@ EXPORTS = [
     "wm_last_error", "wm_version", "wm_init", "wm_shutdown", "wm_sync", "wm_device_info", "wm_prof_enable",
     "wm_prof_get", "wm_dev_alloc",
     "wm_dev_free", "wm_memcpy_h2d", "wm_memcpy_d2h", "wm_memcpy_d2d", "wm_geom_create", "wm_geom_create_env", "wm_geom_load", "wm_geom_load_env", "wm_geom_destroy",
-    "wm_geom_npls", "wm_geom_ld", "wm_zfill", "wm_getrf_batched", "wm_getrs_batched", "wm_set_getrf_block", "wm_get_getrf_block", "wm_tune_getrf", "wm_set_graphs", "wm_solve_opts_default",
+    "wm_geom_npls", "wm_geom_ld", "wm_zfill", "wm_getrf_batched", "wm_getrs_batched", "wm_set_getrf_block", "wm_get_getrf_block", "wm_tune_getrf", "wm_set_fill_freqs", "wm_get_fill_freqs",
+    "wm_tune_fill", "wm_tune_save", "wm_tune_load", "wm_set_graphs", "wm_solve_opts_default",
     "wm_solve_sweep", "wm_solve_sweep_ex", "wm_transient", "wm_spectrum_dexp", "wm_server_opts_default", "wm_server_create", "wm_server_submit", "wm_server_step",
     "wm_server_result", "wm_server_status", "wm_server_done", "wm_server_destroy", "geoinp", "config", "configwire", "tablice", "zmatrix",
     "zmatrix_gpu", "zmatrix_gpu_rank", "LU_cuBLAS", "CalculateByFreq",
@@ -139,6 +140,11 @@ def lib() -> C.CDLL:
         L.wm_set_graphs.argtypes = [vp, C.c_int]
         L.wm_get_getrf_block.argtypes = [vp, C.c_int, C.c_int]
         L.wm_tune_getrf.argtypes = [vp, C.c_int, C.c_int, C.c_int, ip, dp]
+        L.wm_set_fill_freqs.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+        L.wm_get_fill_freqs.argtypes = [vp, C.c_int, C.c_int]
+        L.wm_tune_fill.argtypes = [vp, vp, ip, dp]
+        L.wm_tune_save.argtypes = [vp, C.c_char_p]
+        L.wm_tune_load.argtypes = [vp, C.c_char_p]
         L.wm_solve_opts_default.argtypes = [_p(SolveOpts)]
         L.wm_solve_opts_default.restype = None
         L.wm_solve_sweep.argtypes = [vp, vp, _p(SolveOpts), C.c_int, fp, C.c_int, ip, C.c_float, C.c_float, vp, vp,
@@ -311,6 +317,33 @@ class Context:
 
     def getrf_block(self, prec, n: int) -> int:
         return lib().wm_get_getrf_block(self.h, _prec(prec), n)
+
+    def set_fill_freqs(self, ienv: int, n: int, freqs: int):
+        """Frequencies per fill launch (1-4, 0 = default) for environment ienv and the size class of n."""
+        _check(lib().wm_set_fill_freqs(self.h, ienv, n, freqs))
+
+    def fill_freqs(self, ienv: int, n: int) -> int:
+        return lib().wm_get_fill_freqs(self.h, ienv, n)
+
+    def tune_fill(self, geom):
+        """Times the fill of geom at 1-4 frequencies per launch, keeps the fastest; returns (freqs, ms per frequency)."""
+        best, ms = C.c_int(), C.c_double()
+        _check(lib().wm_tune_fill(self.h, geom.h, C.byref(best), C.byref(ms)))
+        return best.value, ms.value
+
+    def tune_save(self, path: str) -> int:
+        """Writes this context's tuning settings as JSON lines; returns the number of entries."""
+        n = lib().wm_tune_save(self.h, path.encode())
+        if n < 0:
+            _check(n)
+        return n
+
+    def tune_load(self, path: str) -> int:
+        """Applies the entries of a tuning cache made on this architecture; returns how many."""
+        n = lib().wm_tune_load(self.h, path.encode())
+        if n < 0:
+            _check(n)
+        return n
 
     def tune_getrf(self, prec, n: int, batch: int = 0):
         """Times each outer-block depth on synthetic matrices, keeps the fastest; returns (panels, ms)."""

@@ -250,6 +250,8 @@ int wm_init(int device, wm_ctx **out)
     c->hbm_bytes = p.totalGlobalMem;
     c->max_lds = p.sharedMemPerBlock;
     std::snprintf(c->name, sizeof c->name, "%s (%s)", p.name, p.gcnArchName);
+    std::snprintf(c->arch, sizeof c->arch, "%s", p.gcnArchName);
+    if (char *colon = std::strchr(c->arch, ':')) *colon = 0;  // "gfx950:sramecc+:xnack-" -> "gfx950"
     WM_HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     WM_HIP_TRY(hipMalloc(&c->d_sync, sizeof(int) * wm_ctx::kSyncCap));
     WM_HIP_TRY(hipMalloc(&c->d_err, sizeof(int)));
@@ -651,7 +653,10 @@ __global__ void k_tune_fill(void *A, int n, int esz, int batch)
     else ((float2 *)A)[t] = make_float2((float)(x + d), (float)y);
 }
 
-int valid_block(int panels) { return panels == 0 || panels == 1 || panels == 2 || panels == 4 || panels == 8; }
+int valid_block(int panels)
+{
+    return panels == 0 || panels == 1 || panels == 2 || panels == 4 || panels == 8 || panels == 16;
+}
 
 }  // namespace
 
@@ -669,7 +674,7 @@ int wm_set_graphs(wm_ctx *c, int on)
 int wm_set_getrf_block(wm_ctx *c, int prec, int n, int panels)
 {
     if (!c || prec < 0 || prec > 2 || n <= 0 || !valid_block(panels)) {
-        set_error("wm_set_getrf_block: bad arguments (panels must be 0, 1, 2, 4 or 8)");
+        set_error("wm_set_getrf_block: bad arguments (panels must be 0, 1, 2, 4, 8 or 16)");
         return WM_ERR_ARG;
     }
     std::lock_guard<std::recursive_mutex> lk(c->mu);
@@ -708,7 +713,7 @@ int wm_tune_getrf(wm_ctx *c, int prec, int n, int batch, int *best, double *ms_b
     float bestms = 0.f;
     int rc = WM_OK;
     const unsigned nb = (unsigned)((nn * batch + 255) / 256);
-    for (int panels : {1, 2, 4, 8}) {
+    for (int panels : {1, 2, 4, 8, 16}) {
         if (panels > 1 && panels * 64 > n) break;
         slot = panels;
         float ms = 0.f;
@@ -735,6 +740,172 @@ int wm_tune_getrf(wm_ctx *c, int prec, int n, int batch, int *best, double *ms_b
     if (best) *best = bestp;
     if (ms_best) *ms_best = bestms;
     return WM_OK;
+}
+
+int wm_set_fill_freqs(wm_ctx *c, int ienv, int n, int freqs)
+{
+    if (!c || (ienv != 1 && ienv != 2) || n <= 0 || freqs < 0 || freqs > 4) {
+        set_error("wm_set_fill_freqs: bad arguments (ienv 1 or 2, freqs 0-4)");
+        return WM_ERR_ARG;
+    }
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    c->fill_freqs[ienv - 1][std::min(n / 1024, wm_ctx::kTuneSizes - 1)] = freqs;
+    return WM_OK;
+}
+
+int wm_get_fill_freqs(wm_ctx *c, int ienv, int n)
+{
+    if (!c || (ienv != 1 && ienv != 2) || n <= 0) return 0;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    return c->fill_freqs[ienv - 1][std::min(n / 1024, wm_ctx::kTuneSizes - 1)];
+}
+
+int wm_tune_fill(wm_ctx *c, const wm_geom *g, int *best, double *ms_per_freq)
+{
+    if (!c || !g || g->ctx != c) {
+        set_error("wm_tune_fill: bad arguments");
+        return WM_ERR_ARG;
+    }
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    (void)hipSetDevice(c->device);
+    const int ld = g->ld, nf = 4;
+    const int64_t nn = (int64_t)ld * ld;
+    // four wavenumbers of a low-frequency sweep (the fill's cost hardly depends on k)
+    double ka[nf];
+    for (int f = 0; f < nf; ++f) ka[f] = wavenumber_mhz(1.0f + 2.0f * f);
+    DevMem Z;
+    WM_TRY(Z.alloc((size_t)nn * nf * sizeof(double2)));
+    hipEvent_t e0, e1;
+    WM_HIP_TRY(hipEventCreate(&e0));
+    WM_HIP_TRY(hipEventCreate(&e1));
+    int &slot = c->fill_freqs[g->ienv == 2 ? 1 : 0][std::min(ld / 1024, wm_ctx::kTuneSizes - 1)];
+    const int saved = slot;
+    int bestf = 0, rc = WM_OK;
+    float bestms = 0.f;
+    for (int F = 1; F <= 4 && rc == WM_OK; ++F) {
+        slot = F;
+        float ms = 0.f;
+        for (int rep = 0; rep < 2 && rc == WM_OK; ++rep) {  // the first run warms up
+            if (hipEventRecord(e0, c->stream) != hipSuccess) rc = WM_ERR_HIP;
+            if (rc == WM_OK) rc = fill_launch(c, g, nf, ka, Z.p, nn, nullptr, 0, c->stream);
+            if (rc == WM_OK && hipEventRecord(e1, c->stream) != hipSuccess) rc = WM_ERR_HIP;
+            if (rc == WM_OK && hipEventSynchronize(e1) != hipSuccess) rc = WM_ERR_HIP;
+            if (rc == WM_OK && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = WM_ERR_HIP;
+        }
+        if (rc == WM_OK && (bestf == 0 || ms < bestms)) {
+            bestf = F;
+            bestms = ms;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc == WM_OK) rc = check_device_error(c, c->stream);
+    slot = rc == WM_OK ? bestf : saved;
+    if (rc != WM_OK) return rc;
+    if (best) *best = bestf;
+    if (ms_per_freq) *ms_per_freq = bestms / nf;
+    return WM_OK;
+}
+
+namespace {
+
+// Value of "key": <int> in a JSON line, or false.
+bool json_int(const char *line, const char *key, long *out)
+{
+    char pat[64];
+    std::snprintf(pat, sizeof pat, "\"%s\":", key);
+    const char *p = std::strstr(line, pat);
+    if (!p) return false;
+    p += std::strlen(pat);
+    char *end = nullptr;
+    const long v = std::strtol(p, &end, 10);
+    if (end == p) return false;
+    *out = v;
+    return true;
+}
+
+// Value of "key": "<string>" in a JSON line, or empty.
+std::string json_str(const char *line, const char *key)
+{
+    char pat[64];
+    std::snprintf(pat, sizeof pat, "\"%s\":", key);
+    const char *p = std::strstr(line, pat);
+    if (!p) return std::string();
+    p += std::strlen(pat);
+    while (*p == ' ') ++p;
+    if (*p != '"') return std::string();
+    const char *q = std::strchr(++p, '"');
+    return q ? std::string(p, q) : std::string();
+}
+
+}  // namespace
+
+int wm_tune_save(wm_ctx *c, const char *path)
+{
+    if (!c || !path) {
+        set_error("wm_tune_save: bad arguments");
+        return WM_ERR_ARG;
+    }
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    FILE *f = std::fopen(path, "w");
+    if (!f) {
+        set_error("wm_tune_save: cannot write %s", path);
+        return WM_ERR_ARG;
+    }
+    int n = 0;
+    for (int p = 0; p < 3; ++p)
+        for (int sc = 0; sc < wm_ctx::kTuneSizes; ++sc)
+            if (c->getrf_block[p][sc] > 0) {
+                std::fprintf(f, "{\"arch\": \"%s\", \"knob\": \"getrf_block\", \"prec\": %d, \"size_class\": %d, "
+                                "\"value\": %d}\n", c->arch, p, sc, c->getrf_block[p][sc]);
+                ++n;
+            }
+    for (int e = 0; e < 2; ++e)
+        for (int sc = 0; sc < wm_ctx::kTuneSizes; ++sc)
+            if (c->fill_freqs[e][sc] > 0) {
+                std::fprintf(f, "{\"arch\": \"%s\", \"knob\": \"fill_freqs\", \"ienv\": %d, \"size_class\": %d, "
+                                "\"value\": %d}\n", c->arch, e + 1, sc, c->fill_freqs[e][sc]);
+                ++n;
+            }
+    const bool ok = std::fclose(f) == 0;
+    if (!ok) {
+        set_error("wm_tune_save: write to %s failed", path);
+        return WM_ERR_ARG;
+    }
+    return n;
+}
+
+int wm_tune_load(wm_ctx *c, const char *path)
+{
+    if (!c || !path) {
+        set_error("wm_tune_load: bad arguments");
+        return WM_ERR_ARG;
+    }
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    FILE *f = std::fopen(path, "r");
+    if (!f) {
+        set_error("wm_tune_load: cannot read %s", path);
+        return WM_ERR_ARG;
+    }
+    char line[512];
+    int applied = 0;
+    while (std::fgets(line, sizeof line, f)) {
+        if (json_str(line, "arch") != c->arch) continue;
+        const std::string knob = json_str(line, "knob");
+        long sc = -1, v = -1, k = -1;
+        if (!json_int(line, "size_class", &sc) || !json_int(line, "value", &v) || sc < 0 ||
+            sc >= wm_ctx::kTuneSizes)
+            continue;
+        if (knob == "getrf_block" && json_int(line, "prec", &k) && k >= 0 && k <= 2 && valid_block((int)v) && v > 0) {
+            c->getrf_block[k][sc] = (int)v;
+            ++applied;
+        } else if (knob == "fill_freqs" && json_int(line, "ienv", &k) && (k == 1 || k == 2) && v >= 1 && v <= 4) {
+            c->fill_freqs[k - 1][sc] = (int)v;
+            ++applied;
+        }
+    }
+    std::fclose(f);
+    return applied;
 }
 
 int wm_getrs_batched(wm_ctx *c, int prec, int n, int nrhs, const void *A, int lda, int64_t strideA, const int *ipiv,

@@ -32,6 +32,8 @@
 // PEC ground (ienv 2, "wersja referencyjna/Zmatrix.cpp":94-145, 174-224): every
 // source half is followed by its image in z = 0 (coordinates scaled by the
 // environment signs, weight sgnenv), which doubles the kernel evaluations.
+#include <algorithm>
+
 #include "wm_internal.h"
 #include "wm_sincos.h"
 
@@ -295,8 +297,11 @@ int fill_launch(wm_ctx *ctx, const wm_geom *g, int nf, const double *ka, void *Z
                       g->diam);
             return WM_ERR_ARG;
         }
-    for (int f0 = 0; f0 < nf; f0 += kMaxF) {
-        const int F = (nf - f0 < kMaxF) ? nf - f0 : kMaxF;
+    // frequencies per launch: the tuned setting for the environment and size, else kMaxF
+    const int tuned = ctx->fill_freqs[g->ienv == 2 ? 1 : 0][std::min(ld / 1024, wm_ctx::kTuneSizes - 1)];
+    const int fmax = tuned >= 1 && tuned <= kMaxF ? tuned : kMaxF;
+    for (int f0 = 0; f0 < nf; f0 += fmax) {
+        const int F = (nf - f0 < fmax) ? nf - f0 : fmax;
         FreqArgs fa;
         for (int f = 0; f < kMaxF; ++f) {
             const double k = ka[f0 + (f < F ? f : 0)];

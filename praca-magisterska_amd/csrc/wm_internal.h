@@ -101,6 +101,10 @@ struct wm_ctx {
     // wm_tune_getrf or wm_set_getrf_block
     static const int kTuneSizes = 32;
     int getrf_block[3][kTuneSizes] = {};
+    // frequencies per fill launch per environment (free space, PEC ground)
+    // and size class (ld / 1024), 0 = default; wm_tune_fill or wm_set_fill_freqs
+    int fill_freqs[2][kTuneSizes] = {};
+    char arch[64] = {0};    // gcnArchName of the device (tuning cache key)
     wm::Prof prof;          // per-kernel-class event timing (off by default)
     // panel workgroups assumed co-resident (0 = the occupancy query's count);
     // lowered only by tests, to split panel launches at small sizes

@@ -819,10 +819,14 @@ __global__ __launch_bounds__(256, 2) void k_gemm_f32(float2 *Abase, int lda, int
 // chunks of 16.  Lane l holds srcA[l%32][8*(l/32)+i], srcB[8*(l/32)+i][l%32]
 // (i = 0..7) and D[8*(v/4)+4*(l/32)+v%4][l%32].  The chunk is staged in LDS as
 // four bf16 planes (re hi, re lo, im hi, im lo) with k contiguous per row of L
-// and per column of U, so each operand is one 16-byte LDS read.
+// and per column of U (32 bytes per row: the 16-byte reads and writes of eight
+// consecutive lanes cover all 64 banks once), so each operand is one 16-byte
+// LDS read.  Two LDS buffers (64 KiB, two workgroups per CU): while the waves
+// multiply one chunk they split and store the next (already in registers)
+// into the other buffer and load the one after, with one barrier per chunk.
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-constexpr int GB_BM = 128, GB_BN = 128, GB_KC = 16, GB_KP = GB_KC + 8;  // padded k stride (bf16)
+constexpr int GB_BM = 128, GB_BN = 128, GB_KC = 16;
 
 __device__ inline void split_bf16(float x, __bf16 &hi, __bf16 &lo)
 {
@@ -855,9 +859,9 @@ __global__ __launch_bounds__(256, 2) void k_gemm_b3(float2 *Abase, int lda, int6
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wi = (wave & 1) * 64, wj = (wave >> 1) * 64;
 
-    // planes: 0 re hi, 1 re lo, 2 im hi, 3 im lo
-    __shared__ __attribute__((aligned(16))) __bf16 sL[4][GB_BM][GB_KP];
-    __shared__ __attribute__((aligned(16))) __bf16 sU[4][GB_BN][GB_KP];
+    // [buffer][plane: 0 re hi, 1 re lo, 2 im hi, 3 im lo][row][k]
+    __shared__ __attribute__((aligned(16))) __bf16 sL[2][4][GB_BM][GB_KC];
+    __shared__ __attribute__((aligned(16))) __bf16 sU[2][4][GB_BN][GB_KC];
     // chunk elements moved by this thread: 8 consecutive k of one row of L and of one column of U
     const int lr = tid % GB_BM, lh = tid / GB_BM;  // L: row lr, k = 8*lh + e
     const int uc = tid / 2, uh = tid % 2;           // U: column uc, k = 8*uh + e
@@ -871,7 +875,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_b3(float2 *Abase, int lda, int6
 #pragma unroll
         for (int e = 0; e < 8; ++e) pb[e] = uok ? pB[e] : float2{0.f, 0.f};
     };
-    auto stash = [&]() {
+    auto stash = [&](int buf) {
         bf16x8 v[4];
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
@@ -884,7 +888,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_b3(float2 *Abase, int lda, int6
             v[3][e] = l;
         }
 #pragma unroll
-        for (int q = 0; q < 4; ++q) *(bf16x8 *)&sL[q][lr][8 * lh] = v[q];
+        for (int q = 0; q < 4; ++q) *(bf16x8 *)&sL[buf][q][lr][8 * lh] = v[q];
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
             __bf16 h, l;
@@ -896,7 +900,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_b3(float2 *Abase, int lda, int6
             v[3][e] = l;
         }
 #pragma unroll
-        for (int q = 0; q < 4; ++q) *(bf16x8 *)&sU[q][uc][8 * uh] = v[q];
+        for (int q = 0; q < 4; ++q) *(bf16x8 *)&sU[buf][q][uc][8 * uh] = v[q];
     };
     v16f accR[2][2], accI[2][2];
 #pragma unroll
@@ -906,12 +910,8 @@ __global__ __launch_bounds__(256, 2) void k_gemm_b3(float2 *Abase, int lda, int6
 #pragma unroll
             for (int v = 0; v < 16; ++v) { accR[x][y][v] = 0.f; accI[x][y][v] = 0.f; }
 
-    fetch(0);
-    stash();
-    __syncthreads();
     const int kofs = 8 * (lane / 32);
-    for (int kc = 0; kc < K; kc += GB_KC) {
-        if (kc + GB_KC < K) fetch(kc + GB_KC);
+    auto compute = [&](int buf) {
         // U operands (MFMA rows = columns of C), L operands (MFMA columns = rows of C)
         bf16x8 ur[2][2], ui[2][2], un[2][2], lre[2][2], lim[2][2];
 #pragma unroll
@@ -919,11 +919,11 @@ __global__ __launch_bounds__(256, 2) void k_gemm_b3(float2 *Abase, int lda, int6
             const int cj = wj + q * 32 + lane % 32, ri = wi + q * 32 + lane % 32;
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
-                ur[q][h] = *(const bf16x8 *)&sU[0 + h][cj][kofs];
-                ui[q][h] = *(const bf16x8 *)&sU[2 + h][cj][kofs];
+                ur[q][h] = *(const bf16x8 *)&sU[buf][0 + h][cj][kofs];
+                ui[q][h] = *(const bf16x8 *)&sU[buf][2 + h][cj][kofs];
                 un[q][h] = neg_bf16x8(ui[q][h]);
-                lre[q][h] = *(const bf16x8 *)&sL[0 + h][ri][kofs];
-                lim[q][h] = *(const bf16x8 *)&sL[2 + h][ri][kofs];
+                lre[q][h] = *(const bf16x8 *)&sL[buf][0 + h][ri][kofs];
+                lim[q][h] = *(const bf16x8 *)&sL[buf][2 + h][ri][kofs];
             }
         }
         // split terms (hi,hi), (hi,lo), (lo,hi); real and imaginary accumulators alternate
@@ -931,38 +931,49 @@ __global__ __launch_bounds__(256, 2) void k_gemm_b3(float2 *Abase, int lda, int6
         for (int term = 0; term < 3; ++term) {
             const int hu = term == 2 ? 1 : 0, hl = term == 1 ? 1 : 0;
 #pragma unroll
-            for (int tj = 0; tj < 2; ++tj)
+            for (int x = 0; x < 2; ++x)
 #pragma unroll
-                for (int ti = 0; ti < 2; ++ti) {
-                    accR[tj][ti] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ur[tj][hu], lre[ti][hl], accR[tj][ti], 0, 0, 0);
-                    accI[tj][ti] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ur[tj][hu], lim[ti][hl], accI[tj][ti], 0, 0, 0);
+                for (int y = 0; y < 2; ++y) {
+                    accR[x][y] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ur[x][hu], lre[y][hl], accR[x][y], 0, 0, 0);
+                    accI[x][y] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ur[x][hu], lim[y][hl], accI[x][y], 0, 0, 0);
                 }
 #pragma unroll
-            for (int tj = 0; tj < 2; ++tj)
+            for (int x = 0; x < 2; ++x)
 #pragma unroll
-                for (int ti = 0; ti < 2; ++ti) {
-                    accR[tj][ti] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(un[tj][hu], lim[ti][hl], accR[tj][ti], 0, 0, 0);
-                    accI[tj][ti] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ui[tj][hu], lre[ti][hl], accI[tj][ti], 0, 0, 0);
+                for (int y = 0; y < 2; ++y) {
+                    accR[x][y] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(un[x][hu], lim[y][hl], accR[x][y], 0, 0, 0);
+                    accI[x][y] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ui[x][hu], lre[y][hl], accI[x][y], 0, 0, 0);
                 }
         }
+    };
+
+    fetch(0);
+    stash(0);
+    if (GB_KC < K) fetch(GB_KC);
+    __syncthreads();
+    int buf = 0;
+    for (int kc = 0; kc < K; kc += GB_KC, buf ^= 1) {
+        // the next chunk (in registers) to the other buffer, the one after
+        // from memory, then this chunk's products
         if (kc + GB_KC < K) {
-            __syncthreads();
-            stash();
-            __syncthreads();
+            stash(buf ^ 1);
+            if (kc + 2 * GB_KC < K) fetch(kc + 2 * GB_KC);
         }
+        compute(buf);
+        __syncthreads();
     }
 #pragma unroll
-    for (int tj = 0; tj < 2; ++tj)
+    for (int x = 0; x < 2; ++x)
 #pragma unroll
-        for (int ti = 0; ti < 2; ++ti)
+        for (int y = 0; y < 2; ++y)
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
-                const int gi = i0 + wi + ti * 32 + lane % 32;
-                const int gj = j0 + wj + tj * 32 + 8 * (v / 4) + 4 * (lane / 32) + (v % 4);
+                const int gi = i0 + wi + y * 32 + lane % 32;
+                const int gj = j0 + wj + x * 32 + 8 * (v / 4) + 4 * (lane / 32) + (v % 4);
                 if (gi < mrows && gj < ncols) {
                     float2 c = C[gi + (size_t)gj * lda];
-                    c.x -= accR[tj][ti][v];
-                    c.y -= accI[tj][ti][v];
+                    c.x -= accR[x][y][v];
+                    c.y -= accI[x][y][v];
                     C[gi + (size_t)gj * lda] = c;
                 }
             }

@@ -209,7 +209,7 @@ def test_trailing_update_step(wm, ctx, prec, bound):
     assert np.array_equal(got[:r], A64[:r]) and np.array_equal(got[:, :r], A64[:, :r])
 
 
-@pytest.mark.parametrize("panels", [1, 2, 8])
+@pytest.mark.parametrize("panels", [1, 2, 8, 16])
 def test_getrf_outer_block_depths(wm, ctx, oracle, panels):
     """Every outer-block depth (panels per trailing update) gives LAPACK's pivots."""
     n = 576
@@ -233,10 +233,44 @@ def test_tune_getrf(wm, ctx):
     try:
         best, ms = ctx.tune_getrf("f32", 1024)
         print("tuned", best, ms)
-        assert best in (1, 2, 4, 8) and ms > 0
+        assert best in (1, 2, 4, 8, 16) and ms > 0
         assert ctx.getrf_block("f32", 1024) == best
     finally:
         ctx.set_getrf_block("f32", 1024, 0)
+
+
+def test_tune_fill_and_cache(wm, geometries, ctx, geo_dir, tmp_path):
+    """The fill tuner times 1-4 frequencies per launch and keeps the fastest
+    (the matrices are the same bit for bit whatever it picks); the settings
+    survive a JSON cache written by one context and loaded by another, entries
+    of another architecture are skipped."""
+    p = os.path.join(geo_dir, "tune_grid.geo")
+    geometries.planar_grid(p, 2000, seed=5)
+    g = wm.Geometry.load(ctx, p)
+    freqs = [30.0, 12.0, 4.0, 1.0, 0.3]
+    Z0 = g.fill(freqs)
+    c2 = wm.Context(0)
+    try:
+        best, ms = ctx.tune_fill(g)
+        print("fill tuned", best, ms)
+        assert best in (1, 2, 3, 4) and ms > 0 and ctx.fill_freqs(1, g.ld) == best
+        for f in (1, 3):
+            ctx.set_fill_freqs(1, g.ld, f)
+            Z1 = g.fill(freqs)
+            assert np.array_equal(np.ascontiguousarray(Z1).view(np.uint64), np.ascontiguousarray(Z0).view(np.uint64))
+        ctx.set_fill_freqs(1, g.ld, best)
+        ctx.set_getrf_block("bf16x3", 8192, 16)
+        path = str(tmp_path / "tune.json")
+        assert ctx.tune_save(path) == 2
+        with open(path, "a") as fh:  # another architecture's entry is not applied
+            fh.write('{"arch": "gfx000", "knob": "getrf_block", "prec": 0, "size_class": 8, "value": 1}\n')
+        assert c2.tune_load(path) == 2
+        assert c2.fill_freqs(1, g.ld) == best and c2.getrf_block("bf16x3", 8192) == 16
+        assert c2.getrf_block("f64", 8192) == 8  # default kept
+    finally:
+        ctx.set_fill_freqs(1, g.ld, 0)
+        ctx.set_getrf_block("bf16x3", 8192, 0)
+        c2.close()
 
 
 @pytest.mark.parametrize("prec", ["f64", "bf16x3"])

@@ -92,6 +92,14 @@ def host_info():
             "cpu_model": model}
 
 
+_T0 = time.perf_counter()
+
+
+def progress(stage: str):
+    """One line on stderr per finished stage (the run is minutes long)."""
+    print(f"bench: {stage} done at {time.perf_counter() - _T0:.0f} s", file=sys.stderr, flush=True)
+
+
 def timed(ctx, fn, steps, warmup):
     """Mean wall time of fn over `steps` calls after `warmup` untimed ones, the
     device synchronised on both sides; returns (seconds per call, last result)."""
@@ -251,9 +259,10 @@ def serving_leg(wm, ctx, tmp, count, rate, warm=0, sizes=(512, 1024, 2048, 4096,
 
 
 def config_line(wm, ctx, tmp, name, make, freqs, feeds_of, recycle, cpu, threads, steps=2):
-    """Solves/s of one configuration: the plain FP64 path, FP32 + refinement and
-    the recycled sweep, each timed after one untimed call, with its Zin compared
-    with the FP64 sweep's and the roofline of its dominant kernel."""
+    """Solves/s of one configuration: the plain FP64 path, FP32 and split-BF16x3
+    factors + refinement and the recycled sweep, each timed after one untimed
+    call, with its Zin compared with the FP64 sweep's and the roofline of its
+    dominant kernel."""
     geo = os.path.join(tmp, f"{name}.geo")
     make(geo)
     g = wm.Geometry.load(ctx, geo)
@@ -261,29 +270,31 @@ def config_line(wm, ctx, tmp, name, make, freqs, feeds_of, recycle, cpu, threads
     gaps = feeds_of(t)
     res = {"N": int(g.npls), "nfreq": len(freqs), "nrhs": len(gaps)}
     runs = {}
-    for label, kw in (("f64", dict(prec="f64")), ("f32_ir", dict(prec="f32")),
-                      ("recycled", dict(prec="f64", recycle=recycle))):
+    kinds = {"f64": dict(prec="f64"), "f32_ir": dict(prec="f32"), "bf16x3_ir": dict(prec="bf16x3"),
+             "recycled": dict(prec="f64", recycle=recycle)}
+    for label, kw in kinds.items():
         dt, (z, _, st) = timed(ctx, lambda: wm.solve_sweep(g, freqs, gaps, **kw), steps, 1)
         runs[label] = (z, st, dt)
         res[label] = {"solves_per_s": len(freqs) / dt, "ms_per_sweep": 1e3 * dt}
+        progress(f"{name} {label}")
     _, _, s64 = wm.solve_sweep(g, freqs, gaps, "f64", cond_est=True)
     kappa = [s["cond1"] for s in s64]
     res["kappa1_range"] = [float(min(kappa)), float(max(kappa))]
     z64 = runs["f64"][0]
-    for label in ("f32_ir", "recycled"):
+    for label in ("f32_ir", "bf16x3_ir", "recycled"):
         z, st, dt = runs[label]
         res[label].update(compare(z[:, 0] if z.ndim > 1 else z, z64[:, 0], kappa))
         res[label].update(stats_summary(st))
         res[label]["speedup_vs_fp64"] = runs["f64"][2] / dt
-    best = max(("f64", "f32_ir", "recycled"), key=lambda k: res[k]["solves_per_s"])
-    prof = profiled(ctx, lambda: wm.solve_sweep(g, freqs, gaps, **dict(
-        f64=dict(prec="f64"), f32_ir=dict(prec="f32"), recycled=dict(prec="f64", recycle=recycle))[best]))
+    best = max(kinds, key=lambda k: res[k]["solves_per_s"])
+    prof = profiled(ctx, lambda: wm.solve_sweep(g, freqs, gaps, **kinds[best]))
     res["roofline"] = roofline(prof)
     res["roofline"]["of"] = best
     if cpu:
         res["cpu_baseline"] = cpu_baseline(geo, float(freqs[-1]), threads,
                                            fill_cols=(min(64, g.npls), min(1024, g.npls)),
                                            lu_m=min(4096, g.npls))
+        progress(f"{name} CPU baseline")
     g.close()
     return res
 
@@ -378,6 +389,7 @@ def main():
     if rx:
         rx.roctxRangePop()
     value = a.steps * len(freqs) / dt
+    progress(f"value sweep ({value:.1f} freq-solves/s)")
     if a.profile_only:
         prof = profiled(ctx, lambda: sweep(a.prec, a.recycle))
         out = {"value": value, "unit": "freq-solves/s", "steps": a.steps, "warmup": a.warmup,
@@ -394,6 +406,7 @@ def main():
     fp64 = len(freqs) / dt64
     _, _, s64 = wm.solve_sweep(g, freqs, [gap], "f64", cond_est=True)
     kappa = [s["cond1"] for s in s64]
+    progress("FP64 sweep and condition estimates")
 
     # mixed precision without recycling: FP32 and split-BF16x3 factors + FP64 refinement (ladder)
     mixed = {}
@@ -401,6 +414,7 @@ def main():
         dtm, (zm, _, stm) = timed(ctx, lambda: sweep(prec), 2, 1)
         mixed[prec] = {"solves_per_s": len(freqs) / dtm, "speedup_vs_fp64": dt64 / dtm,
                        **compare(zm[:, 0], z64[:, 0], kappa), **stats_summary(stm)}
+        progress(f"mixed sweep {prec}")
 
     # a band where every frequency refines in complex64 (2.5 - 10 MHz, 32 log-spaced)
     band = None
@@ -410,6 +424,7 @@ def main():
         for prec in ("f64", "f32", "bf16x3"):
             d_, (zb_, _, stb_) = timed(ctx, lambda: sweep(prec, fb=fb), 2, 1)
             runs[prec] = (zb_, stb_, d_)
+            progress(f"refinable band {prec}")
         d_, (zb_, _, stb_) = timed(ctx, lambda: sweep("f64", a.recycle, fb=fb), 2, 1)
         runs["recycled"] = (zb_, stb_, d_)
         z64b, _, dt64b = runs["f64"]
@@ -424,6 +439,7 @@ def main():
     prof = profiled(ctx, lambda: sweep(a.prec, a.recycle))
     roof = roofline(prof)
     prof64 = profiled(ctx, lambda: sweep("f64"))
+    progress("profiled steps")
 
     # whole-sweep roofline fraction: t_ideal / t_measured per freq-solve, each
     # factored frequency's LU (8/3 N^3 real flops) at the peak of its precision,
@@ -452,10 +468,12 @@ def main():
         }
 
     serving = None if a.no_serving else serving_leg(wm, ctx, tmp, a.requests, a.rate)
+    progress("serving leg")
 
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(geo, float(freqs[-1]), threads)
+        progress("CPU baseline")
 
     recyc = a.recycle > 1.0
     out = {

@@ -496,6 +496,7 @@ double panel_launches(int n, int count, int width)
 // 24 N^2 bytes per member at about 5 TB/s; about 6 iterations.
 constexpr double kPanelColF64 = 7.4e-6, kPanelColC64 = 7.3e-6;
 constexpr double kRateF64 = 76e12, kRateF32 = 122e12, kRateB3 = 180e12;
+constexpr double kComplex64Margin = 1.2;
 
 double refine_cost(double n, int count)
 {
@@ -510,7 +511,12 @@ bool prefer_f64(wm_ctx *c, int prec, int n, int count)
     const double t64 = 64 * kPanelColF64 * panel_launches(n, count, w64) + count * flu / kRateF64;
     const double rate = prec == WM_PREC_BF16X3 ? kRateB3 : kRateF32;
     const double tmix = 64 * kPanelColC64 * panel_launches(n, count, w32) + count * flu / rate + refine_cost(nn, count);
-    return t64 <= tmix;
+    // complex64 factors must be predicted clearly cheaper: the first batch of a
+    // structure is routed before any contraction rate is known, a member that
+    // then fails costs an FP64 factorization on top, and the rates above were
+    // measured at n = 8192 (smaller updates run slower); at n = 4096 with 12
+    // frequencies the model's 9 % margin for BF16x3 turned into a 1.9x loss
+    return t64 <= kComplex64Margin * tmix;
 }
 
 bool join_f64(wm_ctx *c, int prec, int n, int r)

@@ -309,6 +309,8 @@ def main():
     ap.add_argument("--recycle", type=float, default=2.0,
                     help="value sweep: anchors within this frequency factor (0: one LU per frequency)")
     ap.add_argument("--nfreq", type=int, default=64)
+    ap.add_argument("--refinable", action="store_true",
+                    help="with --profile-only: sweep the refinable band (32 frequencies 2.5 - 10 MHz) instead")
     ap.add_argument("--batch", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
@@ -355,6 +357,8 @@ def main():
     geo = os.path.join(tmp, "c3.geo")
     G.lightning(geo, 8192)
     freqs = G.freqs_c3()[: a.nfreq]
+    if a.refinable:
+        freqs = np.logspace(np.log10(2.5), 1.0, 32).astype(np.float32)
     g = wm.Geometry.load(ctx, geo, os.path.join(tmp, "configwire.txt"))
     tuning = {"cache": None, "entries_applied": 0}
     if a.tune:

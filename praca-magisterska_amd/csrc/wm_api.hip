@@ -227,6 +227,39 @@ struct DevMem {
 
 }  // namespace
 
+namespace {
+
+// Value of "key": <int> in a JSON line, or false.
+bool json_int(const char *line, const char *key, long *out)
+{
+    char pat[64];
+    std::snprintf(pat, sizeof pat, "\"%s\":", key);
+    const char *p = std::strstr(line, pat);
+    if (!p) return false;
+    p += std::strlen(pat);
+    char *end = nullptr;
+    const long v = std::strtol(p, &end, 10);
+    if (end == p) return false;
+    *out = v;
+    return true;
+}
+
+// Value of "key": "<string>" in a JSON line, or empty.
+std::string json_str(const char *line, const char *key)
+{
+    char pat[64];
+    std::snprintf(pat, sizeof pat, "\"%s\":", key);
+    const char *p = std::strstr(line, pat);
+    if (!p) return std::string();
+    p += std::strlen(pat);
+    while (*p == ' ') ++p;
+    if (*p != '"') return std::string();
+    const char *q = std::strchr(++p, '"');
+    return q ? std::string(p, q) : std::string();
+}
+
+}  // namespace
+
 extern "C" {
 
 const char *wm_last_error(void) { return g_err; }
@@ -807,38 +840,6 @@ int wm_tune_fill(wm_ctx *c, const wm_geom *g, int *best, double *ms_per_freq)
     return WM_OK;
 }
 
-namespace {
-
-// Value of "key": <int> in a JSON line, or false.
-bool json_int(const char *line, const char *key, long *out)
-{
-    char pat[64];
-    std::snprintf(pat, sizeof pat, "\"%s\":", key);
-    const char *p = std::strstr(line, pat);
-    if (!p) return false;
-    p += std::strlen(pat);
-    char *end = nullptr;
-    const long v = std::strtol(p, &end, 10);
-    if (end == p) return false;
-    *out = v;
-    return true;
-}
-
-// Value of "key": "<string>" in a JSON line, or empty.
-std::string json_str(const char *line, const char *key)
-{
-    char pat[64];
-    std::snprintf(pat, sizeof pat, "\"%s\":", key);
-    const char *p = std::strstr(line, pat);
-    if (!p) return std::string();
-    p += std::strlen(pat);
-    while (*p == ' ') ++p;
-    if (*p != '"') return std::string();
-    const char *q = std::strchr(++p, '"');
-    return q ? std::string(p, q) : std::string();
-}
-
-}  // namespace
 
 int wm_tune_save(wm_ctx *c, const char *path)
 {

@@ -19,11 +19,11 @@
 // column store is one coalesced 1 KiB run), each wave walks kColsPerWave
 // source bases n; n is wave-uniform, so the source records arrive through
 // scalar loads.  R and 1/R are shared by every frequency of the launch: the
-// kernel e^{-jkR}/R is taken as (cos kR, -sin kR) * (1/R) with one correctly
-// rounded reciprocal per quadrature point, and the divisions by a segment
-// length as products with its reciprocal, where the reference divides
-// ("wersja referencyjna/Zmatrix.cpp":679-688); the results differ from its
-// quotients by about an ulp and the kernel issues a third fewer FP64 operations.
+// kernel e^{-jkR}/R is taken as (cos kR, -sin kR) * (1/R), 1/R from one
+// reciprocal square root of R^2 per quadrature point and R = R^2 (1/R), and
+// the divisions by a segment length as products with its reciprocal, where
+// the reference takes a square root and divides ("wersja referencyjna/
+// Zmatrix.cpp":679-688); the results differ from its by about an ulp.
 // cos kR and sin kR come from sincos_phase (wm_sincos.h: Cody-Waite reduction
 // and Taylor polynomials, within about 1.5 ulp of the exact values for kR up
 // to 100 and 2.5 ulp up to its limit), which needs a fraction of the
@@ -200,8 +200,8 @@ __global__ __launch_bounds__(256) void k_fill(const double *__restrict__ obs, co
                     for (int qp = 0; qp < 2; ++qp) {
                         const double s = qp == 0 ? half + off : half - off;
                         const double dx = fma(-s, dcx, bx), dy = fma(-s, dcy, by), dz = fma(-s, dcz, bz);
-                        const double R = sqrt(fma(dz, dz, fma(dy, dy, fma(dx, dx, an * an))));
-                        const double rR = 1.0 / R;
+                        const double R2 = fma(dz, dz, fma(dy, dy, fma(dx, dx, an * an)));
+                        const double rR = rsqrt(R2), R = R2 * rR;
                         const double w = (g == 0 ? s : dl - s) * rdl;
 #pragma unroll
                         for (int f = 0; f < F; ++f) {

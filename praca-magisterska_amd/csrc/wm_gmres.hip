@@ -171,26 +171,6 @@ __global__ void k_update_max(double2 *X, const double2 *D, int n, int64_t sV, in
     }
 }
 
-// ||Z||_inf of item b = bidx[member] over its leading np x np block (largest row
-// sum of moduli; one thread per row, columns walked in order, coalesced across
-// the rows of a wave), as the ordered bit pattern of a non-negative double.
-__global__ void k_norminf(const double2 *Z, int64_t sZ, int ld, int np, const int *bidx, unsigned long long *out)
-{
-    const int row = blockIdx.x * blockDim.x + threadIdx.x, mem = blockIdx.y;
-    const int b = bidx[mem];
-    double s = 0.0;
-    if (row < np) {
-        const double2 *z = Z + (size_t)b * sZ + row;
-        for (int j = 0; j < np; ++j) {
-            const double2 v = z[(size_t)j * ld];
-            s += hypot(v.x, v.y);
-        }
-        if (!(s <= DBL_MAX)) s = INFINITY;
-    }
-    for (int off = 32; off >= 1; off >>= 1) s = fmax(s, __shfl_xor(s, off));
-    if ((threadIdx.x & 63) == 0) atomicMax(&out[b], (unsigned long long)__double_as_longlong(s));
-}
-
 // max |V[b][r]| per (item, rhs) as ordered bit patterns (non-finite as +inf)
 __global__ void k_maxabs(const double2 *V, int n, int64_t sV, int nr, const int *bidx, unsigned long long *mx)
 {
@@ -272,7 +252,8 @@ int gmres_ir(wm_ctx *c, const wm_solve_opts &o, const GmresArgs &a, const std::v
     const dim3 blk(256);
     auto gridv = [&](int members) { return dim3((unsigned)((n + 255) / 256), (unsigned)(members * nr)); };
 
-    // X = 0, R = B; ||Z||_inf and ||B||_inf per item for the backward error
+    // X = 0, R = B; ||B||_inf per item for the backward error now, ||Z||_inf
+    // from the first GMRES step's product Z z_0 (the same pass over Z)
     std::vector<double> znorm(cnt, 0.0), bnorm((size_t)cnt * nr, 0.0);
     std::vector<unsigned long long> hm2((size_t)cnt + (size_t)cnt * nr);
     auto bits_to = [](unsigned long long u) { double d; std::memcpy(&d, &u, 8); return d; };
@@ -282,14 +263,10 @@ int gmres_ir(wm_ctx *c, const wm_solve_opts &o, const GmresArgs &a, const std::v
         hipLaunchKernelGGL(k_zero_set, dim3((unsigned)((sV + 255) / 256), na), blk, 0, s, a.X, sV, a.dact);
         hipLaunchKernelGGL(k_copy_set, dim3((unsigned)((sV + 255) / 256), na), blk, 0, s, a.R, a.B, sV, a.dact);
         hipLaunchKernelGGL(k_zero_words64, dim3((unsigned)((hm2.size() + 255) / 256)), blk, 0, s, a.mx2, (int)hm2.size());
-        hipLaunchKernelGGL(k_norminf, dim3((unsigned)((a.np + 255) / 256), na), blk, 0, s, a.Z, a.sZ, n, a.np, a.dact,
-                           a.mx2);
         hipLaunchKernelGGL(k_maxabs, gridv(na), blk, 0, s, a.B, n, sV, nr, a.dact, a.mx2 + cnt);
         WM_TRY(copy_d2h(hm2.data(), a.mx2, hm2.size() * sizeof(unsigned long long), s));
-        for (int b : active) {
-            znorm[b] = bits_to(hm2[b]);
+        for (int b : active)
             for (int r = 0; r < nr; ++r) bnorm[(size_t)b * nr + r] = bits_to(hm2[cnt + (size_t)b * nr + r]);
-        }
     }
     for (int outer = 0; outer < outer_max && !active.empty(); ++outer) {
         const int na = (int)active.size();
@@ -333,7 +310,8 @@ int gmres_ir(wm_ctx *c, const wm_solve_opts &o, const GmresArgs &a, const std::v
             std::vector<int> fl(lst.size());
             for (size_t q = 0; q < lst.size(); ++q) fl[q] = facOf[lst[q]];
             WM_TRY(getrs_grouped(c, a.fprec, n, nr, a.F, n, a.sF, a.perm, n, zk, sV, lst, fl, s, dinv, &slot_of));
-            WM_TRY(residual_batched(c, n, nr, a.Z, n, a.sZ, zk, nullptr, n, sV, w, nl, s, a.dact));
+            WM_TRY(residual_batched(c, n, nr, a.Z, n, a.sZ, zk, nullptr, n, sV, w, nl, s, a.dact,
+                                    outer == 0 && k == 0 ? a.mx2 : nullptr, a.np));
             // classical Gram-Schmidt, twice; the coefficients stay on the device
             double2 *h1 = a.red, *h2 = a.red + (size_t)nl * (m + 1) * nr, *nw = h2 + (size_t)nl * (m + 1) * nr;
             hipLaunchKernelGGL(k_dots, dim3(nl, k + 1, nr), blk, 0, s, Vb, setS, k + 1, w, sV, n, nr, a.dact, h1);
@@ -383,6 +361,10 @@ int gmres_ir(wm_ctx *c, const wm_solve_opts &o, const GmresArgs &a, const std::v
             }
             out.inner_iters += nl;
             for (int q : inner) ++out.iters[active[q]];
+        }
+        if (outer == 0) {  // ||Z||_inf of the items, from the first step's product
+            WM_TRY(copy_d2h(hm2.data(), a.mx2, (size_t)cnt * sizeof(unsigned long long), s));
+            for (int b : active) znorm[b] = bits_to(hm2[b]);
         }
         // correction D = sum_j y_j z_j per member, y from the triangular system R y = g
         std::vector<double2> yv((size_t)na * m * nr, make_double2(0.0, 0.0));

@@ -203,9 +203,13 @@ int getrs_batched(wm_ctx *ctx, int prec, int n, int nrhs, const void *A, int lda
                   const int *ipiv, int64_t strideP, void *B, int ldb, int64_t strideB, int batch,
                   hipStream_t s);
 // R = B - Z X for complex128 Z (n x n), X, B (n x nrhs); batched.  B null: R = Z X.
+// znorm (optional): ||Z||_inf of each matrix over its leading np rows, from the
+// same pass over Z, atomically maxed into znorm[matrix] as ordered bit
+// patterns (the caller zeroes it).
 int residual_batched(wm_ctx *ctx, int n, int nrhs, const void *Z, int ldz, int64_t strideZ,
                      const void *X, const void *B, int ldb, int64_t strideB, void *R, int batch,
-                     hipStream_t s, const int *d_bidx = nullptr);
+                     hipStream_t s, const int *d_bidx = nullptr, unsigned long long *znorm = nullptr,
+                     int np = 0);
 
 int check_device_error(wm_ctx *ctx, hipStream_t s);
 

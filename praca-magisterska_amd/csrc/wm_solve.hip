@@ -13,6 +13,7 @@
 // dispatched before it, so the chain always makes progress; every wait is
 // bounded.  Arithmetic is FP64; complex64 factors are widened on load.
 #include <algorithm>
+#include <cfloat>
 #include <cstring>
 #include <mutex>
 #include <vector>
@@ -338,10 +339,12 @@ __global__ __launch_bounds__(256) void k_trsv(TrsvArgs a, int r_off)
 // row's sum depends on n only.
 typedef double d2v __attribute__((ext_vector_type(2)));
 
-template <int NR>
+// ABS: also the split's partial row sums of |Z| (the infinity norm of Z, taken
+// in the same pass over Z), Pabs[(member * nsplit + split) * n + row].
+template <int NR, bool ABS>
 __global__ __launch_bounds__(256) void k_resid_part(const double2 *Z, int ldz, int64_t strideZ, int n, int nrhs,
                                                     const double2 *X, int ldb, int64_t strideB, int cols_per_split,
-                                                    int nsplit, double2 *P, int r_off, const int *bidx)
+                                                    int nsplit, double2 *P, int r_off, const int *bidx, double *Pabs)
 {
     constexpr int kZB = 8;
     const int bl = blockIdx.z, split = blockIdx.y, lane = threadIdx.x & 63;
@@ -356,6 +359,7 @@ __global__ __launch_bounds__(256) void k_resid_part(const double2 *Z, int ldz, i
     double2 acc[NR];
 #pragma unroll
     for (int r = 0; r < NR; ++r) acc[r] = make_double2(0.0, 0.0);
+    double asum = 0.0;
     const double2 *zp = Zb + row + (size_t)c0 * ldz;
     for (int c = 0; c < cw; c += kZB) {  // cw is a multiple of kZB
         double2 z[kZB];
@@ -368,12 +372,16 @@ __global__ __launch_bounds__(256) void k_resid_part(const double2 *Z, int ldz, i
         for (int e = 0; e < kZB; ++e) {
 #pragma unroll
             for (int r = 0; r < NR; ++r) cfma(acc[r], z[e], Xb[c0 + c + e + (size_t)r * ldb]);
+            if (ABS) asum += hypot(z[e].x, z[e].y);
         }
     }
     __shared__ double2 s[3][64][NR];
-    if (wave > 0)
+    __shared__ double sa[3][64];
+    if (wave > 0) {
 #pragma unroll
         for (int r = 0; r < NR; ++r) s[wave - 1][lane][r] = acc[r];
+        if (ABS) sa[wave - 1][lane] = asum;
+    }
     __syncthreads();
     if (wave == 0) {
 #pragma unroll
@@ -382,17 +390,34 @@ __global__ __launch_bounds__(256) void k_resid_part(const double2 *Z, int ldz, i
             for (int p = 0; p < 3; ++p) { acc[r].x += s[p][lane][r].x; acc[r].y += s[p][lane][r].y; }
             P[(((size_t)bl * nsplit + split) * NR + r) * n + row] = acc[r];
         }
+        if (ABS) {
+#pragma unroll
+            for (int p = 0; p < 3; ++p) asum += sa[p][lane];
+            Pabs[((size_t)bl * nsplit + split) * n + row] = asum;
+        }
     }
 }
 
-// R = B - sum_split P, in split order.
+// R = B - sum_split P, in split order.  With Pabs: the row sums of |Z| (in
+// split order) over the leading np rows, their maximum into znorm[b] as the
+// ordered bit pattern of a non-negative double (non-finite as +inf).
 template <int NR>
 __global__ void k_resid_final(const double2 *P, int nsplit, int n, int nrhs, const double2 *B, int ldb,
-                              int64_t strideB, double2 *R, int r_off, const int *bidx)
+                              int64_t strideB, double2 *R, int r_off, const int *bidx, const double *Pabs, int np,
+                              unsigned long long *znorm)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int bl = blockIdx.y;
     const int b = bidx ? bidx[bl] : bl;
+    if (Pabs) {  // uniform per launch
+        double a = 0.0;
+        if (i < np) {
+            for (int sp = 0; sp < nsplit; ++sp) a += Pabs[((size_t)bl * nsplit + sp) * n + i];
+            if (!(a <= DBL_MAX)) a = INFINITY;
+        }
+        for (int off = 32; off >= 1; off >>= 1) a = fmax(a, __shfl_xor(a, off));
+        if ((threadIdx.x & 63) == 0) atomicMax(&znorm[b], (unsigned long long)__double_as_longlong(a));
+    }
     if (i >= n) return;
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
@@ -777,7 +802,8 @@ int getrs_batched(wm_ctx *ctx, int prec, int n, int nrhs, const void *A, int lda
 }
 
 int residual_batched(wm_ctx *ctx, int n, int nrhs, const void *Z, int ldz, int64_t strideZ, const void *X,
-                     const void *B, int ldb, int64_t strideB, void *R, int batch, hipStream_t s, const int *d_bidx)
+                     const void *B, int ldb, int64_t strideB, void *R, int batch, hipStream_t s, const int *d_bidx,
+                     unsigned long long *znorm, int np)
 {
     if (n % 64 != 0) {
         set_error("residual: n=%d not a multiple of 64", n);
@@ -789,16 +815,26 @@ int residual_batched(wm_ctx *ctx, int n, int nrhs, const void *Z, int ldz, int64
     const int cps = n / nsplit;
     const int nrp = nrhs >= 8 ? 8 : nrhs >= 4 ? 4 : nrhs >= 2 ? 2 : 1;
     DevBuf &partial = ctx->ws_resid;
-    const size_t need = (size_t)batch * nsplit * nrp * n * sizeof(double2);
+    const size_t pbytes = (size_t)batch * nsplit * nrp * n * sizeof(double2);
+    const size_t need = pbytes + (znorm ? (size_t)batch * nsplit * n * sizeof(double) : 0);
     WM_TRY(partial.ensure(need));
+    double *pabs = znorm ? (double *)((char *)partial.ptr + pbytes) : nullptr;
     for (int r0 = 0; r0 < nrhs; r0 += rhs_chunk(nrhs - r0)) {
         const int left = rhs_chunk(nrhs - r0);
         const dim3 g1(n / 64, nsplit, batch), g2((n + 255) / 256, batch);
-#define WM_RESID(NR)                                                                                        \
-    hipLaunchKernelGGL(k_resid_part<NR>, g1, dim3(256), 0, s, (const double2 *)Z, ldz, strideZ, n, nrhs,    \
-                       (const double2 *)X, ldb, strideB, cps, nsplit, (double2 *)partial.ptr, r0, d_bidx);  \
-    hipLaunchKernelGGL(k_resid_final<NR>, g2, dim3(256), 0, s, (const double2 *)partial.ptr, nsplit, n, nrhs, \
-                       (const double2 *)B, ldb, strideB, (double2 *)R, r0, d_bidx)
+        // the norm of Z rides along the first right-hand-side chunk only
+        double *pa = r0 == 0 ? pabs : nullptr;
+#define WM_RESID(NR)                                                                                          \
+    if (pa)                                                                                                   \
+        hipLaunchKernelGGL((k_resid_part<NR, true>), g1, dim3(256), 0, s, (const double2 *)Z, ldz, strideZ, n, \
+                           nrhs, (const double2 *)X, ldb, strideB, cps, nsplit, (double2 *)partial.ptr, r0,     \
+                           d_bidx, pa);                                                                       \
+    else                                                                                                      \
+        hipLaunchKernelGGL((k_resid_part<NR, false>), g1, dim3(256), 0, s, (const double2 *)Z, ldz, strideZ, n, \
+                           nrhs, (const double2 *)X, ldb, strideB, cps, nsplit, (double2 *)partial.ptr, r0,     \
+                           d_bidx, (double *)nullptr);                                                         \
+    hipLaunchKernelGGL(k_resid_final<NR>, g2, dim3(256), 0, s, (const double2 *)partial.ptr, nsplit, n, nrhs,   \
+                       (const double2 *)B, ldb, strideB, (double2 *)R, r0, d_bidx, (const double *)pa, np, znorm)
         prof_begin(ctx, K_RESID, s);
         if (left == 8) { WM_RESID(8); }
         else if (left == 4) { WM_RESID(4); }

@@ -36,7 +36,11 @@ namespace wm {
 
 namespace {
 
-constexpr double kInnerTol0 = 1e-13;  // inner tolerance with X = 0 (relative to ||B||)
+// inner tolerance with X = 0 (relative to ||B||): the acceptance test is the
+// backward error (below), which ||Z|| ||X|| >> ||B|| makes reachable from a
+// looser GMRES residual; 1e-12 against 1e-13: 2 % fewer product steps on the
+// lightning sweep at N = 8192, the same results (1e-11: no further gain)
+constexpr double kInnerTol0 = 1e-12;
 constexpr double kInnerTol = 1e-3;    // inner tolerance of later corrections
 // A result whose normwise backward error ||R|| / (||Z|| ||X|| + ||B||) (infinity
 // norms, R the FP64 residual) is at most this is accepted at once: an LU

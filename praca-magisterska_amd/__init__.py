@@ -479,7 +479,7 @@ def solve_sweep(geom: Geometry, freqs_mhz, gaps, prec="f64", ampl: float = 1.0, 
 
 def solve_sweep_ex(geom: Geometry, freqs_mhz, nrhs: int, sources, loads=(), prec="f64", batch: int = 0,
                    max_iter: int = 20, tol: float = 1e-12, fallback: bool = True, size_policy: bool = True,
-                   plane_waves=()):
+                   plane_waves=(), recycle: float = 0.0, cond_est: bool = False):
     """Sweep with general generators, plane waves and lumped loads (wm_solve_sweep_ex).
 
     sources: (basis, rhs, ampl, phas) per generator; loads: (basis, r_ohm, l_uh, c_uf) per load;
@@ -497,6 +497,7 @@ def solve_sweep_ex(geom: Geometry, freqs_mhz, nrhs: int, sources, loads=(), prec
     o.prec = PREC_NAMES[prec] if isinstance(prec, str) else int(prec)
     o.batch, o.max_iter, o.tol, o.fallback_f64 = batch, max_iter, tol, int(fallback)
     o.size_policy = int(size_policy)
+    o.recycle, o.cond_est = float(recycle), int(cond_est)
     I = np.zeros((nf, nrhs, n), np.complex128)
     zin = np.zeros((nf, len(sources)), np.complex128)
     st = (FreqStats * nf)()

@@ -104,3 +104,29 @@ def test_recycled_result_independent_of_batch(wm, geometries, ctx, geo_dir):
     assert same
     for i in same:
         assert za[i, 0] == zb[i, 0], (f[i], za[i, 0], zb[i, 0])
+
+
+def test_recycled_pec_loads_generators_plane_wave(wm, geometries, ctx, geo_dir):
+    """Recycling with everything else the sweep supports: the PEC ground (image
+    terms; the anchor radius uses the diameter with the images), lumped loads,
+    two generators and a plane wave over three right-hand sides, 10 frequencies
+    0.2 - 20 MHz: every generator's V/I and every current within 1e-10 of the
+    FP64 sweep wherever kappa_1 u64 <= 1e-11, else within kappa_1 u64."""
+    p = os.path.join(geo_dir, "rec_rods.geo")
+    geometries.lightning(p, 1024, depth=0.0, ring=False)
+    g = wm.Geometry.load(ctx, p, ienv=2)
+    f = np.logspace(np.log10(0.2), np.log10(20.0), 10).astype(np.float32)
+    sources = [(40, 0, 1.0, 0.0), (700, 1, 0.5, 45.0)]
+    loads = [(300, 25.0, 0.0, 0.0), (500, 0.0, 0.5, 0.0), (900, 5.0, 0.0, 1e-3)]
+    waves = [(2, 60.0, 30.0, 1.0 + 0.5j, 0.0)]
+    z64, I64, s64 = wm.solve_sweep_ex(g, f, 3, sources, loads, "f64", plane_waves=waves, cond_est=True)
+    zr, Ir, sr = wm.solve_sweep_ex(g, f, 3, sources, loads, "f64", plane_waves=waves, recycle=2.0)
+    ku = np.array([s["cond1"] for s in s64]) * U64
+    bound = np.where(ku <= 1e-11, 1e-10, np.maximum(1e-10, ku))
+    anchors = sum(1 for s in sr if s["anchor_mhz"] > 0 and s["iters"] == 0)
+    assert 1 <= anchors < len(f)
+    for i in range(len(f)):
+        ez = (np.abs(zr[i] - z64[i]) / np.abs(z64[i])).max()
+        eI = (np.abs(Ir[i] - I64[i]).max(axis=1) / np.abs(I64[i]).max(axis=1)).max()
+        print(float(f[i]), sr[i]["anchor_mhz"], sr[i]["iters"], ez, eI, bound[i])
+        assert ez <= bound[i] and eI <= 10 * bound[i], (float(f[i]), ez, eI, bound[i])

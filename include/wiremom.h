@@ -136,6 +136,13 @@ int wm_getrf_batched(wm_ctx *ctx, int prec, int n, void *A, int lda, int64_t str
  * context used while other threads drive the same device should launch
  * directly (the request server's lanes do). */
 int wm_set_graphs(wm_ctx *ctx, int on);
+/* Recycled sweeps (wm_solve_opts.recycle > 1) fill the matrices of the
+ * frequencies refined by GMRES on a second stream of the context while the
+ * anchor frequencies are factored on its own stream (default on; off while
+ * the profiler is enabled, and for sweeps with lumped loads, condition
+ * estimates or structures of several sizes).  Results are identical either
+ * way.  wm_set_fill_overlap(ctx, 0) keeps every launch on one stream. */
+int wm_set_fill_overlap(wm_ctx *ctx, int on);
 /* Tuning (replaces the reference's fixed (8, 8) launch shape of the fill,
  * "wersja referencyjna/Wire_MoM.cu":4110-4113, and its fixed cuBLAS calls).
  *

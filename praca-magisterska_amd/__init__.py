@@ -37,7 +37,7 @@ EXPORTS = [
     "wm_prof_get", "wm_dev_alloc",
     "wm_dev_free", "wm_memcpy_h2d", "wm_memcpy_d2h", "wm_memcpy_d2d", "wm_geom_create", "wm_geom_create_env", "wm_geom_load", "wm_geom_load_env", "wm_geom_destroy",
     "wm_geom_npls", "wm_geom_ld", "wm_zfill", "wm_getrf_batched", "wm_getrs_batched", "wm_set_getrf_block", "wm_get_getrf_block", "wm_tune_getrf", "wm_set_fill_freqs", "wm_get_fill_freqs",
-    "wm_tune_fill", "wm_tune_save", "wm_tune_load", "wm_set_graphs", "wm_solve_opts_default",
+    "wm_tune_fill", "wm_tune_save", "wm_tune_load", "wm_set_graphs", "wm_set_fill_overlap", "wm_solve_opts_default",
     "wm_solve_sweep", "wm_solve_sweep_ex", "wm_transient", "wm_spectrum_dexp", "wm_server_opts_default", "wm_server_create", "wm_server_submit", "wm_server_step",
     "wm_server_result", "wm_server_status", "wm_server_done", "wm_server_destroy", "geoinp", "config", "configwire", "tablice", "zmatrix",
     "zmatrix_gpu", "zmatrix_gpu_rank", "LU_cuBLAS", "CalculateByFreq",
@@ -138,6 +138,7 @@ def lib() -> C.CDLL:
                                        C.c_int, C.c_int64, C.c_int]
         L.wm_set_getrf_block.argtypes = [vp, C.c_int, C.c_int, C.c_int]
         L.wm_set_graphs.argtypes = [vp, C.c_int]
+        L.wm_set_fill_overlap.argtypes = [vp, C.c_int]
         L.wm_get_getrf_block.argtypes = [vp, C.c_int, C.c_int]
         L.wm_tune_getrf.argtypes = [vp, C.c_int, C.c_int, C.c_int, ip, dp]
         L.wm_set_fill_freqs.argtypes = [vp, C.c_int, C.c_int, C.c_int]
@@ -310,6 +311,11 @@ class Context:
     def set_graphs(self, on: bool):
         """Replay factorizations from captured HIP graphs (default) or launch directly."""
         _check(lib().wm_set_graphs(self.h, int(on)))
+
+    def set_fill_overlap(self, on: bool):
+        """Fill the GMRES frequencies of recycled sweeps on a second stream while the anchors are factored
+        (default) or keep every launch on one stream."""
+        _check(lib().wm_set_fill_overlap(self.h, int(on)))
 
     def set_getrf_block(self, prec, n: int, panels: int):
         """Panels per trailing update of the factorization for (prec, size class of n); 0 = default."""

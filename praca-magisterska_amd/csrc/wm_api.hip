@@ -302,9 +302,11 @@ int wm_shutdown(wm_ctx *c)
     if (!c) return WM_OK;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
+    if (c->aux) (void)hipStreamSynchronize(c->aux);
     drop_graphs(c);
     c->ws_info.release();
     c->ws_fill.release();
+    c->ws_fill2.release();
     c->ws_lu.release();
     c->ws_solve.release();
     c->ws_pkt.release();
@@ -317,6 +319,9 @@ int wm_shutdown(wm_ctx *c)
     (void)hipFree(c->d_sync);
     (void)hipFree(c->d_err);
     (void)hipFree(c->d_tag);
+    for (auto e : c->aux_ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->aux) (void)hipStreamDestroy(c->aux);
     (void)hipStreamDestroy(c->stream);
     if (c == g_default_ctx) g_default_ctx = nullptr;
     delete c;
@@ -701,6 +706,14 @@ int wm_set_graphs(wm_ctx *c, int on)
     (void)hipStreamSynchronize(c->stream);
     c->use_graphs = on != 0;
     if (!on) drop_graphs(c);
+    return WM_OK;
+}
+
+int wm_set_fill_overlap(wm_ctx *c, int on)
+{
+    if (!c) return WM_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    c->overlap_fill = on != 0;
     return WM_OK;
 }
 

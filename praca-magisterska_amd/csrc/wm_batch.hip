@@ -730,15 +730,46 @@ int solve_items(wm_ctx *c, const wm_solve_opts &o, WorkItem *items, int count, h
         for (int b = 0; b < count; ++b) hn[b] = items[b].g->ld;
         WM_TRY(copy_h2d(dnmem, hn.data(), count * sizeof(int), s));
     }
+    // Recycled sweeps fill the frequencies that GMRES will solve on a second
+    // stream, so their fills run while the anchors are factored (whose panels
+    // leave most CUs idle); the first GMRES step waits for them.  The profiler
+    // keeps every launch on one stream, so each kernel's time is its own.
+    const bool overlap = recycle && !uneven && nl == 0 && !o.cond_est && c->overlap_fill && !c->prof.on;
+    auto is_own = [&](int b) { return items[b].anchor < 0 || items[b].anchor == b; };
+    if (overlap) {
+        if (!c->aux) {
+            WM_HIP_TRY(hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking));
+            for (auto &e : c->aux_ev) WM_HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        }
+        WM_HIP_TRY(hipEventRecord(c->aux_ev[0], s));  // after everything already queued
+        WM_HIP_TRY(hipStreamWaitEvent(c->aux, c->aux_ev[0], 0));
+    }
+    // the launch stream waits for the second one before the matrices are used,
+    // and on every early return, so no later call overlaps fills still running
+    struct AuxJoin {
+        hipStream_t s = nullptr, aux = nullptr;
+        hipEvent_t e = nullptr;
+        int join()
+        {
+            if (!aux) return WM_OK;
+            const hipError_t r1 = hipEventRecord(e, aux), r2 = hipStreamWaitEvent(s, e, 0);
+            aux = nullptr;
+            return r1 == hipSuccess && r2 == hipSuccess ? WM_OK : WM_ERR_HIP;
+        }
+        ~AuxJoin() { (void)join(); }
+    } aux_join;
+    if (overlap) aux_join = AuxJoin{s, c->aux, c->aux_ev[1]};
     for (int b0 = 0; b0 < count;) {
         int b1 = b0 + 1;
-        while (b1 < count && items[b1].g == items[b0].g) ++b1;
+        while (b1 < count && items[b1].g == items[b0].g && (!overlap || is_own(b1) == is_own(b0))) ++b1;
         const wm_geom *g = items[b0].g;
         std::vector<double> ka(b1 - b0);
         for (int b = b0; b < b1; ++b) ka[b - b0] = wavenumber_mhz(items[b].freq);
         if (g->ld == n) {
+            const bool side = overlap && !is_own(b0);
             WM_TRY(fill_launch(c, g, b1 - b0, ka.data(), Z + (size_t)b0 * sZ, sZ,
-                               mixed ? (void *)(Z32 + (size_t)b0 * sZ) : nullptr, sZ, s));
+                               mixed ? (void *)(Z32 + (size_t)b0 * sZ) : nullptr, sZ, side ? c->aux : s,
+                               side ? &c->ws_fill2 : nullptr));
         } else {
             const int lg = g->ld;
             const int64_t sT = (int64_t)lg * lg;
@@ -799,6 +830,7 @@ int solve_items(wm_ctx *c, const wm_solve_opts &o, WorkItem *items, int count, h
         };
         if (!own.empty()) WM_TRY(factor_solve(own));
         for (int b : own) items[b].anchor_used = b;
+        WM_TRY(aux_join.join());  // the other items' matrices are filled
         // the others: FP64 GMRES refinement preconditioned by the anchor's factors
         // (an anchor with a zero pivot preconditions nothing: its dependents are factored)
         std::vector<int> gdeps, gfac, redo;
@@ -1040,3 +1072,4 @@ extern "C" int wmx_refine_decide(int nrhs, const double *dm, const double *xm, d
     if (nrhs <= 0 || !dm || !xm || !prev || !first || !rate || !worst) return -100;
     return wm::refine_decide(nrhs, dm, xm, prev, it, tol, max_iter, *first, *rate, *worst);
 }
+

@@ -272,7 +272,7 @@ void launch_fill_f(dim3 grid, hipStream_t s, const wm_geom *g, const double2 *se
 }  // namespace
 
 int fill_launch(wm_ctx *ctx, const wm_geom *g, int nf, const double *ka, void *Z, int64_t strideZ,
-                void *Z32, int64_t strideZ32, hipStream_t s)
+                void *Z32, int64_t strideZ32, hipStream_t s, DevBuf *self_ws)
 {
     if (!g || nf <= 0 || !ka || !Z) {
         set_error("fill: bad arguments");
@@ -285,7 +285,7 @@ int fill_launch(wm_ctx *ctx, const wm_geom *g, int nf, const double *ka, void *Z
     }
     // self-term workspace: kMaxF * ld * 4 complex128
     const size_t self_bytes = (size_t)kMaxF * ld * 4 * sizeof(double2);
-    DevBuf &ws = ctx->ws_fill;
+    DevBuf &ws = self_ws ? *self_ws : ctx->ws_fill;
     WM_TRY(ws.ensure(self_bytes));
     double2 *self = (double2 *)ws.ptr;
     const dim3 grid(ld / 64, ld / (kWaves * kColsPerWave));

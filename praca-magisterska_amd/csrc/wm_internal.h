@@ -112,6 +112,15 @@ struct wm_ctx {
     // contexts that may run panel launches on the device at the same time (the
     // request server's lanes): each takes this share of the resident workgroups
     int panel_share = 1;
+    // recycled sweeps: the dependents' matrices are filled on this second
+    // stream (own self-term workspace) while the anchors are factored on
+    // `stream`; created on first use
+    // second stream of recycled sweeps (wm_set_fill_overlap): the GMRES items'
+    // fills, with their own self-term workspace; created on first use
+    bool overlap_fill = true;
+    hipStream_t aux = nullptr;
+    hipEvent_t aux_ev[2] = {nullptr, nullptr};  // launch stream -> aux, aux -> launch stream
+    wm::DevBuf ws_fill2;
 };
 
 // Geometry as the fill kernel reads it.  Observation data per basis (SoA,
@@ -149,7 +158,8 @@ namespace wm {
 
 // Fill: nf frequencies, complex128 output (+ optional complex64 copy).
 int fill_launch(wm_ctx *ctx, const wm_geom *g, int nf, const double *ka, void *Z, int64_t strideZ,
-                void *Z32, int64_t strideZ32, hipStream_t s);
+                void *Z32, int64_t strideZ32, hipStream_t s,
+                DevBuf *self_ws = nullptr);
 
 // LU: batched blocked right-looking factorization with partial pivoting.
 // d_bidx (device, optional): member i of the launch works on matrix d_bidx[i]

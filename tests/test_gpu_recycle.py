@@ -106,6 +106,28 @@ def test_recycled_result_independent_of_batch(wm, geometries, ctx, geo_dir):
         assert za[i, 0] == zb[i, 0], (f[i], za[i, 0], zb[i, 0])
 
 
+def test_recycled_fill_on_second_stream_bit_identical(wm, geometries, ctx, geo_dir):
+    """The GMRES items' matrices are filled on a second stream while the anchors
+    are factored; the result is the same bit for bit as with every fill on the
+    launch stream, over three sweeps in a row (the streams' order holds across
+    calls), in free space and over the PEC ground."""
+    rods = os.path.join(geo_dir, "rec_rods.geo")
+    geometries.lightning(rods, 1024, depth=0.0, ring=False)
+    f = np.logspace(-2, 1, 16).astype(np.float32)
+    try:
+        for p, ienv in ((_light(geometries, geo_dir, 1024), 1), (rods, 2)):
+            g = wm.Geometry.load(ctx, p, ienv=ienv)
+            gap = [40]
+            ctx.set_fill_overlap(False)
+            z0, I0, _ = wm.solve_sweep(g, f, gap, "f64", recycle=2.0)
+            ctx.set_fill_overlap(True)
+            for _ in range(3):
+                z1, I1, _ = wm.solve_sweep(g, f, gap, "f64", recycle=2.0)
+                assert np.array_equal(z0, z1) and np.array_equal(I0, I1)
+    finally:
+        ctx.set_fill_overlap(True)
+
+
 def test_recycled_pec_loads_generators_plane_wave(wm, geometries, ctx, geo_dir):
     """Recycling with everything else the sweep supports: the PEC ground (image
     terms; the anchor radius uses the diameter with the images), lumped loads,

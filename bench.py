@@ -306,7 +306,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--prec", default="f64", help="factor precision of the value sweep (f64 with --recycle: "
                                                    "FP64 anchor factors + FP64 GMRES refinement)")
-    ap.add_argument("--recycle", type=float, default=2.0,
+    ap.add_argument("--recycle", type=float, default=2.5,
                     help="value sweep: anchors within this frequency factor (0: one LU per frequency)")
     ap.add_argument("--nfreq", type=int, default=64)
     ap.add_argument("--refinable", action="store_true",
@@ -318,6 +318,8 @@ def main():
     ap.add_argument("--no-band", action="store_true", help="skip the 2.5 - 10 MHz band where every frequency refines")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch factorizations directly instead of replaying captured HIP graphs")
+    ap.add_argument("--no-overlap", action="store_true",
+                    help="recycled sweeps: fill on the launch stream only (every kernel's time its own)")
     ap.add_argument("--serving-spec", action="store_true",
                     help="only the serving stream at its specification: 1000 requests after 50 warm-up at "
                          "0.8 x the measured capacity")
@@ -345,6 +347,8 @@ def main():
     ctx = wm.Context(0)
     if a.no_graphs:
         ctx.set_graphs(False)
+    if a.no_overlap:
+        ctx.set_fill_overlap(False)
     name, ncu, hbm = ctx.info()
     if not name.split("(")[0].strip() and "gfx950" in name:
         name = "AMD Instinct MI355X " + name.strip()
@@ -397,7 +401,7 @@ def main():
     if a.profile_only:
         prof = profiled(ctx, lambda: sweep(a.prec, a.recycle))
         out = {"value": value, "unit": "freq-solves/s", "steps": a.steps, "warmup": a.warmup,
-               "recycle": a.recycle, "prec": a.prec, "hip_graphs": not a.no_graphs, "tuning": tuning,
+               "recycle": a.recycle, "prec": a.prec, "hip_graphs": not a.no_graphs, "fill_overlap": not a.no_overlap, "tuning": tuning,
                "roofline": roofline(prof), "kernel_ms_profiled_step": {k: round(v[0], 3) for k, v in prof.items()}}
         if a.calibrate_copy:
             out["calibrate_copy_bytes"] = calibrate_copy(wm, ctx, a.calibrate_copy)
@@ -514,7 +518,7 @@ def main():
         "cpu_baseline": cpu,
         "reference_best_published_solves_per_s": REF_BEST_PUBLISHED,
         "device": f"{name} x{ncu} CU",
-        "hip_graphs": not a.no_graphs,
+        "hip_graphs": not a.no_graphs, "fill_overlap": not a.no_overlap,
         "tuning": tuning,
         "timing": "value and every solves/s figure: wall time of whole sweeps, profiler off; per-kernel times "
                   "and rooflines from one separate profiled step",

@@ -206,9 +206,10 @@ typedef struct {
                               with the final factors, about ten extra triangular solves per batch */
     double recycle;        /* recycled sweep, default 0 (off).  A value r > 1 factors (in FP64) only
                               "anchor" frequencies, chosen so that every frequency f of the sweep has
-                              an anchor within a factor r(f) of it -- r while the structure is
-                              electrically small (k D <= 1, D its diameter), 1 + (r - 1) / (k D)
-                              above -- and solves each other frequency by FP64 GMRES iterative
+                              an anchor within a factor r(f) of it -- r^(1 + log10(1 / (k D))) while
+                              the structure is electrically small (k D <= 1, D its diameter: one more
+                              power of r per decade below k D = 1), 1 + (r - 1) / (k D) above -- and
+                              solves each other frequency by FP64 GMRES iterative
                               refinement preconditioned with its anchor's factors, until the
                               correction is below tol or stops shrinking at the FP64 floor; a frequency
                               whose refinement does not converge is factored itself.  prec is

@@ -975,15 +975,19 @@ int run_sweep_recycled(wm_ctx *c, const wm_geom *g, const wm_solve_opts &o, std:
         batch = (int)std::max<size_t>(1, (size_t)(frac * (double)(freeb + c->ws_sweep.bytes)) / per);
         batch = std::min(batch, 128);
     }
-    // covering factor at f: o.recycle while the structure is electrically small
-    // (k D <= 1, D its diameter), shrinking as 1 + (r - 1) / (k D) above, where
-    // the resonances between an anchor and its neighbours lengthen GMRES; it
-    // does not grow with f, so a frequency between an anchor and the highest
-    // frequency it covers is covered as well
+    // covering factor at f, r = o.recycle: while the structure is electrically
+    // small (k D <= 1, D its diameter) r^(1 + log10(1 / (k D))), i.e. one more
+    // power of r per decade below k D = 1 (quasi-static: Z(f) ~ jwL + P/(jw),
+    // whose two parts the anchor's factors capture at any distance, so GMRES
+    // converges in a few steps however far the anchor); above, 1 + (r - 1) / (k D),
+    // where the resonances between an anchor and its neighbours lengthen GMRES.
+    // It does not grow with f, so a frequency between an anchor and the highest
+    // frequency it covers is covered as well.
     const double r = o.recycle;
     auto radius = [&](float f) {
         const double kd = wavenumber_mhz(f) * g->diam;
-        return kd <= 1.0 ? r : 1.0 + (r - 1.0) / kd;
+        if (kd > 1.0) return 1.0 + (r - 1.0) / kd;
+        return std::pow(r, 1.0 - std::log10(std::max(kd, 1e-12)));
     };
     int status = WM_OK;
     for (int f0 = 0; f0 < nfreq;) {

@@ -1073,3 +1073,11 @@ extern "C" int wmx_refine_decide(int nrhs, const double *dm, const double *xm, d
     return wm::refine_decide(nrhs, dm, xm, prev, it, tol, max_iter, *first, *rate, *worst);
 }
 
+// Internal hook: right-hand sides per launch member of the GMRES
+// preconditioner's grouped solves (1, 2, 4 or 8).
+extern "C" int wmx_set_group_cols(wm_ctx *ctx, int w)
+{
+    if (!ctx || (w != 1 && w != 2 && w != 4 && w != 8)) return WM_ERR_ARG;
+    ctx->group_cols = w;
+    return WM_OK;
+}

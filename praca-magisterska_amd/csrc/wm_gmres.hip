@@ -313,7 +313,8 @@ int gmres_ir(wm_ctx *c, const wm_solve_opts &o, const GmresArgs &a, const std::v
             hipLaunchKernelGGL(k_copy_set, dim3((unsigned)((sV + 255) / 256), nl), blk, 0, s, zk, vk, sV, a.dact);
             std::vector<int> fl(lst.size());
             for (size_t q = 0; q < lst.size(); ++q) fl[q] = facOf[lst[q]];
-            WM_TRY(getrs_grouped(c, a.fprec, n, nr, a.F, n, a.sF, a.perm, n, zk, sV, lst, fl, s, dinv, &slot_of));
+            WM_TRY(getrs_grouped(c, a.fprec, n, nr, a.F, n, a.sF, a.perm, n, zk, sV, lst, fl, s, dinv, &slot_of,
+                                 c->group_cols));
             WM_TRY(residual_batched(c, n, nr, a.Z, n, a.sZ, zk, nullptr, n, sV, w, nl, s, a.dact,
                                     outer == 0 && k == 0 ? a.mx2 : nullptr, a.np));
             // classical Gram-Schmidt, twice; the coefficients stay on the device

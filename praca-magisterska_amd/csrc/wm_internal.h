@@ -118,6 +118,7 @@ struct wm_ctx {
     // second stream of recycled sweeps (wm_set_fill_overlap): the GMRES items'
     // fills, with their own self-term workspace; created on first use
     bool overlap_fill = true;
+    int group_cols = 4;  // preconditioner solves: right-hand sides per launch member (1, 2, 4 or 8)
     hipStream_t aux = nullptr;
     hipEvent_t aux_ev[2] = {nullptr, nullptr};  // launch stream -> aux, aux -> launch stream
     wm::DevBuf ws_fill2;

@@ -54,6 +54,11 @@ CLASSES = {
 # HBM bytes / algorithmic bytes of k_gemm_f64, from FETCH_SIZE + WRITE_SIZE passes
 # calibrated on a plain copy (profiles/gemm_f64_traffic_c3_n8192.md)
 GEMM_F64_HBM_RATIO = 1.91
+# the same for the FP64 matrix products of GMRES (k_resid_part), over every launch
+# of the value sweep (profiles/resid_traffic_c3_n8192.md)
+RESID_HBM_RATIO = 1.00
+HBM_RATIO = {"gemm": (GEMM_F64_HBM_RATIO, "profiles/gemm_f64_traffic_c3_n8192.md"),
+             "resid": (RESID_HBM_RATIO, "profiles/resid_traffic_c3_n8192.md")}
 # tuning cache written by `bench.py --tune` on the benchmark's device (JSON lines, wm_tune_save)
 TUNE_CACHE = os.path.join(ROOT, "praca-magisterska_amd", "tune_cache.json")
 REF_BEST_PUBLISHED = 0.148  # freq-solves/s at N=8192, reference's best variant (GPU fill + MKL LU, 6 procs)
@@ -145,11 +150,12 @@ def roofline(prof):
             # (6 flops); the same update at the standard 8 flops is reported beside
             r["flops_per_complex_madd"] = 6
             r["complex_equivalent_achieved"] = ach * 8.0 / 6.0
-    traffic = GEMM_F64_HBM_RATIO * by / max(nl, 1) if dom == "gemm" else None
+    ratio, src = HBM_RATIO.get(dom, (None, None))
+    traffic = ratio * by / max(nl, 1) if ratio else None
     r.update({"kernel": dom, "launches": nl, "avg_launch_us": 1e3 * ms / max(nl, 1), "traffic": traffic,
               "algorithmic_bytes_per_launch": by / max(nl, 1), "algorithmic_flops_per_launch": fl / max(nl, 1)})
     if traffic is not None:
-        r["traffic_source"] = "rocprofv3 FETCH_SIZE+WRITE_SIZE ratio, profiles/gemm_f64_traffic_c3_n8192.md"
+        r["traffic_source"] = "rocprofv3 FETCH_SIZE+WRITE_SIZE ratio, " + src
     return r
 
 

@@ -747,8 +747,8 @@ int solve_items(wm_ctx *c, const wm_solve_opts &o, WorkItem *items, int count, h
     // the launch stream waits for the second one before the matrices are used,
     // and on every early return, so no later call overlaps fills still running
     struct AuxJoin {
-        hipStream_t s = nullptr, aux = nullptr;
-        hipEvent_t e = nullptr;
+        hipStream_t s, aux;  // aux null: nothing to join
+        hipEvent_t e;
         int join()
         {
             if (!aux) return WM_OK;
@@ -757,8 +757,7 @@ int solve_items(wm_ctx *c, const wm_solve_opts &o, WorkItem *items, int count, h
             return r1 == hipSuccess && r2 == hipSuccess ? WM_OK : WM_ERR_HIP;
         }
         ~AuxJoin() { (void)join(); }
-    } aux_join;
-    if (overlap) aux_join = AuxJoin{s, c->aux, c->aux_ev[1]};
+    } aux_join{s, overlap ? c->aux : nullptr, c->aux_ev[1]};
     for (int b0 = 0; b0 < count;) {
         int b1 = b0 + 1;
         while (b1 < count && items[b1].g == items[b0].g && (!overlap || is_own(b1) == is_own(b0))) ++b1;

@@ -737,9 +737,10 @@ int solve_items(wm_ctx *c, const wm_solve_opts &o, WorkItem *items, int count, h
     const bool overlap = recycle && !uneven && nl == 0 && !o.cond_est && c->overlap_fill && !c->prof.on;
     auto is_own = [&](int b) { return items[b].anchor < 0 || items[b].anchor == b; };
     if (overlap) {
-        if (!c->aux) {
+        if (!c->aux) {  // events first, the stream last: c->aux set means all exist
+            for (auto &e : c->aux_ev)
+                if (!e) WM_HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
             WM_HIP_TRY(hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking));
-            for (auto &e : c->aux_ev) WM_HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         }
         WM_HIP_TRY(hipEventRecord(c->aux_ev[0], s));  // after everything already queued
         WM_HIP_TRY(hipStreamWaitEvent(c->aux, c->aux_ev[0], 0));

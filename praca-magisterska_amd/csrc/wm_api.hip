@@ -957,11 +957,36 @@ namespace {
 // routing (see below).
 constexpr int kBlindBatch = 16;  // first batch of a mixed sweep (the top quarter of a 64-point sweep)
 
+// Anchors of a recycled batch: f (MHz, descending) gets anchor[i], the index of
+// the frequency whose factors precondition it (itself for an anchor).  The
+// covering factor at f, for r: while the structure is electrically small
+// (k D <= 1, D its diameter) r^(1 + log10(1 / (k D))), i.e. one more power of
+// r per decade below k D = 1 (quasi-static: Z(f) ~ jwL + P/(jw), whose two
+// parts the anchor's factors capture at any distance, so GMRES converges in a
+// few steps however far the anchor); above, 1 + (r - 1) / (k D), where the
+// resonances between an anchor and its neighbours lengthen GMRES.  It does not
+// grow with f, so a frequency between an anchor and the highest frequency it
+// covers is covered as well.  Greedy interval cover in log f: the anchor of
+// the highest uncovered frequency is the lowest one within its factor, and it
+// covers every frequency down to its own factor below.
+void choose_anchors(const float *f, int cnt, double r, double diam, int *anchor)
+{
+    auto radius = [&](float x) {
+        const double kd = wavenumber_mhz(x) * diam;
+        if (kd > 1.0) return 1.0 + (r - 1.0) / kd;
+        return std::pow(r, 1.0 - std::log10(std::max(kd, 1e-12)));
+    };
+    for (int i = 0; i < cnt;) {
+        int a = i;
+        while (a + 1 < cnt && (double)f[i] <= radius(f[i]) * (double)f[a + 1]) ++a;
+        int k = i;
+        while (k < cnt && (double)f[a] <= radius(f[a]) * (double)f[k]) anchor[k++] = a;
+        i = k;
+    }
+}
+
 // Recycled sweep: frequencies highest first in batches as wide as the HBM
-// budget allows; in each batch the fewest anchors such that every frequency
-// lies within a factor o.recycle of its anchor (greedy interval cover in
-// log f: the anchor of the highest uncovered frequency is the lowest one
-// within the factor, and it covers every frequency down to a factor below).
+// budget allows; in each batch the anchors of choose_anchors.
 int run_sweep_recycled(wm_ctx *c, const wm_geom *g, const wm_solve_opts &o, std::vector<WorkItem> &items, int nrhs)
 {
     const int nfreq = (int)items.size();
@@ -975,31 +1000,15 @@ int run_sweep_recycled(wm_ctx *c, const wm_geom *g, const wm_solve_opts &o, std:
         batch = (int)std::max<size_t>(1, (size_t)(frac * (double)(freeb + c->ws_sweep.bytes)) / per);
         batch = std::min(batch, 128);
     }
-    // covering factor at f, r = o.recycle: while the structure is electrically
-    // small (k D <= 1, D its diameter) r^(1 + log10(1 / (k D))), i.e. one more
-    // power of r per decade below k D = 1 (quasi-static: Z(f) ~ jwL + P/(jw),
-    // whose two parts the anchor's factors capture at any distance, so GMRES
-    // converges in a few steps however far the anchor); above, 1 + (r - 1) / (k D),
-    // where the resonances between an anchor and its neighbours lengthen GMRES.
-    // It does not grow with f, so a frequency between an anchor and the highest
-    // frequency it covers is covered as well.
-    const double r = o.recycle;
-    auto radius = [&](float f) {
-        const double kd = wavenumber_mhz(f) * g->diam;
-        if (kd > 1.0) return 1.0 + (r - 1.0) / kd;
-        return std::pow(r, 1.0 - std::log10(std::max(kd, 1e-12)));
-    };
     int status = WM_OK;
     for (int f0 = 0; f0 < nfreq;) {
         const int cnt = std::min(batch, nfreq - f0);
         WorkItem *it = items.data() + f0;
-        for (int i = 0; i < cnt;) {
-            int a = i;
-            while (a + 1 < cnt && (double)it[i].freq <= radius(it[i].freq) * (double)it[a + 1].freq) ++a;
-            int k = i;
-            while (k < cnt && (double)it[a].freq <= radius(it[a].freq) * (double)it[k].freq) it[k++].anchor = a;
-            i = k;
-        }
+        std::vector<float> fr(cnt);
+        std::vector<int> anc(cnt);
+        for (int i = 0; i < cnt; ++i) fr[i] = it[i].freq;
+        choose_anchors(fr.data(), cnt, o.recycle, g->diam, anc.data());
+        for (int i = 0; i < cnt; ++i) it[i].anchor = anc[i];
         const int rc = solve_items(c, o, it, cnt, c->stream);
         if (rc < 0) return rc;
         if (rc != WM_OK && status == WM_OK) status = rc;
@@ -1070,6 +1079,17 @@ int run_sweep(wm_ctx *c, const wm_geom *g, const wm_solve_opts &o, std::vector<W
 }
 
 }  // namespace
+
+// Internal test hook (not part of include/wiremom.h): the anchors of a recycled
+// batch (choose_anchors) for frequencies in descending order, without a device.
+extern "C" int wmx_choose_anchors(const float *f, int cnt, double r, double diam, int *anchor)
+{
+    if (!f || !anchor || cnt <= 0 || !(r > 1.0) || !(diam >= 0.0)) return WM_ERR_ARG;
+    for (int i = 1; i < cnt; ++i)
+        if (!(f[i] <= f[i - 1])) return WM_ERR_ARG;
+    choose_anchors(f, cnt, r, diam, anchor);
+    return WM_OK;
+}
 
 int wm_solve_sweep(wm_ctx *c, const wm_geom *g, const wm_solve_opts *opts, int nfreq, const float *freq_mhz, int nrhs,
                    const int *gaps, float ampl, float phas, wm_c128 *I, wm_c128 *zin, wm_freq_stats *st)

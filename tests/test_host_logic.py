@@ -5,6 +5,10 @@ Refinement convergence (wm::refine_decide, via wmx_refine_decide): after each
 iteration the host reads max|dx| and max|x| per right-hand side (the device
 reduction reports non-finite values as +inf) and decides to continue, stop
 converged or stop failed; a failed item is refactored in FP64.
+
+Anchors of recycled sweeps (choose_anchors, via wmx_choose_anchors): closed
+forms of the covering factor and a brute-force check that the greedy cover is
+valid and uses the fewest anchors.
 """
 import ctypes as C
 import math
@@ -177,3 +181,77 @@ int main() {
     for l in out[5:]:
         _, q, s, c, sr, cr = l.split()
         assert abs(float(s) - float(sr)) <= 1e-15 and abs(float(c) - float(cr)) <= 1e-15, l
+
+
+# Anchors of recycled sweeps (choose_anchors, via wmx_choose_anchors): the
+# covering factor at f is r^(1 + log10(1/(kD))) for kD <= 1 and 1 + (r-1)/(kD)
+# above (k = 2 pi f / c, f in MHz as float32, c = 2.99792448e8).
+
+def _anchors(wm, f, r, diam):
+    L = wm.lib()
+    h = L.wmx_choose_anchors
+    h.argtypes = [C.POINTER(C.c_float), C.c_int, C.c_double, C.c_double, C.POINTER(C.c_int)]
+    h.restype = C.c_int
+    n = len(f)
+    fa = (C.c_float * n)(*f)
+    out = (C.c_int * n)()
+    assert h(fa, n, r, diam, out) == 0
+    return [out[i] for i in range(n)], [fa[i] for i in range(n)]
+
+
+def _kd(f_mhz, diam):
+    return 2.0 * math.pi * f_mhz * 1e6 / 2.99792448e8 * diam
+
+
+def _covers(fa, fk, r, diam):
+    """Frequency fk may take the anchor fa: the ratio is within the covering
+    factor at the higher of the two (the factor does not grow with f)."""
+    hi = max(fa, fk)
+    kd = _kd(hi, diam)
+    fac = 1.0 + (r - 1.0) / kd if kd > 1.0 else r ** (1.0 - math.log10(kd))
+    return max(fa / fk, fk / fa) <= fac * (1 + 1e-12)
+
+
+def test_anchor_cover_special_cases(wm):
+    """Closed forms of the covering factor: at kD = 1 exactly r, a decade below
+    r^2, two decades below r^3, at kD = 2 the factor 1 + (r-1)/2; probed with two
+    frequencies a hair inside and outside each factor."""
+    r, diam = 2.5, 10.0
+    f_kd1 = 1.0 / _kd(1.0, diam)  # MHz at kD = 1
+    for kd_hi, fac in ((1.0, r), (0.1, r * r), (0.01, r ** 3), (2.0, 1.0 + (r - 1.0) / 2.0)):
+        hi = kd_hi * f_kd1
+        for scale, same in ((0.999, True), (1.001, False)):
+            lo = hi / (fac * scale)
+            anc, ff = _anchors(wm, [hi, lo], r, diam)
+            # float32 frequencies: compare on the values the library saw
+            expect = max(ff) / min(ff) <= fac * (1 + 1e-12)
+            assert expect == same, (kd_hi, scale)
+            assert (anc[1] == anc[0]) == same, (kd_hi, scale, anc)
+
+
+def test_anchor_cover_is_valid_and_minimal(wm):
+    """Every frequency lies within the covering factor of its anchor, anchors
+    anchor themselves and their groups are contiguous; on small random sets the
+    number of anchors equals the brute-force minimum of any cover."""
+    import itertools
+    import random
+    rng = random.Random(3)
+    for trial in range(60):
+        n = rng.randint(1, 10)
+        f = sorted((10 ** rng.uniform(-2, 1.5) for _ in range(n)), reverse=True)
+        r = rng.choice([1.5, 2.0, 2.5, 3.0])
+        diam = rng.choice([5.0, 25.0, 80.0])
+        anc, ff = _anchors(wm, f, r, diam)
+        for i, a in enumerate(anc):
+            assert anc[a] == a, (trial, anc)
+            assert _covers(ff[a], ff[i], r, diam), (trial, i, a, ff)
+        assert all(anc[i] <= anc[i + 1] for i in range(n - 1))
+        best = None
+        for k in range(1, n + 1):
+            for sub in itertools.combinations(range(n), k):
+                if all(any(_covers(ff[a], ff[i], r, diam) for a in sub) for i in range(n)):
+                    best = k
+                    break
+            if best:
+                break
+        assert len(set(anc)) == best, (trial, anc, best, ff, r, diam)

@@ -239,7 +239,7 @@ def test_grouped_solve_with_inverted_diagonal_blocks(wm, ctx, prec):
 def test_c3_full_size_sweeps_match_fp64(wm, geometries, ctx, geo_dir):
     """The benchmark workload itself: the lightning structure at N = 8192, all 64
     frequencies 10 kHz - 10 MHz.  The FP32 and split-BF16x3 sweeps (with the
-    ladder) and the recycled sweep (FP64 anchors + FP64 GMRES refinement) against
+    ladder) and the recycled sweep as the benchmark runs it (r = 2.5) against
     the FP64 sweep, Zin element by element: within 1e-10 where kappa_1 u64 <=
     1e-11 (no frequency of this band), else within kappa_1 u64 (kappa_1 from the
     FP64 sweep's estimate); the recycled anchors equal the FP64 sweep bit for bit."""
@@ -252,7 +252,7 @@ def test_c3_full_size_sweeps_match_fp64(wm, geometries, ctx, geo_dir):
     z64, _, s64 = wm.solve_sweep(g, f, [gap], "f64", cond_est=True)
     ku = np.array([s["cond1"] for s in s64]) * 2.0 ** -53
     bound = np.where(ku <= 1e-11, 1e-10, np.maximum(1e-10, ku))
-    for prec, kw in (("f32", {}), ("bf16x3", {}), ("f64", {"recycle": 2.0})):
+    for prec, kw in (("f32", {}), ("bf16x3", {}), ("f64", {"recycle": 2.5})):
         z, _, st = wm.solve_sweep(g, f, [gap], prec, **kw)
         rel = np.abs(z[:, 0] - z64[:, 0]) / np.abs(z64[:, 0])
         print(prec, kw, "max rel", rel.max(), "max rel / bound", (rel / bound).max())

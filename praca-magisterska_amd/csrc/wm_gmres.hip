@@ -249,11 +249,16 @@ int gmres_ir(wm_ctx *c, const wm_solve_opts &o, const GmresArgs &a, const std::v
     double2 *dinv = (double2 *)c->ws_dinv.ptr;
     WM_TRY(diag_inv_batched(c, a.fprec, n, a.F, n, a.sF, facs, dinv, s));
 
+    std::vector<int> uploaded;  // the list a.dact (and a.dfac) hold
     auto upload_idx = [&](const std::vector<int> &lst) -> int {
+        if (lst == uploaded) return WM_OK;
         std::vector<int> f(lst.size());
         for (size_t q = 0; q < lst.size(); ++q) f[q] = facOf[lst[q]];
+        uploaded.clear();
         WM_TRY(copy_h2d(a.dact, lst.data(), lst.size() * sizeof(int), s));
-        return copy_h2d(a.dfac, f.data(), f.size() * sizeof(int), s);
+        WM_TRY(copy_h2d(a.dfac, f.data(), f.size() * sizeof(int), s));
+        uploaded = lst;
+        return WM_OK;
     };
     const dim3 blk(256);
     auto gridv = [&](int members) { return dim3((unsigned)((n + 255) / 256), (unsigned)(members * nr)); };
@@ -328,11 +333,11 @@ int gmres_ir(wm_ctx *c, const wm_solve_opts &o, const GmresArgs &a, const std::v
             hipLaunchKernelGGL(k_dots, dim3(nl, 1, nr), blk, 0, s, w, (int64_t)0, 1, w, sV, n, nr, a.dact, nw);
             hipLaunchKernelGGL(k_normalize, gridv(nl), blk, 0, s, w, w, nw, sV, n, nr, a.dact);
             WM_TRY(check_device_error(c, s));
-            const size_t hn = (size_t)nl * (k + 1) * nr;
-            std::vector<double2> a1(hn), a2(hn), an((size_t)nl * nr);
-            WM_TRY(copy_d2h(a1.data(), h1, hn * sizeof(double2), s));
-            WM_TRY(copy_d2h(a2.data(), h2, hn * sizeof(double2), s));
-            WM_TRY(copy_d2h(an.data(), nw, an.size() * sizeof(double2), s));
+            // h1, h2 and the norms in one copy (they lie in that order in a.red)
+            const size_t hs = (size_t)nl * (m + 1) * nr;
+            std::vector<double2> hh(2 * hs + (size_t)nl * nr);
+            WM_TRY(copy_d2h(hh.data(), h1, hh.size() * sizeof(double2), s));
+            const double2 *a1 = hh.data(), *a2 = hh.data() + hs, *an = hh.data() + 2 * hs;
             for (int p = 0; p < nl; ++p) {
                 const int q = inner[p];
                 for (int r = 0; r < nr; ++r) {

@@ -759,17 +759,36 @@ int solve_items(wm_ctx *c, const wm_solve_opts &o, WorkItem *items, int count, h
         }
         ~AuxJoin() { (void)join(); }
     } aux_join{s, overlap ? c->aux : nullptr, c->aux_ev[1]};
-    for (int b0 = 0; b0 < count;) {
+    if (overlap) {
+        // the anchors on the launch stream, the others on the second one, each
+        // side in launches of up to four frequencies of one structure (the
+        // matrices need not be adjacent); every item's structure has ld = n
+        for (int side = 0; side < 2; ++side) {
+            std::vector<char> done(count, 0);
+            for (int b0 = 0; b0 < count; ++b0) {
+                if (done[b0] || is_own(b0) != (side == 0)) continue;
+                std::vector<int> slots;
+                std::vector<double> ka;
+                for (int b = b0; b < count; ++b)
+                    if (!done[b] && is_own(b) == (side == 0) && items[b].g == items[b0].g) {
+                        done[b] = 1;
+                        slots.push_back(b);
+                        ka.push_back(wavenumber_mhz(items[b].freq));
+                    }
+                WM_TRY(fill_launch(c, items[b0].g, (int)slots.size(), ka.data(), Z, sZ, nullptr, 0,
+                                   side ? c->aux : s, side ? &c->ws_fill2 : nullptr, slots.data()));
+            }
+        }
+    }
+    for (int b0 = 0; b0 < count && !overlap;) {
         int b1 = b0 + 1;
-        while (b1 < count && items[b1].g == items[b0].g && (!overlap || is_own(b1) == is_own(b0))) ++b1;
+        while (b1 < count && items[b1].g == items[b0].g) ++b1;
         const wm_geom *g = items[b0].g;
         std::vector<double> ka(b1 - b0);
         for (int b = b0; b < b1; ++b) ka[b - b0] = wavenumber_mhz(items[b].freq);
         if (g->ld == n) {
-            const bool side = overlap && !is_own(b0);
             WM_TRY(fill_launch(c, g, b1 - b0, ka.data(), Z + (size_t)b0 * sZ, sZ,
-                               mixed ? (void *)(Z32 + (size_t)b0 * sZ) : nullptr, sZ, side ? c->aux : s,
-                               side ? &c->ws_fill2 : nullptr));
+                               mixed ? (void *)(Z32 + (size_t)b0 * sZ) : nullptr, sZ, s));
         } else {
             const int lg = g->ld;
             const int64_t sT = (int64_t)lg * lg;

@@ -53,6 +53,7 @@ struct FreqArgs {
     double ka[kMaxF];
     double ka2[kMaxF];
     double q[kMaxF];  // eta0 / (4 pi k)
+    int64_t zo[kMaxF];  // offset of each frequency's matrix from Z (elements)
 };
 
 // Complete elliptic K from the Abramowitz & Stegun 17.3.34 polynomial in
@@ -242,7 +243,7 @@ __global__ __launch_bounds__(256) void k_fill(const double *__restrict__ obs, co
         for (int f = 0; f < F; ++f) {
             double2 z = make_double2(-fa.q[f] * acc[f].y, fa.q[f] * acc[f].x);
             if (!inside) z = make_double2(m == n ? 1.0 : 0.0, 0.0);
-            Z[f * strideZ + idx] = z;
+            Z[fa.zo[f] + idx] = z;
             if (kW32) Z32[f * strideZ32 + idx] = make_float2((float)z.x, (float)z.y);
         }
     }
@@ -272,9 +273,9 @@ void launch_fill_f(dim3 grid, hipStream_t s, const wm_geom *g, const double2 *se
 }  // namespace
 
 int fill_launch(wm_ctx *ctx, const wm_geom *g, int nf, const double *ka, void *Z, int64_t strideZ,
-                void *Z32, int64_t strideZ32, hipStream_t s, DevBuf *self_ws)
+                void *Z32, int64_t strideZ32, hipStream_t s, DevBuf *self_ws, const int *slots)
 {
-    if (!g || nf <= 0 || !ka || !Z) {
+    if (!g || nf <= 0 || !ka || !Z || (slots && Z32)) {
         set_error("fill: bad arguments");
         return WM_ERR_ARG;
     }
@@ -308,10 +309,11 @@ int fill_launch(wm_ctx *ctx, const wm_geom *g, int nf, const double *ka, void *Z
             fa.ka[f] = k;
             fa.ka2[f] = k * k;
             fa.q[f] = (120.0 * kPi) / (4.0 * kPi * k);
+            fa.zo[f] = (int64_t)(slots ? slots[f0 + (f < F ? f : 0)] : f0 + f) * strideZ;
         }
         hipLaunchKernelGGL(k_self_terms, dim3((2 * ld + 255) / 256), dim3(256), 0, s, g->d_src, g->npls, ld, F,
                            fa, self);
-        double2 *Zf = (double2 *)Z + (size_t)f0 * strideZ;
+        double2 *Zf = (double2 *)Z;  // each frequency's matrix at fa.zo[f]
         float2 *Z32f = Z32 ? (float2 *)Z32 + (size_t)f0 * strideZ32 : nullptr;
         // algorithmic count per frequency: 220 N^2 FP64 operations (440 N^2 with
         // the image terms of the PEC ground), 16 N^2 bytes written

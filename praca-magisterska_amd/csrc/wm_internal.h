@@ -158,9 +158,11 @@ struct wm_geom {
 namespace wm {
 
 // Fill: nf frequencies, complex128 output (+ optional complex64 copy).
+// Frequency f's matrix goes to Z + (slots ? slots[f] : f) * strideZ (slots:
+// complex128 only); self_ws: the self-term workspace (default ctx->ws_fill).
 int fill_launch(wm_ctx *ctx, const wm_geom *g, int nf, const double *ka, void *Z, int64_t strideZ,
                 void *Z32, int64_t strideZ32, hipStream_t s,
-                DevBuf *self_ws = nullptr);
+                DevBuf *self_ws = nullptr, const int *slots = nullptr);
 
 // LU: batched blocked right-looking factorization with partial pivoting.
 // d_bidx (device, optional): member i of the launch works on matrix d_bidx[i]

@@ -163,3 +163,32 @@ def test_recycled_pec_loads_generators_plane_wave(wm, geometries, ctx, geo_dir):
         eI = (np.abs(Ir[i] - I64[i]).max(axis=1) / np.abs(I64[i]).max(axis=1)).max()
         print(float(f[i]), sr[i]["anchor_mhz"], sr[i]["iters"], ez, eI, bound[i])
         assert ez <= bound[i] and eI <= 10 * bound[i], (float(f[i]), ez, eI, bound[i])
+
+
+def test_recycled_reference_127_frequency_sweep(wm, geometries, ctx, geo_dir):
+    """The reference's own lightning sweep grid ("wersja referencyjna/freq1.txt":
+    127 frequencies, 1 kHz - 10 MHz in piecewise-uniform steps of 1 kHz, 10 kHz,
+    50 kHz and 100 kHz) on the lightning structure at N = 1024, recycled with the
+    benchmark's r = 2.5: far fewer anchors than frequencies (the quasi-static
+    decades share anchors), every Zin within max(1e-10, kappa_1 u64) of the FP64
+    sweep, and the anchors equal to it bit for bit."""
+    f = np.concatenate([np.arange(1, 11) * 1e-3, np.arange(2, 11) * 1e-2, 0.1 + np.arange(1, 19) * 0.05,
+                        1.0 + np.arange(1, 11) * 0.1, 2.0 + np.arange(1, 81) * 0.1]).astype(np.float32)
+    assert len(f) == 127
+    p = _light(geometries, geo_dir, 1024)
+    g = wm.Geometry.load(ctx, p)
+    t = wm.configwire(p, cwd=geo_dir)
+    gap = geometries.nearest_basis(t, (0, 0, 19.9))
+    z64, _, s64 = wm.solve_sweep(g, f, [gap], "f64", cond_est=True)
+    zr, _, sr = wm.solve_sweep(g, f, [gap], "f64", recycle=2.5)
+    ku = np.array([s["cond1"] for s in s64]) * U64
+    bound = np.maximum(1e-10, ku)
+    rel = np.abs(zr[:, 0] - z64[:, 0]) / np.abs(z64[:, 0])
+    anchors = [i for i, s in enumerate(sr) if s["anchor_mhz"] == float(f[i])]
+    print("anchors", [float(f[i]) for i in anchors], "max rel / bound", (rel / bound).max(),
+          "iterations", sum(s["iters"] for s in sr))
+    assert 2 <= len(anchors) <= 20
+    assert all(s["fell_back"] == 0 for s in sr)
+    assert (rel <= bound).all(), (f[rel > bound], rel[rel > bound], bound[rel > bound])
+    for i in anchors:
+        assert zr[i, 0] == z64[i, 0], (f[i], zr[i, 0], z64[i, 0])

@@ -16,7 +16,8 @@ with it together with a per-frequency condition estimate.
 
 Also reported: the mixed-precision sweeps without recycling (FP32 and
 split-BF16x3 factors + FP64 refinement), a band where every frequency refines
-in complex64, per-kernel times and the roofline of the dominant kernel (one
+in complex64, the reference's own 127-frequency grid (1 kHz - 10 MHz) recycled
+on the same structure, per-kernel times and the roofline of the dominant kernel (one
 extra profiled step: HIP events around every launch on its stream), the other
 configurations (C1 dipole, C2 planar grid, C4 3-D cage), a serving leg and the
 CPU oracle on bounded samples of the same workloads.
@@ -449,6 +450,15 @@ def main():
                           "zin_rel_err_vs_fp64_max": float((np.abs(zb_[:, 0] - z64b[:, 0]) / np.abs(z64b[:, 0])).max()),
                           **stats_summary(stb_)}
 
+    # the reference's own lightning grid ("wersja referencyjna/freq1.txt": 127
+    # frequencies, 1 kHz - 10 MHz, piecewise uniform), recycled like the value
+    fr = np.concatenate([np.arange(1, 11) * 1e-3, np.arange(2, 11) * 1e-2, 0.1 + np.arange(1, 19) * 0.05,
+                         1.0 + np.arange(1, 11) * 0.1, 2.0 + np.arange(1, 81) * 0.1]).astype(np.float32)
+    dtr, (_, _, str_) = timed(ctx, lambda: sweep(a.prec, a.recycle, fb=fr), 2, 1)
+    ref_grid = {"freqs_mhz": "127, 1 kHz - 10 MHz (the reference's freq1.txt grid)", "solves_per_s": len(fr) / dtr,
+                **stats_summary(str_)}
+    progress("reference grid")
+
     # one profiled step of the value sweep (events around every launch; no graphs)
     prof = profiled(ctx, lambda: sweep(a.prec, a.recycle))
     roof = roofline(prof)
@@ -519,6 +529,7 @@ def main():
         "roofline_fp64_path": roofline(prof64),
         "mixed_no_recycling": mixed,
         "refinable_band": band,
+        "reference_grid_127": ref_grid,
         "configs": configs,
         "serving": serving,
         "cpu_baseline": cpu,
